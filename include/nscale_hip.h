@@ -1,0 +1,153 @@
+/*
+ * nscale_hip.h -- C ABI of libnscale_hip.so, the MI355X (gfx950) implementation of nscale's
+ * nuclei-segmentation morphology chain.
+ *
+ * Conventions
+ *  - Image arguments are DEVICE pointers to row-major, contiguous planes of width w and height
+ *    h (BGR images: 3 interleaved bytes per pixel, OpenCV order).  Outputs are caller-allocated.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the legacy default stream).  Every
+ *    operator is stream-ordered; an operator that returns a host scalar (a count, a status)
+ *    synchronises that stream before returning.
+ *  - Scratch memory comes from HIP's stream-ordered pool (hipMallocAsync) on `stream`, so the
+ *    steady state performs no hipMalloc/hipFree.
+ *  - Return value: NSC_OK (0) or a negative NSC_ERR_* code; nothing throws across the ABI.
+ *    nsc_segment_nuclei returns the reference's chain status codes (>= 0) on success.
+ *  - Connectivity arguments are 4 or 8.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * reference's src/segment/).
+ */
+#ifndef NSCALE_HIP_H_
+#define NSCALE_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSC_OK 0
+#define NSC_ERR_ARG (-10)     /* bad argument (null pointer, size, connectivity) */
+#define NSC_ERR_HIP (-11)     /* a HIP runtime call failed */
+#define NSC_ERR_NOMEM (-12)   /* device allocation failed */
+#define NSC_ERR_LIMIT (-13)   /* an internal capacity was exceeded */
+
+/* chain status codes: HistologicalEntities.h:34-40 */
+#define NSC_SEG_SUCCESS 0
+#define NSC_SEG_BACKGROUND 1
+#define NSC_SEG_BACKGROUND_LIKELY 2
+#define NSC_SEG_NO_CANDIDATES_LEFT 3
+#define NSC_SEG_CONTINUE 4
+#define NSC_SEG_INVALID_IMAGE (-1)
+#define NSC_SEG_RUNTIME_FAILED (-2)
+
+/* ---- library ---------------------------------------------------------------------------- */
+const char* nsc_version(void);
+/* last error message of the calling thread (empty string if none) */
+const char* nsc_last_error(void);
+/* initialise the library on the current device (optional; done lazily otherwise) */
+int nsc_init(void);
+
+/* ---- synthetic inputs (HOST pointers; deterministic in (seed, w, h)) --------------------- */
+int nsc_synth_mask(uint8_t* out, int w, int h, uint64_t seed);
+int nsc_synth_marker(const uint8_t* mask, uint8_t* marker, size_t n);
+int nsc_synth_he(uint8_t* bgr, int w, int h, uint64_t seed);
+
+/* ---- morphological reconstruction ------------------------------------------------------- */
+/* nscale::imreconstruct<uchar>(seeds, image, conn)  MorphologicOperations.h:34, .cpp:1730-1903 */
+int nsc_imreconstruct_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int w, int h, int conn,
+                         void* stream);
+/* nscale::imreconstruct<float>(seeds, image, conn)  MorphologicOperations.h:34, .cpp:1730-1903 */
+int nsc_imreconstruct_f32(const float* marker, const float* mask, float* out, int w, int h, int conn,
+                          void* stream);
+/* nscale::imreconstructBinary<uchar>(seeds, image, conn)  MorphologicOperations.h:61, .cpp:2768-2909 */
+int nsc_imreconstruct_binary_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int w, int h,
+                                int conn, void* stream);
+/* nscale::imfill<uchar>(image, seeds, binary=true, conn)  MorphologicOperations.h:65, .cpp:2913-2937 */
+int nsc_imfill_u8(const uint8_t* img, const uint8_t* seeds, uint8_t* out, int w, int h, int conn, void* stream);
+/* nscale::imfillHoles<uchar>(image, binary=true, conn)  MorphologicOperations.h:68, .cpp:2939-3006 */
+int nsc_imfill_holes_u8(const uint8_t* img, uint8_t* out, int w, int h, int conn, void* stream);
+/* nscale::bwselect<uchar>(binaryImage, seeds, conn)  MorphologicOperations.h:73, .cpp:3009-3031 */
+int nsc_bwselect_u8(const uint8_t* bw, const uint8_t* seeds, uint8_t* out, int w, int h, int conn, void* stream);
+/* nscale::imhmin<float>(image, h, conn)  MorphologicOperations.h:94, .cpp:3342-3364 */
+int nsc_imhmin_f32(const float* img, float* out, int w, int h, float hval, int conn, void* stream);
+
+/* ---- connected components --------------------------------------------------------------- */
+/* ConnComponents::label(img, w, h, label, bgval, conn)  ConnComponents.h:38, .cpp:52-86:
+   union of 4/8-neighbours with EQUAL nonzero values; label = smallest raster index of the
+   component; background (0) pixels get bgval */
+int nsc_cc_label_u8(const uint8_t* img, int32_t* labels, int w, int h, int32_t bgval, int conn, void* stream);
+/* nscale::bwlabel2(binaryImage, conn, relab)  MorphologicOperations.h:77, .cpp:3092-3117 (bg -1) */
+int nsc_bwlabel2_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn, void* stream);
+/* ConnComponents::relabel  ConnComponents.h:41, .cpp:89-118: roots numbered 1..N in raster order
+   of their first pixel, bgval -> 0, in place; *count = N */
+int nsc_cc_relabel(int32_t* labels, int w, int h, int32_t bgval, int* count, void* stream);
+/* nscale::bwareaopen2(img, labeled=false, flatten=true, min, max, conn, count)
+   MorphologicOperations.h:89, .cpp:3281-3314; ConnComponents::areaThresholdLabeled .cpp:205-239.
+   Keeps components with min <= area < max; out = 255/0; *count = number kept (may be NULL). */
+int nsc_bwareaopen2_u8(const uint8_t* img, uint8_t* out, int w, int h, int minSize, int maxSize, int conn,
+                       int* count, void* stream);
+/* Per-label bounding boxes of canonical labels 1..n (as nsc_cc_relabel produces): out5 is an
+   n x 5 SoA block [label | minx | maxx | miny | maxy] (the layout of ConnComponents::boundingBox,
+   ConnComponents.cpp:241-284, and of nscale::gpu::boundingBox2, cuda/ccl_uf.cu:993-1107), plus
+   the per-label pixel count in area (may be NULL).  Device pointers. */
+int nsc_bounding_boxes(const int32_t* canon_labels, int w, int h, int n, int32_t* out5, int32_t* area,
+                       void* stream);
+
+/* ---- morphology, distance, minima, watershed -------------------------------------------- */
+/* cv::erode / cv::dilate with a k x k structuring element (HOST array, k odd, k <= 31), anchor at the
+   centre, and `outside` as the value of pixels beyond the image (morphOpen pads with 255 for the
+   erosion and 0 for the dilation, MorphologicOperations.cpp:3848-3882) */
+int nsc_erode_u8(const uint8_t* img, uint8_t* out, int w, int h, const uint8_t* kernel, int k, int outside,
+                 void* stream);
+int nsc_dilate_u8(const uint8_t* img, uint8_t* out, int w, int h, const uint8_t* kernel, int k, int outside,
+                  void* stream);
+/* nscale::morphOpen<uchar>(image, kernel)  MorphologicOperations.h:114, .cpp:3848-3882 */
+int nsc_morph_open_u8(const uint8_t* img, uint8_t* out, int w, int h, const uint8_t* kernel, int k, void* stream);
+/* cv::distanceTransform(src, dst, CV_DIST_L2, CV_DIST_MASK_PRECISE) as called at
+   HistologicalEntities.cpp:359: exact Euclidean distance of each nonzero pixel to the nearest zero */
+int nsc_dist_transform_u8(const uint8_t* img, float* out, int w, int h, void* stream);
+/* nscale::localMaxima<float> / localMinima<float>  MorphologicOperations.h:108-111, .cpp:3763-3845 */
+int nsc_local_maxima_f32(const float* img, uint8_t* out, int w, int h, int conn, void* stream);
+int nsc_local_minima_f32(const float* img, uint8_t* out, int w, int h, int conn, void* stream);
+/* cv::watershed(image, markers) on a BGR image, markers in/out (as used by
+   MorphologicOperations.cpp:3750) */
+int nsc_cv_watershed(const uint8_t* bgr, int32_t* markers, int w, int h, void* stream);
+/* nscale::watershed2(origImage, image, conn)  MorphologicOperations.h:103, .cpp:3717-3760 */
+int nsc_watershed2(const uint8_t* bgr, const float* img, int32_t* out, int w, int h, int conn, void* stream);
+
+/* ---- the chain -------------------------------------------------------------------------- */
+/* parameters of HistologicalEntities::segmentNuclei (HistologicalEntities.h:52-60) */
+typedef struct nsc_seg_params {
+    uint8_t blue, green, red;
+    double T1, T2;
+    uint8_t G1;
+    int minSize, maxSize;
+    uint8_t G2;
+    int minSizePl, minSizeSeg, maxSizeSeg;
+    int fillHolesConnectivity, reconConnectivity, watershedConnectivity;
+} nsc_seg_params;
+/* the reference's default arguments (HistologicalEntities.h:53) */
+void nsc_seg_params_default(nsc_seg_params* p);
+
+/* Reusable device workspace for tiles up to max_w x max_h (allocation happens here only). */
+typedef struct nsc_workspace nsc_workspace;
+int nsc_workspace_create(int max_w, int max_h, nsc_workspace** ws);
+int nsc_workspace_destroy(nsc_workspace* ws);
+
+/* HistologicalEntities::segmentNuclei(img, output, compcount, bbox, ...)
+   HistologicalEntities.h:52-54, .cpp:473-598.
+   bgr: device BGR tile; labels: device int32 w*h, the reference's output (label = smallest raster
+   index of the nucleus, background -1).  If compcount is non-NULL it receives the number of
+   nuclei.  stage (may be NULL) is an array of 32 device pointers; a non-NULL entry i receives the
+   intermediate image with the reference's stage id i (IntermediateResultHandler ids,
+   UtilsCVImageIO.h:32-51), 24 = filled final mask, 25 = labels.  stage_ms (may be NULL, host,
+   32 doubles) receives per-stage GPU times in milliseconds.  Returns the chain status code. */
+int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p, int32_t* labels,
+                       int* compcount, nsc_workspace* ws, void** stage, double* stage_ms, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NSCALE_HIP_H_ */

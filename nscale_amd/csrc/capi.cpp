@@ -1,0 +1,65 @@
+// Library-level C entry points: errors, init, and the thin operator wrappers whose
+// implementations live in the kernel translation units.
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace nsc {
+
+static thread_local char g_err[512] = {0};
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+void clear_error() { g_err[0] = 0; }
+
+int ensure_init() {
+    static std::once_flag once;
+    static int status = NSC_OK;
+    std::call_once(once, [] {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) {
+            set_error("no HIP device: %s", hipGetErrorString(e));
+            status = NSC_ERR_HIP;
+            return;
+        }
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+    });
+    return status;
+}
+
+}  // namespace nsc
+
+using namespace nsc;
+
+extern "C" {
+
+const char* nsc_version(void) { return "nscale_amd 0.1 (gfx950)"; }
+const char* nsc_last_error(void) { return g_err; }
+int nsc_init(void) { return ensure_init(); }
+
+int nsc_imreconstruct_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int w, int h, int conn,
+                         void* stream) {
+    NSC_ARG(marker && mask && out && w > 0 && h > 0 && valid_conn(conn), "imreconstruct_u8");
+    NSC_CHECK(ensure_init());
+    return recon_impl<uint8_t>(marker, mask, out, w, h, conn, as_stream(stream));
+}
+
+int nsc_imreconstruct_f32(const float* marker, const float* mask, float* out, int w, int h, int conn,
+                          void* stream) {
+    NSC_ARG(marker && mask && out && w > 0 && h > 0 && valid_conn(conn), "imreconstruct_f32");
+    NSC_CHECK(ensure_init());
+    return recon_impl<float>(marker, mask, out, w, h, conn, as_stream(stream));
+}
+
+}  // extern "C"

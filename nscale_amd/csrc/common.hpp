@@ -1,0 +1,117 @@
+// Shared host/device helpers for libnscale_hip (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "nscale_hip.h"
+
+namespace nsc {
+
+// ---- error reporting --------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define NSC_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ::nsc::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),      \
+                             __FILE__, __LINE__);                                        \
+            return NSC_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+#define NSC_CHECK(call)                 \
+    do {                                \
+        int r_ = (call);                \
+        if (r_ != NSC_OK) return r_;    \
+    } while (0)
+
+#define NSC_ARG(cond, msg)                                   \
+    do {                                                     \
+        if (!(cond)) {                                       \
+            ::nsc::set_error("invalid argument: %s", msg);   \
+            return NSC_ERR_ARG;                              \
+        }                                                    \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int ensure_init();
+
+// ---- stream-ordered scratch memory -------------------------------------------------------
+// Allocations come from the device's default memory pool (hipMallocAsync), whose release
+// threshold is raised at init so steady-state calls do not go back to the driver.
+struct Scratch {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    Scratch() = default;
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+    ~Scratch() { release(); }
+    int alloc(size_t bytes, hipStream_t stream) {
+        release();
+        s = stream;
+        if (bytes == 0) bytes = 256;
+        hipError_t e = hipMallocAsync(&p, bytes, stream);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
+            return NSC_ERR_NOMEM;
+        }
+        return NSC_OK;
+    }
+    void release() {
+        if (p) (void)hipFreeAsync(p, s);
+        p = nullptr;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+inline int valid_conn(int c) { return c == 4 || c == 8; }
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ---- device helpers ----------------------------------------------------------------------
+// max/min with the selection rule of std::max / std::min (first argument on ties)
+template <typename R>
+__device__ __forceinline__ R smax(R a, R b) { return (a < b) ? b : a; }
+template <typename R>
+__device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// value from lane-1 (lane 0 receives `old`) and from lane+1 (lane 63 receives `old`):
+// full-wave DPP shifts (wave_shr:1 / wave_shl:1, GFX9 DPP controls 0x138 / 0x130)
+__device__ __forceinline__ int dpp_from_left(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dpp_from_right(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, 0x130, 0xF, 0xF, false);
+}
+template <typename R>
+__device__ __forceinline__ R from_left(R old, R v) {
+    return __builtin_bit_cast(R, dpp_from_left(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v)));
+}
+template <typename R>
+__device__ __forceinline__ R from_right(R old, R v) {
+    return __builtin_bit_cast(R, dpp_from_right(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v)));
+}
+
+// ---- internal entry points shared between translation units ------------------------------
+// connected-component labelling with min-index roots; pred: 0 = nonzero, 1 = equal u8 values,
+// 2 = equal float values restricted to fg != 0 (fg given separately)
+struct CCL;
+int ccl_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn, bool equal_values, int32_t bgval,
+           hipStream_t s);
+int ccl_mask(const uint8_t* fg, const float* vals, int32_t* labels, int w, int h, int conn, hipStream_t s);
+
+template <typename T>
+int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s);
+
+}  // namespace nsc

@@ -1,0 +1,266 @@
+// Grayscale morphological reconstruction by dilation (nscale::imreconstruct<T>,
+// MorphologicOperations.cpp:1730-1903) for uint8, int32 and float, 4- or 8-connectivity.
+//
+// The result is the unique reconstruction of min(marker, mask) under mask on the image padded
+// with a constant-0 frame (the reference pads both inputs with 0, :1736-1739), so any schedule
+// that reaches the fixpoint gives identical values.  Schedule used here:
+//
+//  * The image is cut into 62x62 owned tiles; one wave64 processes a 64x64 window = the tile plus
+//    a 1-pixel frozen halo.  Frozen pixels (halo, and anything outside the image, which holds the
+//    frame value 0) get mask = value, so the sweep arithmetic leaves them unchanged without
+//    special cases.
+//  * Inside the window the wave alternates direction sweeps until nothing changes:
+//      - column state (lane = column, 64 values in VGPRs): top-down and bottom-up sweeps,
+//        out = min(max(out, up-left, up, up-right), mask); the diagonal neighbours come from
+//        the adjacent lanes by full-wave DPP shifts;
+//      - row state (lane = row, after a transpose through LDS): left-right and right-left
+//        sweeps, out = min(max(out, left), mask).
+//    A fixpoint of these four sweeps satisfies every 8- (or 4-) neighbour constraint.
+//  * Pass 0 runs every tile once, reading marker and mask (halo = min(marker, mask)) and writing
+//    the output: 3 bytes/px for uint8, the algorithmic minimum.  A tile whose owned border
+//    changed marks the neighbours whose halo it feeds; later passes re-run only marked tiles
+//    (reading out + mask) until no tile is marked.
+#include "common.hpp"
+
+namespace nsc {
+namespace {
+
+constexpr int TW = 64;  // window width (= wave size)
+constexpr int TH = 64;  // window height
+constexpr int OW = TW - 2;
+constexpr int OH = TH - 2;
+
+template <typename T> struct RegOf { using type = T; };
+template <> struct RegOf<uint8_t> { using type = uint32_t; };
+
+enum { CH_TOP = 1, CH_BOT = 2, CH_LEFT = 4, CH_RIGHT = 8 };
+
+__device__ __forceinline__ void mark_tile(int t, unsigned next_pass, unsigned* stamp, int* next_list, int* next_count) {
+    unsigned old = atomicMax(&stamp[t], next_pass);
+    if (old < next_pass) {
+        int k = atomicAdd(next_count, 1);
+        next_list[k] = t;
+    }
+}
+
+template <typename T, int CONN>
+__global__ __launch_bounds__(64) void recon_tiles(const T* marker, const T* __restrict__ mask,
+                                                  T* out, int w, int h, int tiles_x, int tiles_y,
+                                                  const int* __restrict__ list, const int* __restrict__ list_count,
+                                                  int* __restrict__ next_list, int* __restrict__ next_count,
+                                                  unsigned* __restrict__ stamp, int pass) {
+    using R = typename RegOf<T>::type;
+    __shared__ T lm[TH][TW + 4];  // mask per window pixel (frozen pixels: mask = value)
+    __shared__ T tr[TH][TW + 1];  // transpose buffer (values fit the storage type)
+    __shared__ R init_l[TH], init_r[TH];
+    __shared__ unsigned chg_bits;
+
+    int tile;
+    if (pass == 0) {
+        tile = blockIdx.x;
+    } else {
+        if ((int)blockIdx.x >= *list_count) return;
+        tile = list[blockIdx.x];
+    }
+    const int tx = tile % tiles_x, ty = tile / tiles_x;
+    const int lane = threadIdx.x;
+    const int gx = tx * OW - 1 + lane;
+    const bool col_in = (gx >= 0 && gx < w);
+    const bool col_frozen = (lane == 0 || lane == TW - 1 || !col_in);
+    if (lane == 0) chg_bits = 0;
+
+    R v[TH];
+#pragma unroll
+    for (int y = 0; y < TH; ++y) {
+        const int gy = ty * OH - 1 + y;
+        const bool inb = col_in && gy >= 0 && gy < h;
+        R val = 0, mk = 0;
+        if (inb) {
+            const size_t i = (size_t)gy * w + gx;
+            mk = (R)mask[i];
+            const R a = (pass == 0) ? (R)marker[i] : (R)out[i];
+            val = smin(a, mk);
+        }
+        const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb;
+        v[y] = val;
+        lm[y][lane] = (T)(frozen ? val : mk);
+    }
+    if (lane == 1) {
+#pragma unroll
+        for (int y = 0; y < TH; ++y) init_l[y] = v[y];
+    }
+    if (lane == TW - 2) {
+#pragma unroll
+        for (int y = 0; y < TH; ++y) init_r[y] = v[y];
+    }
+    const R init_top = v[1], init_bot = v[TH - 2];
+    __syncthreads();
+
+    bool any_change = false;
+    for (int round = 0; round < 4096; ++round) {
+        bool changed = false;
+        // ---- column state: vertical sweeps ----
+        {
+            R pc = v[0];
+            R pl = from_left(pc, pc), pr = from_right(pc, pc);
+#pragma unroll
+            for (int y = 1; y < TH - 1; ++y) {
+                R up = pc;
+                if (CONN == 8) up = smax(up, smax(pl, pr));
+                const R mk = (R)lm[y][lane];
+                const R nv = smin(smax(v[y], up), mk);
+                changed |= (nv != v[y]);
+                v[y] = nv;
+                pc = nv;
+                if (CONN == 8) { pl = from_left(pc, pc); pr = from_right(pc, pc); }
+            }
+        }
+        {
+            R pc = v[TH - 1];
+            R pl = from_left(pc, pc), pr = from_right(pc, pc);
+#pragma unroll
+            for (int y = TH - 2; y >= 1; --y) {
+                R dn = pc;
+                if (CONN == 8) dn = smax(dn, smax(pl, pr));
+                const R mk = (R)lm[y][lane];
+                const R nv = smin(smax(v[y], dn), mk);
+                changed |= (nv != v[y]);
+                v[y] = nv;
+                pc = nv;
+                if (CONN == 8) { pl = from_left(pc, pc); pr = from_right(pc, pc); }
+            }
+        }
+        // ---- transpose to row state (lane = row) ----
+#pragma unroll
+        for (int y = 0; y < TH; ++y) tr[y][lane] = (T)v[y];
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < TW; ++x) v[x] = (R)tr[lane][x];
+        {
+            R prev = v[0];
+#pragma unroll
+            for (int x = 1; x < TW - 1; ++x) {
+                const R mk = (R)lm[lane][x];
+                const R nv = smin(smax(v[x], prev), mk);
+                changed |= (nv != v[x]);
+                v[x] = nv;
+                prev = nv;
+            }
+            prev = v[TW - 1];
+#pragma unroll
+            for (int x = TW - 2; x >= 1; --x) {
+                const R mk = (R)lm[lane][x];
+                const R nv = smin(smax(v[x], prev), mk);
+                changed |= (nv != v[x]);
+                v[x] = nv;
+                prev = nv;
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < TW; ++x) tr[lane][x] = (T)v[x];
+        __syncthreads();
+#pragma unroll
+        for (int y = 0; y < TH; ++y) v[y] = (R)tr[y][lane];
+        __syncthreads();
+        const bool wave_changed = __any(changed);
+        any_change |= wave_changed;
+        if (!wave_changed) break;
+    }
+
+    // pass 0 writes every owned pixel (the output starts as min(marker, mask)); later passes
+    // only rewrite tiles that changed
+    if (pass != 0 && !any_change) return;
+
+    // ---- write the owned pixels ----
+    if (!col_frozen) {
+#pragma unroll
+        for (int y = 1; y < TH - 1; ++y) {
+            const int gy = ty * OH - 1 + y;
+            if (gy < h) out[(size_t)gy * w + gx] = (T)v[y];
+        }
+    }
+    // ---- owned-border changes -> mark neighbours ----
+    unsigned bits = 0;
+    if (lane >= 1 && lane <= TW - 2) {
+        if (v[1] != init_top) bits |= CH_TOP;
+        if (v[TH - 2] != init_bot) bits |= CH_BOT;
+    }
+    if (lane == 1) {
+#pragma unroll
+        for (int y = 1; y < TH - 1; ++y)
+            if (v[y] != init_l[y]) bits |= CH_LEFT;
+    }
+    if (lane == TW - 2) {
+#pragma unroll
+        for (int y = 1; y < TH - 1; ++y)
+            if (v[y] != init_r[y]) bits |= CH_RIGHT;
+    }
+    if (bits) atomicOr(&chg_bits, bits);
+    __syncthreads();
+    if (lane == 0) {
+        const unsigned b = chg_bits;
+        const unsigned np = (unsigned)pass + 1;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) ||
+                            (dx == -1 && (b & CH_LEFT)) || (dx == 1 && (b & CH_RIGHT));
+                const int nx = tx + dx, ny = ty + dy;
+                if (need && nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y)
+                    mark_tile(ny * tiles_x + nx, np, stamp, next_list, next_count);
+            }
+    }
+}
+
+}  // namespace
+
+template <typename T>
+int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s) {
+    const int tiles_x = (int)cdiv(w, OW), tiles_y = (int)cdiv(h, OH);
+    const int nt = tiles_x * tiles_y;
+    const int max_pass = 4 * nt + 8;
+    Scratch sc;
+    const size_t bytes = sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2);
+    NSC_CHECK(sc.alloc(bytes, s));
+    unsigned* stamp = sc.as<unsigned>();
+    int* listA = reinterpret_cast<int*>(stamp + nt);
+    int* listB = listA + nt;
+    int* counts = listB + nt;
+    NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt, s));
+    NSC_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (max_pass + 2), s));
+    auto launch = [&](int pass, const int* lst, const int* cnt, int* nl, int* nc) {
+        if (conn == 8)
+            hipLaunchKernelGGL((recon_tiles<T, 8>), dim3(nt), dim3(64), 0, s, marker, mask, out, w, h, tiles_x,
+                               tiles_y, lst, cnt, nl, nc, stamp, pass);
+        else
+            hipLaunchKernelGGL((recon_tiles<T, 4>), dim3(nt), dim3(64), 0, s, marker, mask, out, w, h, tiles_x,
+                               tiles_y, lst, cnt, nl, nc, stamp, pass);
+    };
+    launch(0, nullptr, nullptr, listA, &counts[0]);
+    NSC_HIP(hipGetLastError());
+    int pass = 1;
+    for (;;) {
+        const int batch = 4;
+        for (int b = 0; b < batch && pass <= max_pass; ++b, ++pass) {
+            int* cur = (pass & 1) ? listA : listB;
+            int* nxt = (pass & 1) ? listB : listA;
+            launch(pass, cur, &counts[pass - 1], nxt, &counts[pass]);
+        }
+        NSC_HIP(hipGetLastError());
+        int c = 0;
+        NSC_HIP(hipMemcpyAsync(&c, &counts[pass - 1], sizeof(int), hipMemcpyDeviceToHost, s));
+        NSC_HIP(hipStreamSynchronize(s));
+        if (c == 0) break;
+        if (pass > max_pass) {
+            set_error("imreconstruct: no convergence after %d passes", max_pass);
+            return NSC_ERR_LIMIT;
+        }
+    }
+    return NSC_OK;
+}
+
+template int recon_impl<uint8_t>(const uint8_t*, const uint8_t*, uint8_t*, int, int, int, hipStream_t);
+template int recon_impl<int32_t>(const int32_t*, const int32_t*, int32_t*, int, int, int, hipStream_t);
+template int recon_impl<float>(const float*, const float*, float*, int, int, int, hipStream_t);
+
+}  // namespace nsc

@@ -1,0 +1,482 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) against checks that do not share its code:
+brute-force fixpoints, scipy.ndimage, closed forms, hand-derived cases, the reference's own
+ConnComponents.cpp (oracle/_ref) and the golden vectors under tests/golden/.
+
+Parity status per oracle function (also listed in DESIGN.md):
+  imreconstruct u8/f32/i32 ........ brute-force fixpoint on the zero-padded domain
+  imreconstructBinary, bwselect ... scipy.ndimage.label component selection
+  imfillHoles (binary) ............ scipy.ndimage.binary_fill_holes + MATLAB-doc vector
+  imfill (seeded) ................. MATLAB imfill doc example (comment in imfillTest.cpp:26-39)
+  ConnComponents label/relabel/areaThresholdLabeled/boundingBox ... reference ConnComponents.cpp
+  bwareaopen2 ..................... scipy label + bincount, sizePhantom.m closed-form areas
+  erode/dilate/morphOpen .......... scipy grey_erosion / grey_dilation
+  distanceTransform ............... scipy distance_transform_edt (<= 1e-5)
+  imhmin .......................... brute-force reconstruction
+  localMaxima/localMinima ......... brute-force regional-extremum definition
+  cv::watershed ................... hand-derived cases + structural invariants (tie order: partial)
+  border .......................... hand-derived cases
+  getBackground / getRBC .......... exhaustive numpy float32 predicate tables
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+from scipy import ndimage as ndi
+
+from tests._lib import (CROSS3, DISK19, o_bwareaopen2, o_bwselect, o_cc_label, o_cv_watershed, o_edt,
+                        o_imfill, o_imhmin, o_local_extrema, o_morph, o_recon, o_recon_binary, o_segment,
+                        o_u8_unary, o_watershed2, oracle, ptr, ref_cc, structure)
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ------------------------------------------------------------------ brute-force references
+def brute_recon(marker, mask, conn):
+    """Reconstruction by dilation on the domain padded with a fixed zero frame (the reference's
+    copyMakeBorder(..., 0)), iterated to its fixpoint."""
+    mk = np.pad(marker, 1).astype(np.float64)
+    ms = np.pad(mask, 1).astype(np.float64)
+    out = np.minimum(mk, ms)
+    fp = structure(conn)
+    while True:
+        nxt = np.minimum(ms, ndi.grey_dilation(out, footprint=fp, mode="constant", cval=-np.inf))
+        nxt[0, :] = nxt[-1, :] = nxt[:, 0] = nxt[:, -1] = 0.0
+        if np.array_equal(nxt, out):
+            break
+        out = nxt
+    return out[1:-1, 1:-1]
+
+
+def rand_u8(rng, h, w, levels=256):
+    return (rng.integers(0, levels, (h, w)) * (255 // max(levels - 1, 1))).astype(np.uint8)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("seed", range(6))
+def test_recon_u8_bruteforce(conn, seed):
+    rng = np.random.default_rng(seed)
+    h, w = (int(v) for v in rng.integers(5, 48, 2))
+    mask = ndi.grey_dilation(rand_u8(rng, h, w), size=(2, 2))
+    marker = np.minimum(mask, rand_u8(rng, h, w) // 3).astype(np.uint8)
+    marker[rng.random((h, w)) < 0.2] = 255  # marker above the mask is clamped
+    got = o_recon(marker, mask, conn)
+    assert np.array_equal(got, brute_recon(marker, mask, conn).astype(np.uint8))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("seed", range(4))
+def test_recon_f32_negative_values_and_border(conn, seed):
+    # negative values make the zero frame visible: it acts as a constant-0 neighbour
+    rng = np.random.default_rng(100 + seed)
+    h, w = (int(v) for v in rng.integers(4, 30, 2))
+    mask = rng.uniform(-4, 4, (h, w)).astype(np.float32)
+    marker = (mask - rng.uniform(0, 3, (h, w))).astype(np.float32)
+    got = o_recon(marker, mask, conn)
+    assert np.array_equal(got, brute_recon(marker, mask, conn).astype(np.float32))
+
+
+def test_recon_i32_bruteforce():
+    rng = np.random.default_rng(7)
+    mask = rng.integers(-50, 1000, (33, 41)).astype(np.int32)
+    marker = (mask - rng.integers(0, 200, mask.shape)).astype(np.int32)
+    for conn in (4, 8):
+        assert np.array_equal(o_recon(marker, mask, conn), brute_recon(marker, mask, conn).astype(np.int32))
+
+
+def test_recon_synthetic_mask_properties():
+    """Larger input from the config-0 generator: brute force at 128^2 plus invariants."""
+    from nscale_amd.synth import synth_mask
+    mask = synth_mask(128, 128, 11)
+    marker = np.where(mask > 10, mask - 10, 0).astype(np.uint8)
+    got = o_recon(marker, mask, 8)
+    assert np.array_equal(got, brute_recon(marker, mask, 8).astype(np.uint8))
+    assert np.all(got >= marker) and np.all(got <= mask)
+    assert np.array_equal(o_recon(got, mask, 8), got)  # idempotent
+
+
+# ------------------------------------------------------------------ binary reconstruction family
+def components_selected(passable, sources, conn):
+    lab, _ = ndi.label(passable, structure=structure(conn))
+    sel = np.unique(lab[sources & passable])
+    sel = sel[sel > 0]
+    return np.isin(lab, sel) & passable
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("seed", range(5))
+def test_recon_binary_scipy(conn, seed):
+    rng = np.random.default_rng(200 + seed)
+    h, w = (int(v) for v in rng.integers(5, 60, 2))
+    mask = np.where(rng.random((h, w)) < 0.55, rng.integers(1, 256, (h, w)), 0).astype(np.uint8)
+    marker = np.where(rng.random((h, w)) < 0.05, rng.integers(1, 256, (h, w)), 0).astype(np.uint8)
+    got = o_recon_binary(marker, mask, conn)
+    reach = components_selected(mask != 0, marker != 0, conn)
+    exp = np.where(marker != 0, marker, np.where(reach, 255, 0)).astype(np.uint8)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("seed", range(5))
+def test_fill_holes_scipy(conn, seed):
+    rng = np.random.default_rng(300 + seed)
+    h, w = (int(v) for v in rng.integers(3, 70, 2))
+    img = (ndi.binary_opening(rng.random((h, w)) < 0.6) * 255).astype(np.uint8) if seed % 2 else \
+        ((rng.random((h, w)) < 0.5) * 255).astype(np.uint8)
+    got = o_u8_unary("orc_imfill_holes_u8", img, conn)
+    exp = ndi.binary_fill_holes(img == 255, structure=structure(conn)) * 255
+    assert np.array_equal(got, exp.astype(np.uint8))
+
+
+def test_fill_holes_nonbinary_values():
+    # values other than 255 are passable for the background flood (mask = 255 - x != 0)
+    img = np.array([[255, 255, 255, 255, 255],
+                    [255, 7, 0, 7, 255],
+                    [255, 255, 255, 255, 255],
+                    [0, 9, 0, 0, 0]], np.uint8)
+    got = o_u8_unary("orc_imfill_holes_u8", img, 4)
+    exp = np.array([[255] * 5, [255] * 5, [255] * 5, [0] * 5], np.uint8)
+    assert np.array_equal(got, exp)
+
+
+def test_imfill_matlab_doc_example():
+    # BW1 / BW2 = imfill(BW1,[3 3],8) from the MATLAB imfill documentation quoted in
+    # the reference's test/src/imfillTest.cpp:26-39
+    bw1 = np.load(os.path.join(GOLD, "imfill_bw1.npz"))
+    seeds = np.zeros_like(bw1["bw1"])
+    seeds[2, 2] = 255
+    got = o_imfill(bw1["bw1"], seeds, 8)
+    assert np.array_equal(got, bw1["bw2"])
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("seed", range(4))
+def test_bwselect_scipy(conn, seed):
+    rng = np.random.default_rng(400 + seed)
+    h, w = (int(v) for v in rng.integers(5, 60, 2))
+    bw = ((rng.random((h, w)) < 0.5) * 255).astype(np.uint8)
+    seeds = ((rng.random((h, w)) < 0.03) * 255).astype(np.uint8)
+    got = o_bwselect(bw, seeds, conn)
+    exp = (components_selected(bw != 0, seeds != 0, conn) * 255).astype(np.uint8)
+    assert np.array_equal(got, exp)
+
+
+# ------------------------------------------------------------------ connected components
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("levels", [2, 4])
+def test_cc_label_vs_reference_code(conn, levels):
+    ref = ref_cc()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(conn * 10 + levels)
+    for _ in range(5):
+        h, w = (int(v) for v in rng.integers(3, 80, 2))
+        img = rng.integers(0, levels, (h, w)).astype(np.uint8) * (255 // (levels - 1))
+        exp = np.empty((h, w), np.int32)
+        ref.ref_cc_label(ptr(img), w, h, ptr(exp), -1, conn)
+        assert np.array_equal(o_cc_label(img, conn), exp)
+
+
+def test_cc_area_relabel_bbox_vs_reference_code():
+    ref = ref_cc()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(5)
+    o = oracle()
+    for it in range(6):
+        h, w = (int(v) for v in rng.integers(4, 70, 2))
+        img = ((rng.random((h, w)) < 0.45) * 255).astype(np.uint8)
+        lab = o_cc_label(img, 8)
+        for lo, hi in ((1, 5), (3, 1000), (0, 2)):
+            a = np.empty_like(lab)
+            b = np.empty_like(lab)
+            na = o.orc_cc_area_threshold_labeled(ptr(lab), w, h, ptr(a), -1, lo, hi)
+            nb = ref.ref_cc_area_threshold_labeled(ptr(lab), w, h, ptr(b), -1, lo, hi)
+            assert na == nb and np.array_equal(a, b)
+        r1, r2 = lab.copy(), lab.copy()
+        assert o.orc_cc_relabel(w, h, ptr(r1), -1) == ref.ref_cc_relabel(w, h, ptr(r2), -1)
+        assert np.array_equal(r1, r2)
+        for bg in (0, -1):
+            cap = h * w
+            x1 = np.zeros(5 * cap, np.int32)
+            x2 = np.zeros(5 * cap, np.int32)
+            n1 = o.orc_cc_bounding_box_literal(w, h, ptr(lab), bg, ptr(x1), cap)
+            n2 = ref.ref_cc_bounding_box(w, h, ptr(lab), bg, ptr(x2), cap)
+            assert n1 == n2 and np.array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_cc_label_binary_scipy_canonical(conn):
+    rng = np.random.default_rng(conn)
+    img = ((rng.random((97, 83)) < 0.5) * 255).astype(np.uint8)
+    lab = o_cc_label(img, conn)
+    # label = smallest raster index of the component
+    slab, n = ndi.label(img != 0, structure=structure(conn))
+    flat = np.arange(img.size).reshape(img.shape)
+    mins = ndi.minimum(flat, slab, index=np.arange(1, n + 1)).astype(np.int64)
+    exp = np.where(slab > 0, mins[np.maximum(slab - 1, 0)], -1)
+    assert np.array_equal(lab, exp)
+    # relabel = raster order of first pixel (= scipy's numbering)
+    r = lab.copy()
+    assert oracle().orc_cc_relabel(83, 97, ptr(r), -1) == n
+    assert np.array_equal(r, slab)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_bwareaopen2_scipy(conn):
+    rng = np.random.default_rng(11 + conn)
+    img = ((rng.random((120, 90)) < 0.45) * 255).astype(np.uint8)
+    for lo, hi in ((3, 20), (1, 2), (11, 1000), (30, 2 ** 31 - 1)):
+        got, n = o_bwareaopen2(img, lo, hi, conn)
+        lab, k = ndi.label(img != 0, structure=structure(conn))
+        area = np.bincount(lab.ravel())
+        keep = (area >= lo) & (area < hi)
+        keep[0] = False
+        assert n == int(keep.sum())
+        assert np.array_equal(got, (keep[lab] * 255).astype(np.uint8))
+
+
+def test_bwareaopen2_size_phantom():
+    """sizePhantom.m: disks of radius 1023, 511, 255, ... along the diagonal of a 4096^2 image;
+    the area of each digital disk has a closed form."""
+    ph = np.load(os.path.join(GOLD, "size_phantom_1024.npz"))
+    img, radii = ph["img"], ph["radii"]
+    areas = [int(sum(2 * int(np.floor(np.sqrt(r * r - i * i))) + 1 for i in range(-r, r + 1))) for r in radii]
+    for k, r in enumerate(radii):
+        lo, hi = areas[k], areas[k] + 1
+        got, n = o_bwareaopen2(img, lo, hi, 8)
+        assert n == 1 and int((got > 0).sum()) == areas[k]
+
+
+# ------------------------------------------------------------------ morphology
+@pytest.mark.parametrize("kernel", ["disk19", "cross3", "rand7"])
+def test_erode_dilate_open_scipy(kernel):
+    rng = np.random.default_rng(21)
+    img = rng.integers(0, 256, (61, 77)).astype(np.uint8)
+    k = {"disk19": DISK19, "cross3": CROSS3}.get(kernel)
+    if k is None:
+        r = rng.random((7, 7)) < 0.5
+        k = (r | r[::-1, ::-1]).astype(np.uint8)  # symmetric
+        k[3, 3] = 1
+    fp = k.astype(bool)
+    for outside in (0, 255, 17):
+        e = ndi.grey_erosion(img, footprint=fp, mode="constant", cval=outside)
+        d = ndi.grey_dilation(img, footprint=fp, mode="constant", cval=outside)
+        assert np.array_equal(o_morph("erode", img, k, outside), e)
+        assert np.array_equal(o_morph("dilate", img, k, outside), d)
+    opened = ndi.grey_dilation(ndi.grey_erosion(img, footprint=fp, mode="constant", cval=255),
+                               footprint=fp, mode="constant", cval=0)
+    assert np.array_equal(o_morph("open", img, k), opened)
+
+
+# ------------------------------------------------------------------ distance transform
+@pytest.mark.parametrize("seed", range(4))
+def test_edt_scipy(seed):
+    rng = np.random.default_rng(500 + seed)
+    h, w = (int(v) for v in rng.integers(8, 120, 2))
+    img = (ndi.binary_dilation(rng.random((h, w)) < 0.02, iterations=int(rng.integers(1, 8))) == 0)
+    img = (img * 255).astype(np.uint8)
+    img.flat[rng.integers(0, img.size)] = 0  # at least one zero
+    got = o_edt(img)
+    exp = ndi.distance_transform_edt(img != 0).astype(np.float32)
+    assert np.max(np.abs(got - exp)) <= 1e-5
+    # the values are sqrtf of exact integer squared distances
+    d2 = np.rint(exp.astype(np.float64) ** 2)
+    assert np.array_equal(got, np.sqrt(d2.astype(np.float32)))
+
+
+def test_edt_chain_like_blobs():
+    from nscale_amd.synth import synth_he
+    bgr = synth_he(256, 256, 3)
+    img = ((bgr[:, :, 2] < 150) * 255).astype(np.uint8)
+    assert np.max(np.abs(o_edt(img) - ndi.distance_transform_edt(img != 0).astype(np.float32))) <= 1e-5
+
+
+# ------------------------------------------------------------------ h-minima, regional extrema
+@pytest.mark.parametrize("conn", [4, 8])
+def test_imhmin_bruteforce(conn):
+    rng = np.random.default_rng(31 + conn)
+    img = -ndi.distance_transform_edt(ndi.binary_dilation(rng.random((40, 50)) < 0.04, iterations=4)).astype(np.float32)
+    img[img == 0] = -np.finfo(np.float32).max
+    mask = -img
+    marker = (mask - np.float32(1.0)).astype(np.float32)
+    exp = -brute_recon(marker, mask, conn).astype(np.float32)
+    assert np.array_equal(o_imhmin(img, 1.0, conn), exp)
+
+
+def regional_maxima_bruteforce(img, conn):
+    """candidate (recon(img-1, img) == img-1) AND plateau with no strictly higher conn-neighbour."""
+    marker = (img - np.float32(1)).astype(np.float32)
+    rec = brute_recon(marker, img, conn).astype(np.float32)
+    cand = ~(marker < rec)
+    st = structure(conn)
+    res = np.zeros(img.shape, bool)
+    padded = np.pad(img.astype(np.float64), 1, constant_values=-np.inf)
+    h, w = img.shape
+    higher = np.zeros(img.shape, bool)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if (dy, dx) == (0, 0) or not st[dy + 1, dx + 1]:
+                continue
+            higher |= padded[1 + dy:1 + dy + h, 1 + dx:1 + dx + w] > img
+    for v in np.unique(img):
+        lab, n = ndi.label((img == v) & cand, structure=st)
+        if n == 0:
+            continue
+        bad = np.zeros(n + 1, bool)
+        np.logical_or.at(bad, lab[higher & (lab > 0)], True)
+        res |= (lab > 0) & ~bad[lab]
+    return res
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("seed", range(4))
+def test_local_maxima_bruteforce(conn, seed):
+    rng = np.random.default_rng(600 + seed)
+    h, w = (int(v) for v in rng.integers(5, 40, 2))
+    img = (rng.integers(0, 6, (h, w)) * np.float32(0.75)).astype(np.float32)
+    if seed % 2:
+        img = ndi.grey_dilation(img, size=(2, 3)).astype(np.float32)
+    got = o_local_extrema("maxima", img, conn)
+    assert np.array_equal(got > 0, regional_maxima_bruteforce(img, conn))
+    gotmin = o_local_extrema("minima", -img, conn)
+    assert np.array_equal(gotmin, got)
+
+
+# ------------------------------------------------------------------ watershed
+def flat_bgr(h, w, v=100):
+    return np.full((h, w, 3), v, np.uint8)
+
+
+def test_cv_watershed_hand_strip():
+    # 7x3 image, interior row 1: seeds 1 at x=1 and 2 at x=5 on a flat colour.  Initial queue (level
+    # 0, raster order): x=2, x=4.  Pop x=2 -> 1, pushes x=3.  Pop x=4 -> 2 (x=3 already queued).
+    # Pop x=3 sees 1 and 2 -> -1.  Frame is -1.
+    m = np.zeros((3, 7), np.int32)
+    m[1, 1], m[1, 5] = 1, 2
+    got = o_cv_watershed(flat_bgr(3, 7), m)
+    assert got[1].tolist() == [-1, 1, 1, -1, 2, 2, -1]
+    assert (got[0] == -1).all() and (got[2] == -1).all()
+
+
+def test_cv_watershed_hand_priority():
+    # 9x3, interior x=1..7, seed 1 at x=1, seed 2 at x=7; c_diff = max channel |difference|.
+    m = np.zeros((3, 9), np.int32)
+    m[1, 1], m[1, 7] = 1, 2
+
+    def run(values):
+        bgr = np.zeros((3, 9, 3), np.uint8)
+        bgr[:, :, :] = np.array(values, np.uint8)[None, :, None]
+        return o_cv_watershed(bgr, m)[1].tolist()
+
+    # colours 10 (x<=4) | 60 (x>=5).  Initial queue, level 0 in raster order: x=2 (from seed 1),
+    # x=6 (from seed 2).  FIFO: x2->1 queues x3; x6->2 queues x5; x3->1 queues x4; x5->2 (x4 already
+    # queued); x4 sees 1 and 2 -> dam.  The colour step plays no role (every queued diff is 0).
+    assert run([10] * 5 + [60] * 4) == [-1, 1, 1, 1, -1, 2, 2, 2, -1]
+    # colours 10 (x<=5) | 60 (x>=6): x6->2 queues x5 at level 50; the level-0 flood of seed 1 labels
+    # x3, x4; x5 (level 50) then sees 1 and 2 -> dam at x5.
+    assert run([10] * 6 + [60] * 3) == [-1, 1, 1, 1, 1, -1, 2, 2, -1]
+    # colour 60 only at seed 2: x6 is queued at level 50 in the initial phase; seed 1 floods
+    # x2..x5 at level 0 first; x6 sees 1 and 2 -> dam at x6.
+    assert run([10] * 7 + [60, 10]) == [-1, 1, 1, 1, 1, 1, -1, 2, -1]
+
+
+def test_cv_watershed_negative_markers_reset_and_unreached():
+    m = np.full((5, 6), -1, np.int32)  # all unknown, no seeds: nothing is flooded
+    got = o_cv_watershed(flat_bgr(5, 6), m)
+    assert (got[1:-1, 1:-1] == 0).all()
+    assert (got[0] == -1).all() and (got[-1] == -1).all() and (got[:, 0] == -1).all() and (got[:, -1] == -1).all()
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_cv_watershed_invariants(seed):
+    rng = np.random.default_rng(700 + seed)
+    h, w = (int(v) for v in rng.integers(6, 50, 2))
+    bgr = rng.integers(0, 256, (h, w, 3)).astype(np.uint8)
+    bgr = ndi.uniform_filter(bgr, size=(3, 3, 1)).astype(np.uint8)
+    m = np.where(rng.random((h, w)) < 0.03, rng.integers(1, 6, (h, w)), 0).astype(np.int32)
+    m[rng.random((h, w)) < 0.05] = -1
+    got = o_cv_watershed(bgr, m)
+    inner = np.zeros((h, w), bool)
+    inner[1:-1, 1:-1] = True
+    assert (got[~inner] == -1).all()
+    seeds = inner & (m > 0)
+    assert np.array_equal(got[seeds], m[seeds])
+    pad = np.pad(got, 1, constant_values=-1)
+    nb = [pad[:-2, 1:-1], pad[2:, 1:-1], pad[1:-1, :-2], pad[1:-1, 2:]]
+    for y, x in zip(*np.nonzero(inner & ~seeds)):
+        v = got[y, x]
+        labs = {int(n[y, x]) for n in nb if n[y, x] > 0}
+        if v == -1:
+            assert len(labs) >= 2
+        elif v > 0:
+            assert v in labs
+        else:  # never reached: no labelled, non-dam neighbour
+            assert v == 0 and not labs
+
+
+def test_border_hand_cases():
+    # padded 4x5 watershed output; interior columns 1..3 of rows 1..3 are rewritten
+    img = np.array([[5, 5, 5, 5, 5],
+                    [5, -1, -1, 5, 5],
+                    [-1, 3, 5, -1, 5],
+                    [5, 5, 5, 5, 5]], np.int32)
+    out = np.zeros_like(img)
+    oracle().orc_border_i32(ptr(img), ptr(out), 5, 4, -1)
+    # (2,1): up=-1, p=3, upright=-1 -> rule needs upleft!=-1 & left==-1: upleft=5, left=-1 -> -1
+    # (2,2): up=-1, p=5, upleft=-1 ; upright=5 & right=-1 -> -1
+    # (1,1),(1,2): p itself -1 -> unchanged
+    assert out[2, 1:4].tolist() == [-1, -1, -1]
+    assert out[1, 1:4].tolist() == [-1, -1, 5]
+    assert out[3, 1:4].tolist() == [5, 5, 5]
+
+
+# ------------------------------------------------------------------ pixel predicates
+def test_rbc_predicate_exhaustive():
+    """rbc = (R/(G+eps) > T1) & (R/(B+eps) > 1) on isolated pixels (bwselect keeps every bw1 pixel),
+    against numpy float32 IEEE arithmetic over all 256x256 (R, G) pairs."""
+    eps = np.float32(np.finfo(np.float32).eps)
+    R, G = np.meshgrid(np.arange(256), np.arange(256), indexing="ij")
+    for B in (0, 1, 2, 150, 255):
+        img = np.zeros((512, 512, 3), np.uint8)
+        img[0::2, 0::2, 2] = R
+        img[0::2, 0::2, 1] = G
+        img[0::2, 0::2, 0] = B
+        got = o_u8_unary("orc_get_rbc", img, 5.0, 4.0)[0::2, 0::2]
+        r = R.astype(np.float32)
+        exp = (r / (G.astype(np.float32) + eps) > np.float32(5.0)) & (r / (np.float32(B) + eps) > np.float32(1.0))
+        assert np.array_equal(got > 0, exp)
+
+
+def test_background_predicate_exhaustive_strict():
+    v = np.arange(256, dtype=np.uint8)
+    img = np.stack(list(np.meshgrid(v, v, indexing="ij")) + [np.full((256, 256), 221, np.uint8)], -1)
+    img = np.ascontiguousarray(img)
+    got = o_u8_unary("orc_get_background", img, 220, 220, 220)
+    exp = (img[..., 0] > 220) & (img[..., 1] > 220)
+    assert np.array_equal(got > 0, exp)
+
+
+# ------------------------------------------------------------------ chain
+def test_chain_status_codes():
+    bg = np.full((64, 64, 3), 240, np.uint8)
+    assert o_segment(bg)[0] == 1  # BACKGROUND (ratio >= 0.99)
+    bg2 = bg.copy()
+    bg2[:3] = 100  # ratio 61/64 = 0.953
+    assert o_segment(bg2)[0] == 2  # BACKGROUND_LIKELY (0.9 <= ratio < 0.99)
+    rng = np.random.default_rng(0)
+    noise = rng.integers(150, 200, (64, 64, 3)).astype(np.uint8)
+    assert o_segment(noise)[0] == 3  # NO_CANDIDATES_LEFT
+
+
+def test_chain_synthetic_structure():
+    from nscale_amd.synth import synth_he
+    bgr = synth_he(384, 320, 5)
+    st, lab, s = o_segment(bgr, stages=(15, 20, 24))
+    assert st == 0
+    assert np.array_equal(lab >= 0, s[24] > 0)
+    lab8, n = ndi.label(s[24] > 0, structure=np.ones((3, 3)))
+    assert n == len(np.unique(lab[lab >= 0])) and n > 10
+    assert ((s[20] >= 0) | (s[15] == 0)).sum() < lab.size  # some dams inside seg_big
+    st2, lab2, _ = o_segment(bgr)
+    assert np.array_equal(lab, lab2)  # deterministic

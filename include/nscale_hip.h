@@ -61,6 +61,9 @@ int nsc_imreconstruct_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* ou
 /* nscale::imreconstruct<float>(seeds, image, conn)  MorphologicOperations.h:34, .cpp:1730-1903 */
 int nsc_imreconstruct_f32(const float* marker, const float* mask, float* out, int w, int h, int conn,
                           void* stream);
+/* nscale::imreconstruct<int>(seeds, image, conn)  MorphologicOperations.h:34, .cpp:1730-1903 */
+int nsc_imreconstruct_i32(const int32_t* marker, const int32_t* mask, int32_t* out, int w, int h, int conn,
+                          void* stream);
 /* nscale::imreconstructBinary<uchar>(seeds, image, conn)  MorphologicOperations.h:61, .cpp:2768-2909 */
 int nsc_imreconstruct_binary_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int w, int h,
                                 int conn, void* stream);

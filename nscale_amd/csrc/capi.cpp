@@ -62,4 +62,11 @@ int nsc_imreconstruct_f32(const float* marker, const float* mask, float* out, in
     return recon_impl<float>(marker, mask, out, w, h, conn, as_stream(stream));
 }
 
+int nsc_imreconstruct_i32(const int32_t* marker, const int32_t* mask, int32_t* out, int w, int h, int conn,
+                          void* stream) {
+    NSC_ARG(marker && mask && out && w > 0 && h > 0 && valid_conn(conn), "imreconstruct_i32");
+    NSC_CHECK(ensure_init());
+    return recon_impl<int32_t>(marker, mask, out, w, h, conn, as_stream(stream));
+}
+
 }  // extern "C"

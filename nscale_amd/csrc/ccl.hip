@@ -1,0 +1,561 @@
+// Connected components and everything built on them:
+//   ConnComponents::label / bwlabel2 / relabel / areaThresholdLabeled / boundingBox
+//   (ConnComponents.cpp:24-284, MorphologicOperations.cpp:3092-3117, 3281-3314), and the binary
+//   reconstruction family imreconstructBinary / imfill / imfillHoles / bwselect
+//   (MorphologicOperations.cpp:2768-3031), which are component selections: a pixel of the
+//   passable set is reached iff its component contains a seed.
+//
+// Labelling is union-find with the smallest raster index as root (the reference's merge links
+// the larger root under the smaller, ConnComponents.cpp:34-43, so its labels are also the
+// smallest index of each component):
+//   1. ccl_local: 64x16 tiles in LDS, unions with atomicMin on local indices (local raster
+//      order = global raster order inside a tile), then each pixel stores the global index of
+//      its tile-local root;
+//   2. ccl_seams: pixels on tile seams unite with their cross-tile neighbours in global memory
+//      (atomicMin union, retry while the target stopped being a root);
+//   3. ccl_flatten: every pixel follows its chain to the root (with path compression).
+// Pixel classes: FG_NEQ  -> foreground = value != key; neighbours join when both foreground
+//                FG_EQ   -> foreground = value != 0; neighbours join when values are equal
+//                FG_FEQ  -> foreground = cls != 0; neighbours join when float values are equal
+#include <climits>
+
+#include "common.hpp"
+
+namespace nsc {
+namespace {
+
+constexpr int CT_W = 64, CT_H = 16, CT_N = CT_W * CT_H;
+enum { FG_NEQ = 0, FG_EQ = 1, FG_FEQ = 2 };
+
+struct Px {
+    const uint8_t* img;
+    const float* fv;
+    uint8_t key;
+};
+
+template <int MODE>
+__device__ __forceinline__ bool fg_of(const Px& P, size_t i) {
+    if (MODE == FG_NEQ) return P.img[i] != P.key;
+    return P.img[i] != 0;
+}
+template <int MODE>
+__device__ __forceinline__ bool joins(const Px& P, size_t i, size_t j) {
+    if (MODE == FG_NEQ) return true;  // both known foreground
+    if (MODE == FG_EQ) return P.img[i] == P.img[j];
+    return P.fv[i] == P.fv[j];
+}
+
+__device__ __forceinline__ int lfind(const int* L, int x) {
+    while (true) {
+        int p = L[x];
+        if (p == x) return x;
+        x = p;
+    }
+}
+
+__device__ void lunite(int* L, int a, int b) {
+    while (true) {
+        a = lfind(L, a);
+        b = lfind(L, b);
+        if (a == b) return;
+        if (a > b) { int t = a; a = b; b = t; }
+        int old = atomicMin(&L[b], a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+__device__ __forceinline__ int gfind(const int32_t* L, int x) {
+    while (true) {
+        int p = __builtin_nontemporal_load(&L[x]);
+        if (p == x) return x;
+        x = p;
+    }
+}
+
+__device__ void gunite(int32_t* L, int a, int b) {
+    while (true) {
+        a = gfind(L, a);
+        b = gfind(L, b);
+        if (a == b) return;
+        if (a > b) { int t = a; a = b; b = t; }
+        int old = atomicMin(&L[b], a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+template <int MODE, int CONN>
+__global__ __launch_bounds__(256) void ccl_local(Px P, int32_t* __restrict__ L, int w, int h, int tiles_x) {
+    __shared__ int ll[CT_N];
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int x0 = tx * CT_W, y0 = ty * CT_H;
+    for (int k = threadIdx.x; k < CT_N; k += blockDim.x) {
+        const int lx = k % CT_W, ly = k / CT_W;
+        const int gx = x0 + lx, gy = y0 + ly;
+        bool f = false;
+        if (gx < w && gy < h) f = fg_of<MODE>(P, (size_t)gy * w + gx);
+        ll[k] = f ? k : -1;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < CT_N; k += blockDim.x) {
+        if (ll[k] < 0) continue;
+        const int lx = k % CT_W, ly = k / CT_W;
+        const size_t gi = (size_t)(y0 + ly) * w + (x0 + lx);
+        if (lx > 0 && ll[k - 1] >= 0 && joins<MODE>(P, gi, gi - 1)) lunite(ll, k, k - 1);
+        if (ly > 0 && ll[k - CT_W] >= 0 && joins<MODE>(P, gi, gi - w)) lunite(ll, k, k - CT_W);
+        if (CONN == 8 && ly > 0) {
+            if (lx > 0 && ll[k - CT_W - 1] >= 0 && joins<MODE>(P, gi, gi - w - 1)) lunite(ll, k, k - CT_W - 1);
+            if (lx < CT_W - 1 && ll[k - CT_W + 1] >= 0 && joins<MODE>(P, gi, gi - w + 1)) lunite(ll, k, k - CT_W + 1);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < CT_N; k += blockDim.x) {
+        const int lx = k % CT_W, ly = k / CT_W;
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (gx >= w || gy >= h) continue;
+        int r = ll[k];
+        int32_t out = -1;
+        if (r >= 0) {
+            r = lfind(ll, k);
+            out = (int32_t)((size_t)(y0 + r / CT_W) * w + (x0 + r % CT_W));
+        }
+        L[(size_t)gy * w + gx] = out;
+    }
+}
+
+// unions across tile seams: every pixel in a tile's first row, first column or last column
+// joins its "backward" neighbours (left, up, up-left, up-right) that lie in another tile
+template <int MODE, int CONN>
+__global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h, int tiles_x, int ntiles) {
+    const int per_tile = CT_W + 2 * CT_H;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= ntiles * per_tile) return;
+    const int t = gid / per_tile, e = gid % per_tile;
+    const int tx = t % tiles_x, ty = t / tiles_x;
+    int lx, ly;
+    if (e < CT_W) { lx = e; ly = 0; }
+    else if (e < CT_W + CT_H) { lx = 0; ly = e - CT_W; }
+    else { lx = CT_W - 1; ly = e - CT_W - CT_H; }
+    const int x = tx * CT_W + lx, y = ty * CT_H + ly;
+    if (x >= w || y >= h) return;
+    const size_t i = (size_t)y * w + x;
+    if (L[i] < 0) return;
+    auto cross = [&](int qx, int qy) { return (qx / CT_W != tx) || (qy / CT_H != ty); };
+    auto try_join = [&](int qx, int qy) {
+        if (qx < 0 || qy < 0 || qx >= w || !cross(qx, qy)) return;
+        const size_t j = (size_t)qy * w + qx;
+        if (L[j] < 0 || !joins<MODE>(P, i, j)) return;
+        gunite(L, (int)i, (int)j);
+    };
+    try_join(x - 1, y);
+    try_join(x, y - 1);
+    if (CONN == 8) {
+        try_join(x - 1, y - 1);
+        try_join(x + 1, y - 1);
+    }
+}
+
+__global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        int p = L[i];
+        if (p < 0) {
+            if (bgval != -1) L[i] = bgval;
+            continue;
+        }
+        int r = p;
+        while (true) {
+            int q = L[r];
+            if (q == r) break;
+            r = q;
+        }
+        if (r != p) L[i] = r;
+    }
+}
+
+template <int MODE>
+int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipStream_t s) {
+    const int tiles_x = (int)cdiv(w, CT_W), tiles_y = (int)cdiv(h, CT_H);
+    const int nt = tiles_x * tiles_y;
+    const int nseam = nt * (CT_W + 2 * CT_H);
+    const size_t n = (size_t)w * h;
+    if (conn == 8) {
+        hipLaunchKernelGGL((ccl_local<MODE, 8>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x);
+        hipLaunchKernelGGL((ccl_seams<MODE, 8>), dim3(cdiv(nseam, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
+    } else {
+        hipLaunchKernelGGL((ccl_local<MODE, 4>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x);
+        hipLaunchKernelGGL((ccl_seams<MODE, 4>), dim3(cdiv(nseam, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
+    }
+    hipLaunchKernelGGL(ccl_flatten, dim3(std::min<unsigned>(cdiv(n, 256), 65535u)), dim3(256), 0, s, L, n, bgval);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// operators on top of the labels
+// ------------------------------------------------------------------------------------------
+__global__ void k_area(const int32_t* __restrict__ L, int32_t* __restrict__ area, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        int r = L[i];
+        if (r >= 0) atomicAdd(&area[r], 1);
+    }
+}
+
+// bwareaopen2 flatten output: 255 where lo <= area(root) < hi; count kept roots
+__global__ void k_area_select(const int32_t* __restrict__ L, const int32_t* __restrict__ area, uint8_t* __restrict__ out,
+                              size_t n, int lo, int hi, int* count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        int r = L[i];
+        uint8_t o = 0;
+        if (r >= 0) {
+            int a = area[r];
+            bool keep = a >= lo && a < hi;
+            o = keep ? 255 : 0;
+            if (keep && (size_t)r == i && count) atomicAdd(count, 1);
+        }
+        out[i] = o;
+    }
+}
+
+// flag[root] = 1 for every component that holds a pixel with src[i] != 0 (src may be null:
+// then the image border acts as the source)
+__global__ void k_flag_roots(const int32_t* __restrict__ L, const uint8_t* __restrict__ src, uint8_t* flag, int w, int h) {
+    const size_t n = (size_t)w * h;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        int r = L[i];
+        if (r < 0) continue;
+        bool s;
+        if (src) {
+            s = src[i] != 0;
+        } else {
+            int x = (int)(i % w), y = (int)(i / w);
+            s = (x == 0 || y == 0 || x == w - 1 || y == h - 1);
+        }
+        if (s) flag[r] = 1;
+    }
+}
+
+// imreconstructBinary output: marker where marker != 0, else 255 where the mask component holds a
+// source, else 0.  imfill additionally ORs with the image (mode 1) using mask = 255 - img.
+__global__ void k_binrecon_out(const int32_t* __restrict__ L, const uint8_t* __restrict__ flag,
+                               const uint8_t* __restrict__ marker, uint8_t* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint8_t m = marker[i];
+        int r = L[i];
+        out[i] = m ? m : ((r >= 0 && flag[r]) ? 255 : 0);
+    }
+}
+
+// imfill(img, seeds): img | rec, rec = (255-img) at seeds inside passable (img != 255), 255 at other
+// reached pixels, 0 elsewhere
+__global__ void k_imfill_out(const int32_t* __restrict__ L, const uint8_t* __restrict__ flag, const uint8_t* __restrict__ img,
+                             const uint8_t* __restrict__ seeds, uint8_t* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint8_t v = img[i];
+        uint8_t rec;
+        if (seeds[i]) rec = (uint8_t)(255 - v);
+        else {
+            int r = L[i];
+            rec = (r >= 0 && flag[r]) ? 255 : 0;
+        }
+        out[i] = v | rec;
+    }
+}
+
+// imfillHoles(binary): 255 unless the pixel is passable (!= 255) and its component touches the border
+__global__ void k_fillholes_out(const int32_t* __restrict__ L, const uint8_t* __restrict__ flag, uint8_t* __restrict__ out,
+                                size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        int r = L[i];
+        out[i] = (r >= 0 && flag[r]) ? 0 : 255;
+    }
+}
+
+// bwselect: bw where the component holds a seed pixel
+__global__ void k_bwselect_out(const int32_t* __restrict__ L, const uint8_t* __restrict__ flag, const uint8_t* __restrict__ bw,
+                               uint8_t* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        int r = L[i];
+        out[i] = (r >= 0 && flag[r]) ? bw[i] : 0;
+    }
+}
+
+__global__ void k_and_nonzero(const uint8_t* a, const uint8_t* b, uint8_t* out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (a[i] != 0 && b[i] != 0) ? 255 : 0;
+}
+
+// ---- canonical relabel: rank of each root in raster order ----
+constexpr int SCAN_B = 1024;
+__global__ void k_root_block_count(const int32_t* __restrict__ L, size_t n, int32_t* __restrict__ bsum) {
+    __shared__ int s[SCAN_B / 64];
+    size_t i = (size_t)blockIdx.x * SCAN_B + threadIdx.x;
+    int c = (i < n && L[i] == (int32_t)i) ? 1 : 0;
+    unsigned long long bal = __ballot(c);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) s[wid] = __popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < SCAN_B / 64; ++k) t += s[k];
+        bsum[blockIdx.x] = t;
+    }
+}
+__global__ void k_scan_small(int32_t* bsum, int nb, int* total) {
+    // single block exclusive scan of nb block sums
+    __shared__ int part[1024];
+    int per = (nb + 1023) / 1024;
+    int start = threadIdx.x * per, acc = 0;
+    for (int k = 0; k < per && start + k < nb; ++k) acc += bsum[start + k];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int k = 0; k < 1024; ++k) { int v = part[k]; part[k] = run; run += v; }
+        if (total) *total = run;
+    }
+    __syncthreads();
+    int run = part[threadIdx.x];
+    for (int k = 0; k < per && start + k < nb; ++k) { int v = bsum[start + k]; bsum[start + k] = run; run += v; }
+}
+// rank[i] (stored in place of area scratch) for roots: 1 + number of roots before i
+__global__ void k_root_rank(const int32_t* __restrict__ L, size_t n, const int32_t* __restrict__ boff, int32_t* __restrict__ rank) {
+    __shared__ int s[SCAN_B / 64];
+    size_t i = (size_t)blockIdx.x * SCAN_B + threadIdx.x;
+    int c = (i < n && L[i] == (int32_t)i) ? 1 : 0;
+    unsigned long long bal = __ballot(c);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) s[wid] = __popcll(bal);
+    __syncthreads();
+    int before = boff[blockIdx.x];
+    for (int k = 0; k < wid; ++k) before += s[k];
+    before += __popcll(bal & ((1ull << lane) - 1ull));
+    if (c) rank[i] = before + 1;
+}
+__global__ void k_apply_rank(int32_t* L, const int32_t* __restrict__ rank, size_t n, int32_t bgval) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        int r = L[i];
+        L[i] = (r == bgval || r < 0) ? 0 : rank[r];
+    }
+}
+
+__global__ void k_bbox_init(int32_t* out5, int32_t* area, int n) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    out5[k] = k + 1;
+    out5[n + k] = INT_MAX;
+    out5[2 * n + k] = -1;
+    out5[3 * n + k] = INT_MAX;
+    out5[4 * n + k] = -1;
+    if (area) area[k] = 0;
+}
+__global__ void k_bbox(const int32_t* __restrict__ L, int w, int h, int n, int32_t* out5, int32_t* area) {
+    size_t total = (size_t)w * h;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        int l = L[i];
+        if (l <= 0 || l > n) continue;
+        int k = l - 1, x = (int)(i % w), y = (int)(i / w);
+        atomicMin(&out5[n + k], x);
+        atomicMax(&out5[2 * n + k], x);
+        atomicMin(&out5[3 * n + k], y);
+        atomicMax(&out5[4 * n + k], y);
+        if (area) atomicAdd(&area[k], 1);
+    }
+}
+
+inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u)); }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+int ccl_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn, bool equal_values, int32_t bgval,
+           hipStream_t s) {
+    Px P{img, nullptr, 0};
+    if (equal_values) return ccl_run<FG_EQ>(P, labels, w, h, conn, bgval, s);
+    return ccl_run<FG_NEQ>(P, labels, w, h, conn, bgval, s);
+}
+
+int ccl_neq(const uint8_t* img, uint8_t key, int32_t* labels, int w, int h, int conn, hipStream_t s) {
+    Px P{img, nullptr, key};
+    return ccl_run<FG_NEQ>(P, labels, w, h, conn, -1, s);
+}
+
+int ccl_mask(const uint8_t* fg, const float* vals, int32_t* labels, int w, int h, int conn, hipStream_t s) {
+    Px P{fg, vals, 0};
+    return ccl_run<FG_FEQ>(P, labels, w, h, conn, -1, s);
+}
+
+int bwareaopen_impl(const uint8_t* img, uint8_t* out, int w, int h, int lo, int hi, int conn, int* dcount,
+                    int32_t* labels, int32_t* area, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    NSC_CHECK(ccl_u8(img, labels, w, h, conn, true, -1, s));
+    NSC_HIP(hipMemsetAsync(area, 0, n * sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_area, grid1(n), dim3(256), 0, s, labels, area, n);
+    if (dcount) NSC_HIP(hipMemsetAsync(dcount, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_area_select, grid1(n), dim3(256), 0, s, labels, area, out, n, lo, hi, dcount);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+// component selection: passable = (img != key), sources = src != 0 (or the border when src null)
+static int select_components(const uint8_t* img, uint8_t key, const uint8_t* src, int32_t* labels, uint8_t* flag,
+                             int w, int h, int conn, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    NSC_CHECK(ccl_neq(img, key, labels, w, h, conn, s));
+    NSC_HIP(hipMemsetAsync(flag, 0, n, s));
+    hipLaunchKernelGGL(k_flag_roots, grid1(n), dim3(256), 0, s, labels, src, flag, w, h);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+int fill_holes_impl(const uint8_t* img, uint8_t* out, int w, int h, int conn, int32_t* labels, uint8_t* flag,
+                    hipStream_t s) {
+    NSC_CHECK(select_components(img, 255, nullptr, labels, flag, w, h, conn, s));
+    hipLaunchKernelGGL(k_fillholes_out, grid1((size_t)w * h), dim3(256), 0, s, labels, flag, out, (size_t)w * h);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+int bwselect_impl(const uint8_t* bw, const uint8_t* seeds, uint8_t* out, int w, int h, int conn, int32_t* labels,
+                  uint8_t* flag, hipStream_t s) {
+    NSC_CHECK(select_components(bw, 0, seeds, labels, flag, w, h, conn, s));
+    hipLaunchKernelGGL(k_bwselect_out, grid1((size_t)w * h), dim3(256), 0, s, labels, flag, bw, out, (size_t)w * h);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int32_t* rank, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    const int nb = (int)cdiv(n, SCAN_B);
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(int32_t) * (nb + 1), s));
+    int32_t* bsum = sc.as<int32_t>();
+    hipLaunchKernelGGL(k_root_block_count, dim3(nb), dim3(SCAN_B), 0, s, labels, n, bsum);
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, bsum, nb, dcount);
+    hipLaunchKernelGGL(k_root_rank, dim3(nb), dim3(SCAN_B), 0, s, labels, n, bsum, rank);
+    hipLaunchKernelGGL(k_apply_rank, grid1(n), dim3(256), 0, s, labels, rank, n, bgval);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s) {
+    if (n <= 0) return NSC_OK;
+    hipLaunchKernelGGL(k_bbox_init, dim3(cdiv(n, 256)), dim3(256), 0, s, out5, area, n);
+    hipLaunchKernelGGL(k_bbox, grid1((size_t)w * h), dim3(256), 0, s, canon, w, h, n, out5, area);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+}  // namespace nsc
+
+using namespace nsc;
+
+extern "C" {
+
+int nsc_cc_label_u8(const uint8_t* img, int32_t* labels, int w, int h, int32_t bgval, int conn, void* stream) {
+    NSC_ARG(img && labels && w > 0 && h > 0 && valid_conn(conn), "cc_label_u8");
+    NSC_CHECK(ensure_init());
+    return ccl_u8(img, labels, w, h, conn, true, bgval, as_stream(stream));
+}
+
+int nsc_bwlabel2_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn, void* stream) {
+    return nsc_cc_label_u8(img, labels, w, h, -1, conn, stream);
+}
+
+int nsc_cc_relabel(int32_t* labels, int w, int h, int32_t bgval, int* count, void* stream) {
+    NSC_ARG(labels && w > 0 && h > 0, "cc_relabel");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(int32_t) * ((size_t)w * h + 1), s));
+    int32_t* rank = sc.as<int32_t>();
+    int* dcount = reinterpret_cast<int*>(rank + (size_t)w * h);
+    NSC_CHECK(relabel_impl(labels, w, h, bgval, dcount, rank, s));
+    if (count) {
+        NSC_HIP(hipMemcpyAsync(count, dcount, sizeof(int), hipMemcpyDeviceToHost, s));
+        NSC_HIP(hipStreamSynchronize(s));
+    }
+    return NSC_OK;
+}
+
+int nsc_bwareaopen2_u8(const uint8_t* img, uint8_t* out, int w, int h, int minSize, int maxSize, int conn, int* count,
+                       void* stream) {
+    NSC_ARG(img && out && w > 0 && h > 0 && valid_conn(conn), "bwareaopen2_u8");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    const size_t n = (size_t)w * h;
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(int32_t) * (2 * n + 1), s));
+    int32_t* labels = sc.as<int32_t>();
+    int32_t* area = labels + n;
+    int* dcount = reinterpret_cast<int*>(area + n);
+    NSC_CHECK(bwareaopen_impl(img, out, w, h, minSize, maxSize, conn, dcount, labels, area, s));
+    if (count) {
+        NSC_HIP(hipMemcpyAsync(count, dcount, sizeof(int), hipMemcpyDeviceToHost, s));
+        NSC_HIP(hipStreamSynchronize(s));
+    }
+    return NSC_OK;
+}
+
+int nsc_imreconstruct_binary_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int w, int h, int conn,
+                                void* stream) {
+    NSC_ARG(marker && mask && out && w > 0 && h > 0 && valid_conn(conn), "imreconstruct_binary_u8");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    const size_t n = (size_t)w * h;
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(int32_t) * n + 2 * n, s));
+    int32_t* labels = sc.as<int32_t>();
+    uint8_t* flag = reinterpret_cast<uint8_t*>(labels + n);
+    uint8_t* src = flag + n;
+    hipLaunchKernelGGL(k_and_nonzero, grid1(n), dim3(256), 0, s, marker, mask, src, n);
+    NSC_CHECK(select_components(mask, 0, src, labels, flag, w, h, conn, s));
+    hipLaunchKernelGGL(k_binrecon_out, grid1(n), dim3(256), 0, s, labels, flag, marker, out, n);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+int nsc_imfill_u8(const uint8_t* img, const uint8_t* seeds, uint8_t* out, int w, int h, int conn, void* stream) {
+    NSC_ARG(img && seeds && out && w > 0 && h > 0 && valid_conn(conn), "imfill_u8");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    const size_t n = (size_t)w * h;
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(int32_t) * n + n, s));
+    int32_t* labels = sc.as<int32_t>();
+    uint8_t* flag = reinterpret_cast<uint8_t*>(labels + n);
+    NSC_CHECK(select_components(img, 255, seeds, labels, flag, w, h, conn, s));
+    hipLaunchKernelGGL(k_imfill_out, grid1(n), dim3(256), 0, s, labels, flag, img, seeds, out, n);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+int nsc_imfill_holes_u8(const uint8_t* img, uint8_t* out, int w, int h, int conn, void* stream) {
+    NSC_ARG(img && out && w > 0 && h > 0 && valid_conn(conn), "imfill_holes_u8");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    const size_t n = (size_t)w * h;
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(int32_t) * n + n, s));
+    int32_t* labels = sc.as<int32_t>();
+    return fill_holes_impl(img, out, w, h, conn, labels, reinterpret_cast<uint8_t*>(labels + n), s);
+}
+
+int nsc_bwselect_u8(const uint8_t* bw, const uint8_t* seeds, uint8_t* out, int w, int h, int conn, void* stream) {
+    NSC_ARG(bw && seeds && out && w > 0 && h > 0 && valid_conn(conn), "bwselect_u8");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    const size_t n = (size_t)w * h;
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(int32_t) * n + n, s));
+    int32_t* labels = sc.as<int32_t>();
+    return bwselect_impl(bw, seeds, out, w, h, conn, labels, reinterpret_cast<uint8_t*>(labels + n), s);
+}
+
+int nsc_bounding_boxes(const int32_t* canon_labels, int w, int h, int n, int32_t* out5, int32_t* area, void* stream) {
+    NSC_ARG(canon_labels && out5 && w > 0 && h > 0 && n >= 0, "bounding_boxes");
+    NSC_CHECK(ensure_init());
+    return bbox_impl(canon_labels, w, h, n, out5, area, as_stream(stream));
+}
+
+}  // extern "C"

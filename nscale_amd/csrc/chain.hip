@@ -1,0 +1,348 @@
+// HistologicalEntities::segmentNuclei (HistologicalEntities.cpp:473-598, with
+// plFindNucleusCandidates :115-304 and plSeparateNuclei :308-449) on one tile, every
+// intermediate resident in a reusable device workspace.  Pixel predicates and masks are fused
+// into small elementwise kernels; the morphology, reconstruction, labelling, distance and
+// watershed steps call the operator implementations directly.
+#include <cfloat>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace nsc {
+
+int morph_impl(const uint8_t* in, uint8_t* out, int w, int h, const uint8_t* kernel, int k, int outside, bool erode,
+               hipStream_t s);
+int morph_open_impl(const uint8_t* in, uint8_t* out, uint8_t* tmp, int w, int h, const uint8_t* kernel, int k,
+                    hipStream_t s);
+int bwareaopen_impl(const uint8_t* img, uint8_t* out, int w, int h, int lo, int hi, int conn, int* dcount,
+                    int32_t* labels, int32_t* area, hipStream_t s);
+int fill_holes_impl(const uint8_t* img, uint8_t* out, int w, int h, int conn, int32_t* labels, uint8_t* flag,
+                    hipStream_t s);
+int bwselect_impl(const uint8_t* bw, const uint8_t* seeds, uint8_t* out, int w, int h, int conn, int32_t* labels,
+                  uint8_t* flag, hipStream_t s);
+int edt_impl(const uint8_t* img, float* out, int w, int h, hipStream_t s);
+int imhmin_impl(const float* img, float* out, int w, int h, float hval, int conn, float* s1, float* s2, hipStream_t s);
+int local_max_impl(const float* img, uint8_t* out, int w, int h, int conn, bool minima, void* scratch, hipStream_t s);
+size_t local_max_scratch(int w, int h);
+size_t ws2_scratch(int w, int h);
+int ws2_from_labels(const uint8_t* bgr, const int32_t* labels, int32_t* out, int w, int h, void* scratch, hipStream_t s);
+int ccl_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn, bool equal_values, int32_t bgval, hipStream_t s);
+int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int32_t* rank, hipStream_t s);
+
+namespace {
+
+const uint8_t kDisk19[361] = {  // HistologicalEntities.cpp:169-188 (MATLAB strel('disk',10))
+    0,0,0,0,1,1,1,1,1,1,1,1,1,1,1,0,0,0,0, 0,0,0,1,1,1,1,1,1,1,1,1,1,1,1,1,0,0,0,
+    0,0,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,0,0, 0,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,0,
+    1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1, 1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,
+    1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1, 1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,
+    1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1, 1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,
+    1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1, 1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,
+    1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1, 1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,
+    1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1, 0,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,0,
+    0,0,1,1,1,1,1,1,1,1,1,1,1,1,1,1,1,0,0, 0,0,0,1,1,1,1,1,1,1,1,1,1,1,1,1,0,0,0,
+    0,0,0,0,1,1,1,1,1,1,1,1,1,1,1,0,0,0,0};
+const uint8_t kCross3[9] = {0, 1, 0, 1, 1, 1, 0, 1, 0};  // getStructuringElement(MORPH_ELLIPSE, 3x3)
+
+inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u)); }
+
+// getBackground (HE.cpp:93-109) count, getRBC planes (HE.cpp:33-82), rc = 255 - R (HE.cpp:158)
+__global__ void k_s1_pixels(const uint8_t* __restrict__ bgr, size_t n, uint8_t blue, uint8_t green, uint8_t red, float t1,
+                            float t2, uint8_t* __restrict__ bgmask, uint8_t* __restrict__ bw1, uint8_t* __restrict__ bw2,
+                            uint8_t* __restrict__ r2b, uint8_t* __restrict__ rc, int* __restrict__ bg_count) {
+    int local = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint8_t B = bgr[3 * i], G = bgr[3 * i + 1], R = bgr[3 * i + 2];
+        const bool bg = (B > blue) && (G > green) && (R > red);
+        local += bg;
+        if (bgmask) bgmask[i] = bg ? 255 : 0;
+        const float bd = (float)B + FLT_EPSILON;
+        const float gd = (float)G + FLT_EPSILON;
+        const float rd = (float)R;
+        const float q = rd / gd;
+        bw1[i] = (q > t1) ? 255 : 0;
+        bw2[i] = (q > t2) ? 255 : 0;
+        r2b[i] = (rd / bd > 1.0f) ? 255 : 0;
+        rc[i] = (uint8_t)(255 - R);
+    }
+    // wave reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(bg_count, local);
+}
+
+// diffIm = sat(rc - rc_recon) (HE.cpp:211); thresholds at G1 / G2 (:227, :279)
+__global__ void k_diff_thr(const uint8_t* __restrict__ rc, const uint8_t* __restrict__ rec, size_t n, uint8_t g1,
+                           uint8_t g2, uint8_t* __restrict__ diff, uint8_t* __restrict__ t1, uint8_t* __restrict__ t2) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int d = (int)rc[i] - (int)rec[i];
+        const uint8_t v = d > 0 ? (uint8_t)d : 0;
+        if (diff) diff[i] = v;
+        t1[i] = v > g1 ? 255 : 0;
+        t2[i] = v > g2 ? 255 : 0;
+    }
+}
+
+// rbc = rbcsel & r2b; seg_norbc &= (rbc == 0)   (HE.cpp:76, :295)
+__global__ void k_norbc(uint8_t* __restrict__ seg, const uint8_t* __restrict__ rbcsel, const uint8_t* __restrict__ r2b,
+                        uint8_t* __restrict__ rbc_out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint8_t rbc = rbcsel[i] & r2b[i];
+        if (rbc_out) rbc_out[i] = rbc;
+        seg[i] = seg[i] & (rbc == 0 ? 255 : 0);
+    }
+}
+
+// dist -> distance (HE.cpp:366-374) -> imhmin operands mask = -distance, marker = mask - 1
+// (MO.cpp:3355-3356); nuclei = img masked by seg_big (HE.cpp:407)
+__global__ void k_dist_prep(const float* __restrict__ dist, const uint8_t* __restrict__ seg_big,
+                            const uint8_t* __restrict__ bgr, size_t n, float* __restrict__ distance,
+                            float* __restrict__ hmask, float* __restrict__ hmarker, uint8_t* __restrict__ nuclei) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const bool f = seg_big[i] != 0;
+        const float d = f ? -dist[i] : -FLT_MAX;
+        if (distance) distance[i] = d;
+        const float m = -d;
+        hmask[i] = m;
+        hmarker[i] = m - 1.0f;
+        nuclei[3 * i] = f ? bgr[3 * i] : 0;
+        nuclei[3 * i + 1] = f ? bgr[3 * i + 1] : 0;
+        nuclei[3 * i + 2] = f ? bgr[3 * i + 2] : 0;
+    }
+}
+
+__global__ void k_negate_f(float* a, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = -a[i];
+}
+
+// seg_nonoverlap = seg_big where watermask >= 0  (HE.cpp:423-424)
+__global__ void k_nonoverlap(const uint8_t* __restrict__ seg_big, const int32_t* __restrict__ wm, uint8_t* __restrict__ out,
+                             size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = wm[i] >= 0 ? seg_big[i] : 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+struct Workspace {
+    int max_w = 0, max_h = 0;
+    void* base = nullptr;
+    size_t bytes = 0;
+    // u8 planes
+    uint8_t *bgm, *bw1, *bw2, *r2b, *rc, *rc_open, *rc_recon, *diff, *t1, *t2, *fill1, *bw1_t, *rbcsel, *rbc, *seg_norbc,
+        *seg_nohole, *seg_open, *seg_big_t, *seg_big, *nonov, *seg, *fin, *tmp8, *nuclei;
+    // 32-bit planes
+    int32_t *L, *area, *wm;
+    float *dist, *distance, *hmask, *hmarker, *d2;
+    void* big;  // shared by local minima and the watershed (max of their scratch)
+    int* counts;  // [0] background, [1] compcount S1, [2] compcount final-area, [3] ncomp
+};
+
+static int ws_alloc(Workspace& W, int mw, int mh) {
+    const size_t n = (size_t)mw * mh;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t u8n = al(n), b3 = al(3 * n), i32 = al(4 * n);
+    const size_t big = al(std::max(local_max_scratch(mw, mh), ws2_scratch(mw, mh)));
+    W.bytes = 23 * u8n + b3 + 3 * i32 + 5 * i32 + big + 256;
+    NSC_HIP(hipMalloc(&W.base, W.bytes));
+    uint8_t* p = static_cast<uint8_t*>(W.base);
+    uint8_t** u8s[] = {&W.bgm, &W.bw1, &W.bw2, &W.r2b, &W.rc, &W.rc_open, &W.rc_recon, &W.diff, &W.t1, &W.t2,
+                       &W.fill1, &W.bw1_t, &W.rbcsel, &W.rbc, &W.seg_norbc, &W.seg_nohole, &W.seg_open, &W.seg_big_t,
+                       &W.seg_big, &W.nonov, &W.seg, &W.fin, &W.tmp8};
+    for (auto q : u8s) { *q = p; p += u8n; }
+    W.nuclei = p; p += b3;
+    W.L = reinterpret_cast<int32_t*>(p); p += i32;
+    W.area = reinterpret_cast<int32_t*>(p); p += i32;
+    W.wm = reinterpret_cast<int32_t*>(p); p += i32;
+    W.dist = reinterpret_cast<float*>(p); p += i32;
+    W.distance = reinterpret_cast<float*>(p); p += i32;
+    W.hmask = reinterpret_cast<float*>(p); p += i32;
+    W.hmarker = reinterpret_cast<float*>(p); p += i32;
+    W.d2 = reinterpret_cast<float*>(p); p += i32;
+    W.big = p; p += big;
+    W.counts = reinterpret_cast<int*>(p);
+    W.max_w = mw;
+    W.max_h = mh;
+    return NSC_OK;
+}
+
+}  // namespace nsc
+
+struct nsc_workspace : nsc::Workspace {};
+
+using namespace nsc;
+
+extern "C" {
+
+void nsc_seg_params_default(nsc_seg_params* p) {
+    p->blue = 220; p->green = 220; p->red = 220;
+    p->T1 = 5.0; p->T2 = 4.0;
+    p->G1 = 80;
+    p->minSize = 11; p->maxSize = 1000;
+    p->G2 = 45;
+    p->minSizePl = 30; p->minSizeSeg = 21; p->maxSizeSeg = 1000;
+    p->fillHolesConnectivity = 4; p->reconConnectivity = 8; p->watershedConnectivity = 8;
+}
+
+int nsc_workspace_create(int max_w, int max_h, nsc_workspace** ws) {
+    NSC_ARG(ws && max_w > 0 && max_h > 0, "workspace_create");
+    NSC_CHECK(ensure_init());
+    auto* W = new nsc_workspace();
+    int rc = ws_alloc(*W, max_w, max_h);
+    if (rc != NSC_OK) { delete W; return rc; }
+    *ws = W;
+    return NSC_OK;
+}
+
+int nsc_workspace_destroy(nsc_workspace* ws) {
+    if (!ws) return NSC_OK;
+    if (ws->base) (void)hipFree(ws->base);
+    delete ws;
+    return NSC_OK;
+}
+
+int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* prm, int32_t* labels, int* compcount,
+                       nsc_workspace* ws, void** stage, double* stage_ms, void* stream) {
+    if (!bgr || w <= 0 || h <= 0) return NSC_SEG_INVALID_IMAGE;
+    NSC_ARG(labels && prm && ws, "segment_nuclei");
+    NSC_ARG(w <= ws->max_w && h <= ws->max_h && (size_t)w * h <= (size_t)ws->max_w * ws->max_h, "tile larger than the workspace");
+    NSC_ARG(valid_conn(prm->fillHolesConnectivity) && valid_conn(prm->reconConnectivity) &&
+                valid_conn(prm->watershedConnectivity), "connectivity");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    Workspace& W = *ws;
+    const size_t n = (size_t)w * h;
+    const nsc_seg_params& p = *prm;
+
+    // ---- optional per-stage timing (HIP events on the stream) ----
+    std::vector<std::pair<int, hipEvent_t>> marks;
+    auto mark = [&](int id) {
+        if (!stage_ms) return;
+        hipEvent_t e;
+        if (hipEventCreate(&e) == hipSuccess) {
+            (void)hipEventRecord(e, s);
+            marks.push_back({id, e});
+        }
+    };
+    auto save = [&](int id, const void* src, size_t bytes) -> int {
+        if (stage && stage[id]) NSC_HIP(hipMemcpyAsync(stage[id], src, bytes, hipMemcpyDeviceToDevice, s));
+        return NSC_OK;
+    };
+    auto finish_timing = [&]() {
+        if (!stage_ms || marks.empty()) return;
+        (void)hipEventSynchronize(marks.back().second);
+        for (size_t k = 1; k < marks.size(); ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, marks[k - 1].second, marks[k].second);
+            stage_ms[marks[k].first] += ms;
+        }
+        for (auto& m : marks) (void)hipEventDestroy(m.second);
+    };
+    mark(0);
+    NSC_CHECK(save(0, bgr, 3 * n));
+
+    // ---- S1: plFindNucleusCandidates ----
+    NSC_HIP(hipMemsetAsync(W.counts, 0, 4 * sizeof(int), s));
+    hipLaunchKernelGGL(k_s1_pixels, grid1(n), dim3(256), 0, s, bgr, n, p.blue, p.green, p.red, (float)p.T1,
+                       (float)p.T2, (stage && stage[1]) ? W.bgm : nullptr, W.bw1, W.bw2, W.r2b, W.rc, &W.counts[0]);
+    NSC_HIP(hipGetLastError());
+    int bgArea = 0;
+    NSC_HIP(hipMemcpyAsync(&bgArea, &W.counts[0], sizeof(int), hipMemcpyDeviceToHost, s));
+    NSC_HIP(hipStreamSynchronize(s));
+    const float ratio = (float)bgArea / (float)(w * h);  // HE.cpp:125
+    mark(1);
+    if (ratio >= 0.99) { finish_timing(); return NSC_SEG_BACKGROUND; }
+    if (ratio >= 0.9) { finish_timing(); return NSC_SEG_BACKGROUND_LIKELY; }
+    NSC_CHECK(save(1, W.bgm, n));
+
+    // getRBC: bwselect(bw2, bw1, 8) & imR2B -- an empty bw1 selects nothing, which is the
+    // reference's zero branch (HE.cpp:74-80)
+    NSC_CHECK(bwselect_impl(W.bw2, W.bw1, W.rbcsel, w, h, 8, W.L, W.tmp8, s));
+    mark(2);
+    // rc_open = morphOpen(rc, disk19); rc_recon = imreconstruct(rc_open, rc, reconConnectivity)
+    NSC_CHECK(morph_open_impl(W.rc, W.rc_open, W.tmp8, w, h, kDisk19, 19, s));
+    NSC_CHECK(save(3, W.rc_open, n));
+    mark(3);
+    NSC_CHECK(recon_impl<uint8_t>(W.rc_open, W.rc, W.rc_recon, w, h, p.reconConnectivity, s));
+    NSC_CHECK(save(4, W.rc_recon, n));
+    hipLaunchKernelGGL(k_diff_thr, grid1(n), dim3(256), 0, s, W.rc, W.rc_recon, n, p.G1, p.G2, W.diff, W.t1, W.t2);
+    NSC_CHECK(save(5, W.diff, n));
+    NSC_CHECK(save(6, W.t1, n));
+    mark(4);
+    NSC_CHECK(fill_holes_impl(W.t1, W.fill1, w, h, p.fillHolesConnectivity, W.L, W.tmp8, s));
+    NSC_CHECK(save(7, W.fill1, n));
+    mark(7);
+    NSC_CHECK(bwareaopen_impl(W.fill1, W.bw1_t, w, h, p.minSize, p.maxSize, 8, &W.counts[1], W.L, W.area, s));
+    NSC_CHECK(save(8, W.bw1_t, n));
+    NSC_CHECK(save(9, W.t2, n));
+    mark(8);
+    NSC_CHECK(bwselect_impl(W.t2, W.bw1_t, W.seg_norbc, w, h, 8, W.L, W.tmp8, s));
+    NSC_CHECK(save(10, W.seg_norbc, n));
+    hipLaunchKernelGGL(k_norbc, grid1(n), dim3(256), 0, s, W.seg_norbc, W.rbcsel, W.r2b,
+                       (stage && stage[2]) ? W.rbc : nullptr, n);
+    NSC_CHECK(save(2, W.rbc, n));
+    NSC_CHECK(save(11, W.seg_norbc, n));
+    mark(10);
+
+    // ---- segmentNuclei body ----
+    NSC_CHECK(fill_holes_impl(W.seg_norbc, W.seg_nohole, w, h, 4, W.L, W.tmp8, s));  // HE.cpp:494
+    NSC_CHECK(save(12, W.seg_nohole, n));
+    mark(12);
+    NSC_CHECK(morph_open_impl(W.seg_nohole, W.seg_open, W.tmp8, w, h, kCross3, 3, s));  // HE.cpp:498-499
+    NSC_CHECK(save(13, W.seg_open, n));
+    mark(13);
+
+    // ---- A4: plSeparateNuclei ----
+    NSC_CHECK(bwareaopen_impl(W.seg_open, W.seg_big_t, w, h, p.minSizePl, INT_MAX, 8, nullptr, W.L, W.area, s));
+    NSC_CHECK(save(14, W.seg_big_t, n));
+    mark(14);
+    NSC_CHECK(morph_impl(W.seg_big_t, W.seg_big, w, h, kCross3, 3, 0, false, s));  // HE.cpp:330
+    NSC_CHECK(save(15, W.seg_big, n));
+    mark(15);
+    NSC_CHECK(edt_impl(W.seg_big, W.dist, w, h, s));  // HE.cpp:359
+    NSC_CHECK(save(16, W.dist, 4 * n));
+    hipLaunchKernelGGL(k_dist_prep, grid1(n), dim3(256), 0, s, W.dist, W.seg_big, bgr, n,
+                       (stage && stage[17]) ? W.distance : nullptr, W.hmask, W.hmarker, W.nuclei);
+    NSC_CHECK(save(17, W.distance, 4 * n));
+    mark(16);
+    NSC_CHECK(recon_impl<float>(W.hmarker, W.hmask, W.d2, w, h, 8, s));  // imhmin(distance, 1, 8) HE.cpp:384
+    hipLaunchKernelGGL(k_negate_f, grid1(n), dim3(256), 0, s, W.d2, n);
+    NSC_CHECK(save(18, W.d2, 4 * n));
+    NSC_CHECK(save(19, W.nuclei, 3 * n));
+    mark(18);
+    // watershed2(nuclei, distance2, watershedConnectivity)  HE.cpp:414
+    NSC_CHECK(local_max_impl(W.d2, W.tmp8, w, h, p.watershedConnectivity, true, W.big, s));
+    NSC_CHECK(ccl_u8(W.tmp8, W.L, w, h, p.watershedConnectivity, true, -1, s));
+    NSC_CHECK(ws2_from_labels(W.nuclei, W.L, W.wm, w, h, W.big, s));
+    NSC_CHECK(save(20, W.wm, 4 * n));
+    hipLaunchKernelGGL(k_nonoverlap, grid1(n), dim3(256), 0, s, W.seg_big, W.wm, W.nonov, n);
+    NSC_CHECK(save(21, W.nonov, n));
+    mark(20);
+
+    NSC_CHECK(bwareaopen_impl(W.nonov, W.seg, w, h, p.minSizeSeg, p.maxSizeSeg, 4, &W.counts[2], W.L, W.area, s));  // :548
+    NSC_CHECK(save(23, W.seg, n));
+    mark(23);
+    NSC_CHECK(fill_holes_impl(W.seg, W.fin, w, h, p.fillHolesConnectivity, W.L, W.tmp8, s));  // :573
+    NSC_CHECK(save(24, W.fin, n));
+    mark(24);
+    NSC_CHECK(ccl_u8(W.fin, labels, w, h, 8, true, -1, s));  // bwlabel2(final, 8) :585
+    NSC_CHECK(save(25, labels, 4 * n));
+    if (compcount) {  // number of nuclei = number of roots
+        NSC_HIP(hipMemcpyAsync(W.L, labels, 4 * n, hipMemcpyDeviceToDevice, s));
+        NSC_CHECK(relabel_impl(W.L, w, h, -1, &W.counts[3], W.area, s));
+    }
+    mark(25);
+    NSC_HIP(hipGetLastError());
+    int c[4] = {0, 0, 0, 0};
+    NSC_HIP(hipMemcpyAsync(c, W.counts, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    NSC_HIP(hipStreamSynchronize(s));
+    finish_timing();
+    if (c[1] == 0) return NSC_SEG_NO_CANDIDATES_LEFT;  // HE.cpp:272-274
+    if (c[2] == 0) return NSC_SEG_NO_CANDIDATES_LEFT;  // HE.cpp:554-556
+    if (compcount) *compcount = c[3];
+    return NSC_SEG_SUCCESS;
+}
+
+}  // extern "C"

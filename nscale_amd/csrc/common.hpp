@@ -73,6 +73,22 @@ struct Scratch {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Carves 256-byte-aligned sub-buffers out of one allocation; construct with base = nullptr to
+// measure the total size first.
+struct Carver {
+    uint8_t* base;
+    size_t off = 0;
+    explicit Carver(void* b) : base(static_cast<uint8_t*>(b)) {}
+    template <typename T>
+    T* take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+    size_t size() const { return off + 256; }
+};
+
 inline int valid_conn(int c) { return c == 4 || c == 8; }
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

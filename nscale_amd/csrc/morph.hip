@@ -1,0 +1,198 @@
+// Grey erosion / dilation with a k x k structuring element (cv::erode / cv::dilate as used by
+// nscale::morphOpen, MorphologicOperations.cpp:3848-3882, and HistologicalEntities.cpp:330),
+// anchor at the centre, pixels beyond the image taking a given constant.
+//
+// Each kernel row is a run of ones [a, b] (true for disks, crosses, rectangles); a row with holes
+// is split into several runs.  Per 64x64 output tile (256 threads):
+//   1. the (64+2r)^2 input window is staged in LDS (outside pixels = the given constant);
+//   2. for every distinct run, the horizontal min/max over [x+a, x+b] is computed for all window
+//      rows; runs are processed shortest first and a run that contains the previous one extends
+//      its result by the extra taps only (disk19: 19 taps per pixel for all 5 distinct runs);
+//   3. the output is the min/max over kernel rows of the run result of the matching window row.
+// Traffic: 1 byte in + 1 byte out per pixel (plus the halo re-read, r/32 of the tile).
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+namespace nsc {
+namespace {
+
+constexpr int MT = 64;       // output tile side
+constexpr int MAXK = 31;     // largest kernel side
+constexpr int MAXRUN = 64;   // runs in total (rows with holes contribute several)
+
+struct MorphPlan {
+    int k, r;
+    int nrun;                 // distinct runs
+    int ra[MAXRUN], rb[MAXRUN];
+    int base[MAXRUN];         // index of the run this one extends, -1 = from scratch
+    int nrow;                 // (kernel row, run) pairs
+    int row_i[MAXRUN], row_run[MAXRUN];
+};
+
+template <bool ERODE>
+__device__ __forceinline__ uint32_t pick(uint32_t a, uint32_t b) { return ERODE ? min(a, b) : max(a, b); }
+
+template <bool ERODE>
+__global__ __launch_bounds__(256) void morph_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, int w, int h,
+                                                    MorphPlan P, uint8_t outside, int tiles_x) {
+    extern __shared__ uint8_t sm[];
+    const int r = P.r;
+    const int WW = MT + 2 * r;           // window side
+    const int WS = WW + 4;               // padded stride
+    uint8_t* win = sm;                   // WW x WS
+    uint8_t* H = sm + WW * WS;           // nrun x WW x MT
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int x0 = tx * MT - r, y0 = ty * MT - r;
+    for (int k = threadIdx.x; k < WW * WW; k += blockDim.x) {
+        const int lx = k % WW, ly = k / WW;
+        const int gx = x0 + lx, gy = y0 + ly;
+        uint8_t v = outside;
+        if (gx >= 0 && gx < w && gy >= 0 && gy < h) v = in[(size_t)gy * w + gx];
+        win[ly * WS + lx] = v;
+    }
+    __syncthreads();
+    for (int d = 0; d < P.nrun; ++d) {
+        const int a = P.ra[d], b = P.rb[d], bs = P.base[d];
+        uint8_t* Hd = H + d * WW * MT;
+        for (int k = threadIdx.x; k < WW * MT; k += blockDim.x) {
+            const int x = k % MT, row = k / MT;
+            const uint8_t* src = win + row * WS + x + r;
+            uint32_t v;
+            if (bs >= 0) {
+                const int pa = P.ra[bs], pb = P.rb[bs];
+                v = H[bs * WW * MT + k];
+                for (int j = a; j < pa; ++j) v = pick<ERODE>(v, src[j]);
+                for (int j = pb + 1; j <= b; ++j) v = pick<ERODE>(v, src[j]);
+            } else {
+                v = src[a];
+                for (int j = a + 1; j <= b; ++j) v = pick<ERODE>(v, src[j]);
+            }
+            Hd[k] = (uint8_t)v;
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < MT * MT; k += blockDim.x) {
+        const int x = k % MT, y = k / MT;
+        const int gx = tx * MT + x, gy = ty * MT + y;
+        if (gx >= w || gy >= h) continue;
+        uint32_t v = ERODE ? 255u : 0u;
+        for (int t = 0; t < P.nrow; ++t) {
+            const int i = P.row_i[t];
+            v = pick<ERODE>(v, H[P.row_run[t] * WW * MT + (y + i) * MT + x]);
+        }
+        out[(size_t)gy * w + gx] = (uint8_t)v;
+    }
+}
+
+int make_plan(const uint8_t* kernel, int k, MorphPlan& P) {
+    if (k < 1 || k > MAXK || (k % 2) == 0) {
+        set_error("structuring element must be odd-sized and at most %d", MAXK);
+        return NSC_ERR_ARG;
+    }
+    P.k = k;
+    P.r = k / 2;
+    std::vector<std::pair<int, int>> runs;
+    std::vector<std::pair<int, std::pair<int, int>>> rows;
+    for (int i = 0; i < k; ++i) {
+        int j = 0;
+        while (j < k) {
+            while (j < k && !kernel[i * k + j]) ++j;
+            if (j >= k) break;
+            int s = j;
+            while (j < k && kernel[i * k + j]) ++j;
+            rows.push_back({i, {s - P.r, j - 1 - P.r}});
+            runs.push_back({s - P.r, j - 1 - P.r});
+        }
+    }
+    std::sort(runs.begin(), runs.end(), [](auto& x, auto& y) {
+        int lx = x.second - x.first, ly = y.second - y.first;
+        return lx != ly ? lx < ly : x < y;
+    });
+    runs.erase(std::unique(runs.begin(), runs.end()), runs.end());
+    if ((int)runs.size() > MAXRUN || (int)rows.size() > MAXRUN) {
+        set_error("structuring element has too many runs");
+        return NSC_ERR_ARG;
+    }
+    P.nrun = (int)runs.size();
+    for (int d = 0; d < P.nrun; ++d) {
+        P.ra[d] = runs[d].first;
+        P.rb[d] = runs[d].second;
+        P.base[d] = -1;
+        for (int e = d - 1; e >= 0; --e)
+            if (runs[e].first >= runs[d].first && runs[e].second <= runs[d].second) { P.base[d] = e; break; }
+    }
+    P.nrow = (int)rows.size();
+    for (int t = 0; t < P.nrow; ++t) {
+        P.row_i[t] = rows[t].first;
+        int d = 0;
+        while (!(runs[d] == rows[t].second)) ++d;
+        P.row_run[t] = d;
+    }
+    return NSC_OK;
+}
+
+}  // namespace
+
+int morph_impl(const uint8_t* in, uint8_t* out, int w, int h, const uint8_t* kernel, int k, int outside, bool erode,
+               hipStream_t s) {
+    MorphPlan P;
+    NSC_CHECK(make_plan(kernel, k, P));
+    const int WW = MT + 2 * P.r;
+    const size_t lds = (size_t)WW * (WW + 4) + (size_t)P.nrun * WW * MT;
+    if (lds > 160 * 1024) {
+        set_error("structuring element needs %zu bytes of LDS", lds);
+        return NSC_ERR_LIMIT;
+    }
+    const int tiles_x = (int)cdiv(w, MT), tiles_y = (int)cdiv(h, MT);
+    if (P.nrow == 0) {  // empty element: erosion -> 255, dilation -> 0 everywhere
+        NSC_HIP(hipMemsetAsync(out, erode ? 255 : 0, (size_t)w * h, s));
+        return NSC_OK;
+    }
+    if (erode)
+        hipLaunchKernelGGL((morph_kernel<true>), dim3(tiles_x * tiles_y), dim3(256), lds, s, in, out, w, h, P,
+                           (uint8_t)outside, tiles_x);
+    else
+        hipLaunchKernelGGL((morph_kernel<false>), dim3(tiles_x * tiles_y), dim3(256), lds, s, in, out, w, h, P,
+                           (uint8_t)outside, tiles_x);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+int morph_open_impl(const uint8_t* in, uint8_t* out, uint8_t* tmp, int w, int h, const uint8_t* kernel, int k,
+                    hipStream_t s) {
+    NSC_CHECK(morph_impl(in, tmp, w, h, kernel, k, 255, true, s));
+    return morph_impl(tmp, out, w, h, kernel, k, 0, false, s);
+}
+
+}  // namespace nsc
+
+using namespace nsc;
+
+extern "C" {
+
+int nsc_erode_u8(const uint8_t* img, uint8_t* out, int w, int h, const uint8_t* kernel, int k, int outside,
+                 void* stream) {
+    NSC_ARG(img && out && kernel && w > 0 && h > 0 && outside >= 0 && outside <= 255, "erode_u8");
+    NSC_CHECK(ensure_init());
+    return morph_impl(img, out, w, h, kernel, k, outside, true, as_stream(stream));
+}
+
+int nsc_dilate_u8(const uint8_t* img, uint8_t* out, int w, int h, const uint8_t* kernel, int k, int outside,
+                  void* stream) {
+    NSC_ARG(img && out && kernel && w > 0 && h > 0 && outside >= 0 && outside <= 255, "dilate_u8");
+    NSC_CHECK(ensure_init());
+    return morph_impl(img, out, w, h, kernel, k, outside, false, as_stream(stream));
+}
+
+int nsc_morph_open_u8(const uint8_t* img, uint8_t* out, int w, int h, const uint8_t* kernel, int k, void* stream) {
+    NSC_ARG(img && out && kernel && w > 0 && h > 0, "morph_open_u8");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    Scratch sc;
+    NSC_CHECK(sc.alloc((size_t)w * h, s));
+    return morph_open_impl(img, out, sc.as<uint8_t>(), w, h, kernel, k, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+// cv::watershed (OpenCV 2.4.9 imgproc/segmentation.cpp) on a BGR image with int32 markers, and
+// nscale::watershed2 (MorphologicOperations.cpp:3717-3760) with NeighborOperations::border
+// (NeighborOperations.cpp:17-44).
+//
+// cv::watershed is a sequential priority flood: 256 FIFO levels, pop from the lowest non-empty
+// level, a popped pixel takes the common positive label of its 4-neighbours (-1 when they
+// disagree) and queues its unknown neighbours at their max-channel colour difference.  The state
+// of an unknown pixel only changes through pops of unknown pixels 4-adjacent to it, and labelled
+// pixels never change, so the flood restricted to one 4-connected component of unknown pixels is
+// independent of every other component: the FIFO order inside a component is the global order
+// restricted to it.  The GPU therefore labels the unknown components (CCL) and runs the exact
+// flood of each component on its own lane, all components in parallel.
+#include "common.hpp"
+
+namespace nsc {
+
+int ccl_neq(const uint8_t* img, uint8_t key, int32_t* labels, int w, int h, int conn, hipStream_t s);
+int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int32_t* rank, hipStream_t s);
+int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s);
+
+namespace {
+
+constexpr int NQ = 256;
+constexpr int IN_QUEUE = -2, WSHED = -1;
+constexpr uint16_t NOPUSH = 0xFFFF;
+
+__device__ __forceinline__ int c_diff(const uint8_t* a, const uint8_t* b) {
+    const int db = abs((int)a[0] - (int)b[0]);
+    const int dg = abs((int)a[1] - (int)b[1]);
+    const int dr = abs((int)a[2] - (int)b[2]);
+    return max(max(db, dg), dr);
+}
+
+// initial state: frame -> -1; unknown = interior with marker <= 0 (reset to 0); level of the
+// initial push = min colour difference to a positive 4-neighbour (frame counts as -1)
+__global__ void k_ws_init(const uint8_t* __restrict__ img, const int32_t* __restrict__ m0, int32_t* __restrict__ m,
+                          uint8_t* __restrict__ unknown, uint16_t* __restrict__ lvl, int W, int H) {
+    const size_t n = (size_t)W * H;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W), y = (int)(i / W);
+        const bool frame = (x == 0 || y == 0 || x == W - 1 || y == H - 1);
+        const int v = m0[i];
+        uint16_t L = NOPUSH;
+        uint8_t u = 0;
+        if (!frame && v <= 0) {
+            u = 1;
+            const uint8_t* p = img + 3 * i;
+            int best = 256;
+            auto pos = [&](size_t j, int xx, int yy) {
+                return !(xx == 0 || yy == 0 || xx == W - 1 || yy == H - 1) && m0[j] > 0;
+            };
+            if (pos(i - 1, x - 1, y)) best = min(best, c_diff(p, p - 3));
+            if (pos(i + 1, x + 1, y)) best = min(best, c_diff(p, p + 3));
+            if (pos(i - W, x, y - 1)) best = min(best, c_diff(p, p - 3 * (size_t)W));
+            if (pos(i + W, x, y + 1)) best = min(best, c_diff(p, p + 3 * (size_t)W));
+            if (best < 256) L = (uint16_t)best;
+        }
+        m[i] = frame ? WSHED : (v < 0 ? 0 : v);
+        unknown[i] = u;
+        lvl[i] = L;
+    }
+}
+
+// exact flood of one unknown component per thread
+__global__ void k_ws_flood(const uint8_t* __restrict__ img, int32_t* m, const int32_t* __restrict__ comp,
+                           const uint16_t* __restrict__ lvl, const int32_t* __restrict__ box5, int ncomp, int W,
+                           int* __restrict__ next, int* __restrict__ qhead, int* __restrict__ qtail) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncomp) return;
+    int* head = qhead + (size_t)c * NQ;
+    int* tail = qtail + (size_t)c * NQ;
+    for (int l = 0; l < NQ; ++l) { head[l] = -1; tail[l] = -1; }
+    const int minx = box5[ncomp + c], maxx = box5[2 * ncomp + c];
+    const int miny = box5[3 * ncomp + c], maxy = box5[4 * ncomp + c];
+    const int id = c + 1;
+    int active = NQ;
+    auto push = [&](int l, int p) {
+        next[p] = -1;
+        if (tail[l] >= 0) next[tail[l]] = p;
+        else head[l] = p;
+        tail[l] = p;
+        if (l < active) active = l;
+    };
+    // initial pushes in raster order
+    for (int y = miny; y <= maxy; ++y)
+        for (int x = minx; x <= maxx; ++x) {
+            const int i = y * W + x;
+            if (comp[i] == id && lvl[i] != NOPUSH) {
+                push(lvl[i], i);
+                m[i] = IN_QUEUE;
+            }
+        }
+    if (active == NQ) return;
+    for (;;) {
+        if (head[active] < 0) {
+            int l = active + 1;
+            while (l < NQ && head[l] < 0) ++l;
+            if (l == NQ) break;
+            active = l;
+        }
+        const int p = head[active];
+        const int nx = next[p];
+        head[active] = nx;
+        if (nx < 0) tail[active] = -1;
+        int lab = 0, t;
+        t = m[p - 1];
+        if (t > 0) lab = t;
+        t = m[p + 1];
+        if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+        t = m[p - W];
+        if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+        t = m[p + W];
+        if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+        m[p] = lab;
+        if (lab == WSHED) continue;
+        const uint8_t* ptr = img + 3 * (size_t)p;
+        if (m[p - 1] == 0) { push(c_diff(ptr, ptr - 3), p - 1); m[p - 1] = IN_QUEUE; }
+        if (m[p + 1] == 0) { push(c_diff(ptr, ptr + 3), p + 1); m[p + 1] = IN_QUEUE; }
+        if (m[p - W] == 0) { push(c_diff(ptr, ptr - 3 * (size_t)W), p - W); m[p - W] = IN_QUEUE; }
+        if (m[p + W] == 0) { push(c_diff(ptr, ptr + 3 * (size_t)W), p + W); m[p + W] = IN_QUEUE; }
+    }
+}
+
+// border<int>(padded, -1) at padded (x+1, y+1), written unpadded (the crop of watershed2)
+__global__ void k_ws_border_crop(const int32_t* __restrict__ P, int32_t* __restrict__ out, int w, int h) {
+    const int W = w + 2;
+    const size_t n = (size_t)w * h;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % w) + 1, y = (int)(i / w) + 1;
+        const int32_t* r = P + (size_t)y * W;
+        const int32_t* u = r - W;
+        const int32_t p = r[x];
+        int32_t o = p;
+        if (u[x] == -1 && p != -1 && ((u[x - 1] != -1 && r[x - 1] == -1) || (u[x + 1] != -1 && r[x + 1] == -1))) o = -1;
+        out[i] = o;
+    }
+}
+
+// labels (w x h) -> padded marker image with a 0 frame; BGR image -> padded with (0,0,0)
+__global__ void k_ws_pad(const int32_t* __restrict__ lab, const uint8_t* __restrict__ bgr, int32_t* __restrict__ P,
+                         uint8_t* __restrict__ Pimg, int w, int h) {
+    const int W = w + 2, H = h + 2;
+    const size_t n = (size_t)W * H;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W) - 1, y = (int)(i / W) - 1;
+        const bool in = x >= 0 && y >= 0 && x < w && y < h;
+        const size_t j = (size_t)y * w + x;
+        P[i] = in ? lab[j] : 0;
+        Pimg[3 * i] = in ? bgr[3 * j] : 0;
+        Pimg[3 * i + 1] = in ? bgr[3 * j + 1] : 0;
+        Pimg[3 * i + 2] = in ? bgr[3 * j + 2] : 0;
+    }
+}
+
+inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u)); }
+
+}  // namespace
+
+size_t cvws_scratch(int W, int H) {
+    const size_t n = (size_t)W * H;
+    return n * (sizeof(int32_t) * 3 + 1 + sizeof(uint16_t)) + 64;
+}
+
+// markers m (W x H) in/out; m0 scratch copy is made here
+int cvws_impl(const uint8_t* img, int32_t* m, int W, int H, void* scratch, hipStream_t s) {
+    const size_t n = (size_t)W * H;
+    int32_t* m0 = static_cast<int32_t*>(scratch);
+    int32_t* comp = m0 + n;
+    int32_t* rank = comp + n;  // relabel scratch, then the flood's next[] links
+    uint16_t* lvl = reinterpret_cast<uint16_t*>(rank + n);
+    uint8_t* unknown = reinterpret_cast<uint8_t*>(lvl + n);
+    NSC_HIP(hipMemcpyAsync(m0, m, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_ws_init, grid1(n), dim3(256), 0, s, img, m0, m, unknown, lvl, W, H);
+    NSC_CHECK(ccl_neq(unknown, 0, comp, W, H, 4, s));
+    Scratch cnt;
+    NSC_CHECK(cnt.alloc(sizeof(int), s));
+    NSC_CHECK(relabel_impl(comp, W, H, -1, cnt.as<int>(), rank, s));
+    int ncomp = 0;
+    NSC_HIP(hipMemcpyAsync(&ncomp, cnt.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    NSC_HIP(hipStreamSynchronize(s));
+    if (ncomp == 0) return NSC_OK;
+    Scratch q;
+    NSC_CHECK(q.alloc(sizeof(int32_t) * ((size_t)ncomp * (5 + 2 * NQ)), s));
+    int32_t* box5 = q.as<int32_t>();
+    int* qhead = box5 + 5 * (size_t)ncomp;
+    int* qtail = qhead + (size_t)ncomp * NQ;
+    NSC_CHECK(bbox_impl(comp, W, H, ncomp, box5, nullptr, s));
+    hipLaunchKernelGGL(k_ws_flood, dim3(cdiv(ncomp, 64)), dim3(64), 0, s, img, m, comp, lvl, box5, ncomp, W, rank,
+                       qhead, qtail);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+size_t ws2_scratch(int w, int h) {
+    const size_t N = (size_t)(w + 2) * (h + 2);
+    return N * (sizeof(int32_t) + 3) + cvws_scratch(w + 2, h + 2);
+}
+
+// watershed2 tail: labels of the minima (w x h, bg -1) + BGR -> out (w x h)
+int ws2_from_labels(const uint8_t* bgr, const int32_t* labels, int32_t* out, int w, int h, void* scratch, hipStream_t s) {
+    const size_t N = (size_t)(w + 2) * (h + 2);
+    int32_t* P = static_cast<int32_t*>(scratch);
+    uint8_t* Pimg = reinterpret_cast<uint8_t*>(P + N);
+    void* rest = Pimg + 3 * N + (16 - (3 * N) % 16) % 16;
+    hipLaunchKernelGGL(k_ws_pad, grid1(N), dim3(256), 0, s, labels, bgr, P, Pimg, w, h);
+    NSC_CHECK(cvws_impl(Pimg, P, w + 2, h + 2, rest, s));
+    hipLaunchKernelGGL(k_ws_border_crop, grid1((size_t)w * h), dim3(256), 0, s, P, out, w, h);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
+}  // namespace nsc
+
+using namespace nsc;
+
+namespace nsc {
+int local_max_impl(const float* img, uint8_t* out, int w, int h, int conn, bool minima, void* scratch, hipStream_t s);
+size_t local_max_scratch(int w, int h);
+int ccl_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn, bool equal_values, int32_t bgval, hipStream_t s);
+}  // namespace nsc
+
+extern "C" {
+
+int nsc_cv_watershed(const uint8_t* bgr, int32_t* markers, int w, int h, void* stream) {
+    NSC_ARG(bgr && markers && w > 0 && h > 0, "cv_watershed");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    Scratch sc;
+    NSC_CHECK(sc.alloc(cvws_scratch(w, h), s));
+    return cvws_impl(bgr, markers, w, h, sc.p, s);
+}
+
+int nsc_watershed2(const uint8_t* bgr, const float* img, int32_t* out, int w, int h, int conn, void* stream) {
+    NSC_ARG(bgr && img && out && w > 0 && h > 0 && valid_conn(conn), "watershed2");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    const size_t n = (size_t)w * h;
+    Scratch a, b;
+    NSC_CHECK(a.alloc(std::max(local_max_scratch(w, h), ws2_scratch(w, h)), s));
+    NSC_CHECK(b.alloc(n * (1 + sizeof(int32_t)) + 16, s));
+    uint8_t* minima = b.as<uint8_t>();
+    int32_t* labels = reinterpret_cast<int32_t*>(minima + n + (16 - n % 16) % 16);
+    NSC_CHECK(local_max_impl(img, minima, w, h, conn, true, a.p, s));  // localMinima (MO.cpp:3730)
+    NSC_CHECK(ccl_u8(minima, labels, w, h, conn, true, -1, s));       // bwlabel2 (MO.cpp:3737)
+    return ws2_from_labels(bgr, labels, out, w, h, a.p, s);
+}
+
+}  // extern "C"

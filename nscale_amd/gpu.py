@@ -33,3 +33,215 @@ def imreconstruct(marker, mask, conn=8):
           torch.int32: "nsc_imreconstruct_i32"}[mask.dtype]
     call(fn, _ptr(marker), _ptr(mask), _ptr(out), w, h, conn, _stream())
     return out
+
+
+def _u8out(t):
+    return torch.empty(t.shape[0], t.shape[1], dtype=torch.uint8, device=t.device)
+
+
+def _i32out(t):
+    return torch.empty(t.shape[0], t.shape[1], dtype=torch.int32, device=t.device)
+
+
+def imreconstruct_binary(marker, mask, conn=8):
+    """nscale::imreconstructBinary<uchar>(seeds, image, conn)."""
+    h, w = _hw(mask)
+    out = _u8out(mask)
+    call("nsc_imreconstruct_binary_u8", _ptr(marker), _ptr(mask), _ptr(out), w, h, conn, _stream())
+    return out
+
+
+def imfill(img, seeds, conn=8):
+    """nscale::imfill<uchar>(image, seeds, binary=true, conn)."""
+    h, w = _hw(img)
+    out = _u8out(img)
+    call("nsc_imfill_u8", _ptr(img), _ptr(seeds), _ptr(out), w, h, conn, _stream())
+    return out
+
+
+def imfill_holes(img, conn=4):
+    """nscale::imfillHoles<uchar>(image, binary=true, conn)."""
+    h, w = _hw(img)
+    out = _u8out(img)
+    call("nsc_imfill_holes_u8", _ptr(img), _ptr(out), w, h, conn, _stream())
+    return out
+
+
+def bwselect(bw, seeds, conn=8):
+    """nscale::bwselect<uchar>(binaryImage, seeds, conn)."""
+    h, w = _hw(bw)
+    out = _u8out(bw)
+    call("nsc_bwselect_u8", _ptr(bw), _ptr(seeds), _ptr(out), w, h, conn, _stream())
+    return out
+
+
+def imhmin(img, hval=1.0, conn=8):
+    """nscale::imhmin<float>(image, h, conn)."""
+    h, w = _hw(img)
+    out = torch.empty_like(img)
+    lib().nsc_imhmin_f32.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int, _P]
+    call("nsc_imhmin_f32", _ptr(img), _ptr(out), w, h, float(hval), conn, _stream())
+    return out
+
+
+def cc_label(img, conn=8, bgval=-1):
+    """ConnComponents::label (equal-value union-find, root = smallest raster index)."""
+    h, w = _hw(img)
+    out = _i32out(img)
+    call("nsc_cc_label_u8", _ptr(img), _ptr(out), w, h, bgval, conn, _stream())
+    return out
+
+
+def bwlabel2(img, conn=8):
+    """nscale::bwlabel2(binaryImage, conn, relab): labels = smallest raster index, bg -1."""
+    h, w = _hw(img)
+    out = _i32out(img)
+    call("nsc_bwlabel2_u8", _ptr(img), _ptr(out), w, h, conn, _stream())
+    return out
+
+
+def cc_relabel(labels, bgval=-1):
+    """ConnComponents::relabel in place; returns the number of components."""
+    h, w = _hw(labels)
+    n = ctypes.c_int(0)
+    call("nsc_cc_relabel", _ptr(labels), w, h, bgval, ctypes.byref(n), _stream())
+    return n.value
+
+
+def bwareaopen2(img, min_size, max_size, conn=8):
+    """nscale::bwareaopen2(img, false, true, min, max, conn, count) -> (mask, count)."""
+    h, w = _hw(img)
+    out = _u8out(img)
+    n = ctypes.c_int(0)
+    call("nsc_bwareaopen2_u8", _ptr(img), _ptr(out), w, h, min_size, max_size, conn, ctypes.byref(n), _stream())
+    return out, n.value
+
+
+def bounding_boxes(canon_labels, n):
+    """[label | minx | maxx | miny | maxy] (5 x n) and per-label area for canonical labels 1..n."""
+    h, w = _hw(canon_labels)
+    out5 = torch.empty(5 * max(n, 1), dtype=torch.int32, device=canon_labels.device)
+    area = torch.empty(max(n, 1), dtype=torch.int32, device=canon_labels.device)
+    call("nsc_bounding_boxes", _ptr(canon_labels), w, h, n, _ptr(out5), _ptr(area), _stream())
+    return out5[:5 * n].view(5, n), area[:n]
+
+
+def _kernel_arg(kernel):
+    import numpy as np
+    k = np.ascontiguousarray(np.asarray(kernel, dtype=np.uint8))
+    return k, k.ctypes.data_as(_P), int(k.shape[0])
+
+
+def erode(img, kernel, outside=255):
+    h, w = _hw(img)
+    out = _u8out(img)
+    k, kp, ks = _kernel_arg(kernel)
+    call("nsc_erode_u8", _ptr(img), _ptr(out), w, h, kp, ks, outside, _stream())
+    return out
+
+
+def dilate(img, kernel, outside=0):
+    h, w = _hw(img)
+    out = _u8out(img)
+    k, kp, ks = _kernel_arg(kernel)
+    call("nsc_dilate_u8", _ptr(img), _ptr(out), w, h, kp, ks, outside, _stream())
+    return out
+
+
+def morph_open(img, kernel):
+    """nscale::morphOpen<uchar>(image, kernel)."""
+    h, w = _hw(img)
+    out = _u8out(img)
+    k, kp, ks = _kernel_arg(kernel)
+    call("nsc_morph_open_u8", _ptr(img), _ptr(out), w, h, kp, ks, _stream())
+    return out
+
+
+def dist_transform(img):
+    """cv::distanceTransform(img, CV_DIST_L2, CV_DIST_MASK_PRECISE) as used by the chain."""
+    h, w = _hw(img)
+    out = torch.empty(h, w, dtype=torch.float32, device=img.device)
+    call("nsc_dist_transform_u8", _ptr(img), _ptr(out), w, h, _stream())
+    return out
+
+
+def local_maxima(img, conn=8):
+    h, w = _hw(img)
+    out = _u8out(img)
+    call("nsc_local_maxima_f32", _ptr(img), _ptr(out), w, h, conn, _stream())
+    return out
+
+
+def local_minima(img, conn=8):
+    h, w = _hw(img)
+    out = _u8out(img)
+    call("nsc_local_minima_f32", _ptr(img), _ptr(out), w, h, conn, _stream())
+    return out
+
+
+def cv_watershed(bgr, markers):
+    """cv::watershed(image, markers) in place on a copy; returns the markers."""
+    h, w = _hw(markers)
+    m = markers.clone()
+    call("nsc_cv_watershed", _ptr(bgr), _ptr(m), w, h, _stream())
+    return m
+
+
+def watershed2(bgr, img, conn=8):
+    """nscale::watershed2(origImage, image, conn)."""
+    h, w = _hw(img)
+    out = _i32out(img)
+    call("nsc_watershed2", _ptr(bgr), _ptr(img), _ptr(out), w, h, conn, _stream())
+    return out
+
+
+class Workspace:
+    """Device workspace for segment_nuclei on tiles up to max_w x max_h."""
+
+    def __init__(self, max_w, max_h):
+        self.handle = _P()
+        call("nsc_workspace_create", max_w, max_h, ctypes.byref(self.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().nsc_workspace_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def default_params():
+    p = SegParams()
+    lib().nsc_seg_params_default(ctypes.byref(p))
+    return p
+
+
+STAGE_TYPES = {16: torch.float32, 17: torch.float32, 18: torch.float32, 20: torch.int32, 25: torch.int32}
+
+
+def segment_nuclei(bgr, params=None, ws=None, stages=(), timing=False):
+    """HistologicalEntities::segmentNuclei on a device BGR tile.
+
+    Returns (status, labels, compcount, {stage_id: tensor}, {stage_id: ms})."""
+    h, w = int(bgr.shape[0]), int(bgr.shape[1])
+    p = params or default_params()
+    ws = ws or Workspace(w, h)
+    labels = torch.empty(h, w, dtype=torch.int32, device=bgr.device)
+    arr = (_P * 32)()
+    bufs = {}
+    for s in stages:
+        if s in (0, 19):
+            bufs[s] = torch.zeros(h, w, 3, dtype=torch.uint8, device=bgr.device)
+        else:
+            bufs[s] = torch.zeros(h, w, dtype=STAGE_TYPES.get(s, torch.uint8), device=bgr.device)
+        arr[s] = bufs[s].data_ptr()
+    ms = (ctypes.c_double * 32)()
+    cc = ctypes.c_int(0)
+    L = lib()
+    L.nsc_segment_nuclei.argtypes = [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P, _P, _P]
+    st = L.nsc_segment_nuclei(_ptr(bgr), w, h, ctypes.byref(p), _ptr(labels), ctypes.byref(cc), ws.handle,
+                              arr if stages else None, ms if timing else None, _stream())
+    if st < -2:
+        from nscale_amd._lib import check
+        check(st, "nsc_segment_nuclei")
+    return st, labels, cc.value, bufs, {i: ms[i] for i in range(32) if ms[i] > 0}

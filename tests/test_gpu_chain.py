@@ -1,0 +1,71 @@
+"""GPU parity of HistologicalEntities::segmentNuclei, stage by stage, against the oracle chain."""
+import numpy as np
+import pytest
+
+from tests._lib import STAGES, o_segment
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+ALL_STAGES = tuple(sorted(STAGES))
+
+
+def run_both(bgr, stages=ALL_STAGES):
+    from nscale_amd import gpu
+    st, lab, cc, bufs, _ = gpu.segment_nuclei(torch.from_numpy(bgr).cuda(), stages=stages)
+    torch.cuda.synchronize()
+    ost, olab, obufs = o_segment(bgr, stages=stages)
+    return (st, lab.cpu().numpy(), cc, {k: v.cpu().numpy() for k, v in bufs.items()}), (ost, olab, obufs)
+
+
+@pytest.mark.parametrize("size,seed", [((256, 256), 1), ((384, 520), 2), ((1024, 1024), 3)])
+def test_chain_stages_synthetic(size, seed):
+    from nscale_amd.synth import synth_he
+    bgr = synth_he(size[0], size[1], seed)
+    print("chain", size, seed, flush=True)
+    (st, lab, cc, bufs), (ost, olab, obufs) = run_both(bgr)
+    assert st == ost == 0
+    for k in sorted(obufs):
+        assert np.array_equal(bufs[k], obufs[k]), "stage %d differs" % k
+    assert np.array_equal(lab, olab)
+    assert cc == len(np.unique(olab[olab >= 0]))
+
+
+def test_chain_reference_image():
+    import os
+    from PIL import Image
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "watershed.png")
+    bgr = np.ascontiguousarray(np.asarray(Image.open(gold).convert("RGB"))[:, :, ::-1])
+    print("chain reference image", flush=True)
+    (st, lab, cc, bufs), (ost, olab, obufs) = run_both(bgr)
+    assert st == ost
+    if st == 0:
+        for k in sorted(obufs):
+            assert np.array_equal(bufs[k], obufs[k]), "stage %d differs" % k
+        assert np.array_equal(lab, olab)
+
+
+def test_chain_status_codes():
+    bg = np.full((64, 64, 3), 240, np.uint8)
+    from nscale_amd import gpu
+    assert gpu.segment_nuclei(torch.from_numpy(bg).cuda())[0] == 1
+    bg2 = bg.copy()
+    bg2[:3] = 100
+    assert gpu.segment_nuclei(torch.from_numpy(bg2).cuda())[0] == 2
+    rng = np.random.default_rng(0)
+    noise = rng.integers(150, 200, (64, 64, 3)).astype(np.uint8)
+    assert gpu.segment_nuclei(torch.from_numpy(noise).cuda())[0] == 3
+
+
+def test_chain_rbc_branch():
+    # a red-blood-cell-like blob with R/G > 5 next to nuclei exercises getRBC's selection
+    from nscale_amd.synth import synth_he
+    bgr = synth_he(256, 256, 4).copy()
+    bgr[100:110, 100:110] = (60, 30, 220)   # B, G, R: R/G = 7.3 > T1, R/B > 1
+    bgr[110:114, 100:110] = (60, 50, 215)   # R/G = 4.3: in bw2 only, joined through bwselect
+    (st, lab, cc, bufs), (ost, olab, obufs) = run_both(bgr)
+    assert st == ost
+    assert obufs[2].any(), "the RBC branch should fire"
+    assert np.array_equal(bufs[2], obufs[2])
+    if st == 0:
+        assert np.array_equal(lab, olab)

@@ -117,6 +117,11 @@ int nsc_local_minima_f32(const float* img, uint8_t* out, int w, int h, int conn,
 /* cv::watershed(image, markers) on a BGR image, markers in/out (as used by
    MorphologicOperations.cpp:3750) */
 int nsc_cv_watershed(const uint8_t* bgr, int32_t* markers, int w, int h, void* stream);
+/* process-wide counts of cv::watershed evaluations served by the decomposed parallel flood and by
+   the sequential per-component fallback, and the reason bits of the last fallback (1 leaf capacity,
+   2 comparison walk, 4 tie group too large / level-0 first background pixel, 8 heap capacity,
+   16 tie order matters); no reference counterpart, reporting only */
+void nsc_watershed_stats(int* fast, int* fallback, int* last_reason);
 /* nscale::watershed2(origImage, image, conn)  MorphologicOperations.h:103, .cpp:3717-3760 */
 int nsc_watershed2(const uint8_t* bgr, const float* img, int32_t* out, int w, int h, int conn, void* stream);
 
@@ -146,7 +151,9 @@ int nsc_workspace_destroy(nsc_workspace* ws);
    nuclei.  stage (may be NULL) is an array of 32 device pointers; a non-NULL entry i receives the
    intermediate image with the reference's stage id i (IntermediateResultHandler ids,
    UtilsCVImageIO.h:32-51), 24 = filled final mask, 25 = labels.  stage_ms (may be NULL, host,
-   32 doubles) receives per-stage GPU times in milliseconds.  Returns the chain status code. */
+   32 doubles) accumulates per-stage GPU times in milliseconds under the same ids (the time up to
+   the stage's dump point); id 26 is the imreconstruct<uchar>(rc_open, rc) call alone.  Returns
+   the chain status code. */
 int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p, int32_t* labels,
                        int* compcount, nsc_workspace* ws, void** stage, double* stage_ms, void* stream);
 

@@ -266,6 +266,7 @@ int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p
     NSC_CHECK(save(3, W.rc_open, n));
     mark(3);
     NSC_CHECK(recon_impl<uint8_t>(W.rc_open, W.rc, W.rc_recon, w, h, p.reconConnectivity, s));
+    mark(26);  // timing id 26: the imreconstruct<uchar>(rc_open, rc) call alone
     NSC_CHECK(save(4, W.rc_recon, n));
     hipLaunchKernelGGL(k_diff_thr, grid1(n), dim3(256), 0, s, W.rc, W.rc_recon, n, p.G1, p.G2, W.diff, W.t1, W.t2);
     NSC_CHECK(save(5, W.diff, n));

@@ -264,3 +264,194 @@ template int recon_impl<int32_t>(const int32_t*, const int32_t*, int32_t*, int, 
 template int recon_impl<float>(const float*, const float*, float*, int, int, int, hipStream_t);
 
 }  // namespace nsc
+
+// ------------------------------------------------------------------------------------------
+// Geodesic (4-connected, unit step) distance inside a passable mask from source pixels with
+// dist = 0: the same tile/sweep/pass schedule as the reconstruction above in the (min, +1)
+// algebra.  dist holds the initial values (0 at sources, UINT_MAX elsewhere) and the result.
+// ------------------------------------------------------------------------------------------
+namespace nsc {
+namespace {
+
+constexpr unsigned GINF = 0xFFFFFFFFu;
+
+__device__ __forceinline__ unsigned inc1(unsigned v) { return v == GINF ? GINF : v + 1u; }
+
+__global__ __launch_bounds__(64) void geo_tiles(const uint8_t* __restrict__ pass, unsigned* dist, int w, int h, int tiles_x,
+                                                int tiles_y, const int* __restrict__ list, const int* __restrict__ list_count,
+                                                int* __restrict__ next_list, int* __restrict__ next_count,
+                                                unsigned* __restrict__ stamp, int pass_no) {
+    __shared__ uint8_t lp[TH][TW + 4];       // 1 = free pixel of the window, 0 = frozen
+    __shared__ unsigned tr[TH][TW + 1];
+    __shared__ unsigned init_l[TH], init_r[TH];
+    __shared__ unsigned chg_bits;
+    int tile;
+    if (pass_no == 0) tile = blockIdx.x;
+    else {
+        if ((int)blockIdx.x >= *list_count) return;
+        tile = list[blockIdx.x];
+    }
+    const int tx = tile % tiles_x, ty = tile / tiles_x;
+    const int lane = threadIdx.x;
+    const int gx = tx * OW - 1 + lane;
+    const bool col_in = (gx >= 0 && gx < w);
+    const bool col_frozen = (lane == 0 || lane == TW - 1 || !col_in);
+    if (lane == 0) chg_bits = 0;
+    unsigned v[TH];
+#pragma unroll
+    for (int y = 0; y < TH; ++y) {
+        const int gy = ty * OH - 1 + y;
+        const bool inb = col_in && gy >= 0 && gy < h;
+        unsigned val = GINF;
+        uint8_t ps = 0;
+        if (inb) {
+            const size_t i = (size_t)gy * w + gx;
+            val = dist[i];
+            ps = pass[i] ? 1 : 0;
+        }
+        const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb || !ps;
+        v[y] = val;
+        lp[y][lane] = frozen ? 0 : 1;
+    }
+    if (lane == 1) {
+#pragma unroll
+        for (int y = 0; y < TH; ++y) init_l[y] = v[y];
+    }
+    if (lane == TW - 2) {
+#pragma unroll
+        for (int y = 0; y < TH; ++y) init_r[y] = v[y];
+    }
+    const unsigned init_top = v[1], init_bot = v[TH - 2];
+    __syncthreads();
+    bool any_change = false;
+    for (int round = 0; round < 1 << 20; ++round) {
+        bool changed = false;
+        {
+            unsigned pc = v[0];
+#pragma unroll
+            for (int y = 1; y < TH - 1; ++y) {
+                const unsigned nv = lp[y][lane] ? min(v[y], inc1(pc)) : v[y];
+                changed |= (nv != v[y]);
+                v[y] = nv;
+                pc = nv;
+            }
+            pc = v[TH - 1];
+#pragma unroll
+            for (int y = TH - 2; y >= 1; --y) {
+                const unsigned nv = lp[y][lane] ? min(v[y], inc1(pc)) : v[y];
+                changed |= (nv != v[y]);
+                v[y] = nv;
+                pc = nv;
+            }
+        }
+#pragma unroll
+        for (int y = 0; y < TH; ++y) tr[y][lane] = v[y];
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < TW; ++x) v[x] = tr[lane][x];
+        {
+            unsigned prev = v[0];
+#pragma unroll
+            for (int x = 1; x < TW - 1; ++x) {
+                const unsigned nv = lp[lane][x] ? min(v[x], inc1(prev)) : v[x];
+                changed |= (nv != v[x]);
+                v[x] = nv;
+                prev = nv;
+            }
+            prev = v[TW - 1];
+#pragma unroll
+            for (int x = TW - 2; x >= 1; --x) {
+                const unsigned nv = lp[lane][x] ? min(v[x], inc1(prev)) : v[x];
+                changed |= (nv != v[x]);
+                v[x] = nv;
+                prev = nv;
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < TW; ++x) tr[lane][x] = v[x];
+        __syncthreads();
+#pragma unroll
+        for (int y = 0; y < TH; ++y) v[y] = tr[y][lane];
+        __syncthreads();
+        const bool wave_changed = __any(changed);
+        any_change |= wave_changed;
+        if (!wave_changed) break;
+    }
+    if (!any_change) return;
+    if (!col_frozen) {
+#pragma unroll
+        for (int y = 1; y < TH - 1; ++y) {
+            const int gy = ty * OH - 1 + y;
+            if (gy < h) dist[(size_t)gy * w + gx] = v[y];
+        }
+    }
+    unsigned bits = 0;
+    if (lane >= 1 && lane <= TW - 2) {
+        if (v[1] != init_top) bits |= CH_TOP;
+        if (v[TH - 2] != init_bot) bits |= CH_BOT;
+    }
+    if (lane == 1) {
+#pragma unroll
+        for (int y = 1; y < TH - 1; ++y)
+            if (v[y] != init_l[y]) bits |= CH_LEFT;
+    }
+    if (lane == TW - 2) {
+#pragma unroll
+        for (int y = 1; y < TH - 1; ++y)
+            if (v[y] != init_r[y]) bits |= CH_RIGHT;
+    }
+    if (bits) atomicOr(&chg_bits, bits);
+    __syncthreads();
+    if (lane == 0) {
+        const unsigned b = chg_bits;
+        const unsigned np = (unsigned)pass_no + 1;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if ((dx == 0) == (dy == 0)) continue;  // 4-connected: edge neighbours only
+                bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) || (dx == -1 && (b & CH_LEFT)) ||
+                            (dx == 1 && (b & CH_RIGHT));
+                const int nx = tx + dx, ny = ty + dy;
+                if (need && nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y)
+                    mark_tile(ny * tiles_x + nx, np, stamp, next_list, next_count);
+            }
+    }
+}
+
+}  // namespace
+
+int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t s) {
+    const int tiles_x = (int)cdiv(w, OW), tiles_y = (int)cdiv(h, OH);
+    const int nt = tiles_x * tiles_y;
+    const int max_pass = 4 * nt + 8;
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2), s));
+    unsigned* stamp = sc.as<unsigned>();
+    int* listA = reinterpret_cast<int*>(stamp + nt);
+    int* listB = listA + nt;
+    int* counts = listB + nt;
+    NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt, s));
+    NSC_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (max_pass + 2), s));
+    hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, pass, dist, w, h, tiles_x, tiles_y, nullptr, nullptr, listA,
+                       &counts[0], stamp, 0);
+    int p = 1;
+    for (;;) {
+        for (int b = 0; b < 4 && p <= max_pass; ++b, ++p) {
+            int* cur = (p & 1) ? listA : listB;
+            int* nxt = (p & 1) ? listB : listA;
+            hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, pass, dist, w, h, tiles_x, tiles_y, cur,
+                               &counts[p - 1], nxt, &counts[p], stamp, p);
+        }
+        NSC_HIP(hipGetLastError());
+        int c = 0;
+        NSC_HIP(hipMemcpyAsync(&c, &counts[p - 1], sizeof(int), hipMemcpyDeviceToHost, s));
+        NSC_HIP(hipStreamSynchronize(s));
+        if (c == 0) break;
+        if (p > max_pass) {
+            set_error("geodesic distance: no convergence after %d passes", max_pass);
+            return NSC_ERR_LIMIT;
+        }
+    }
+    return NSC_OK;
+}
+
+}  // namespace nsc

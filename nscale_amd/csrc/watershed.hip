@@ -10,6 +10,8 @@
 // independent of every other component: the FIFO order inside a component is the global order
 // restricted to it.  The GPU therefore labels the unknown components (CCL) and runs the exact
 // flood of each component on its own lane, all components in parallel.
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace nsc {
@@ -156,12 +158,21 @@ inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u
 
 }  // namespace
 
+int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, int* ok, hipStream_t s);
+
+// how often the decomposed flood applied / fell back (process-wide counters, for reporting)
+static int g_ws_fast = 0, g_ws_fallback = 0, g_ws_last_flags = 0;
+void ws_stats_record(int ok) { if (ok) ++g_ws_fast; else ++g_ws_fallback; }
+void ws_last_flags_record(int flags) { g_ws_last_flags = flags; }
+
 size_t cvws_scratch(int W, int H) {
     const size_t n = (size_t)W * H;
     return n * (sizeof(int32_t) * 3 + 1 + sizeof(uint16_t)) + 64;
 }
 
-// markers m (W x H) in/out; m0 scratch copy is made here
+// markers m (W x H) in/out; m0 scratch copy is made here.  The decomposed flood (cvws_fast) is
+// tried first; when its assumptions do not hold it reports so and the plain per-component
+// sequential flood below produces the result.  NSC_WATERSHED=seq forces the sequential flood.
 int cvws_impl(const uint8_t* img, int32_t* m, int W, int H, void* scratch, hipStream_t s) {
     const size_t n = (size_t)W * H;
     int32_t* m0 = static_cast<int32_t*>(scratch);
@@ -170,6 +181,18 @@ int cvws_impl(const uint8_t* img, int32_t* m, int W, int H, void* scratch, hipSt
     uint16_t* lvl = reinterpret_cast<uint16_t*>(rank + n);
     uint8_t* unknown = reinterpret_cast<uint8_t*>(lvl + n);
     NSC_HIP(hipMemcpyAsync(m0, m, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    const char* mode = getenv("NSC_WATERSHED");
+    const bool force_seq = mode && mode[0] == 's';
+    if (!force_seq) {
+        int ok = 0;
+        NSC_CHECK(cvws_fast(img, m0, m, W, H, &ok, s));
+        ws_stats_record(ok);
+        if (ok) return NSC_OK;
+        if (mode && mode[0] == 'f') {
+            set_error("watershed: the decomposed flood does not apply to this input (NSC_WATERSHED=fast)");
+            return NSC_ERR_LIMIT;
+        }
+    }
     hipLaunchKernelGGL(k_ws_init, grid1(n), dim3(256), 0, s, img, m0, m, unknown, lvl, W, H);
     NSC_CHECK(ccl_neq(unknown, 0, comp, W, H, 4, s));
     Scratch cnt;
@@ -220,6 +243,13 @@ int ccl_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn, bool equ
 }  // namespace nsc
 
 extern "C" {
+
+/* counts of watershed calls served by the decomposed flood / by the sequential fallback */
+void nsc_watershed_stats(int* fast, int* fallback, int* last_reason) {
+    if (fast) *fast = g_ws_fast;
+    if (fallback) *fallback = g_ws_fallback;
+    if (last_reason) *last_reason = g_ws_last_flags;
+}
 
 int nsc_cv_watershed(const uint8_t* bgr, int32_t* markers, int w, int h, void* stream) {
     NSC_ARG(bgr && markers && w > 0 && h > 0, "cv_watershed");

@@ -1,0 +1,890 @@
+// Exact parallel decomposition of cv::watershed's sequential flood (OpenCV 2.4.9
+// imgproc/segmentation.cpp) for marker images whose unknown region contains a large black
+// region B -- the shape watershed2 gets inside segmentNuclei: the background plateau holds label
+// 0 (it contains pixel (0,0), MorphologicOperations.cpp:3737 labels a component by its smallest
+// index) and is therefore "unknown", and the masked colour image is (0,0,0) there
+// (HistologicalEntities.cpp:400-407).  Every other unknown pixel is in F (non-black).
+//
+// Facts the decomposition rests on (each B-B colour difference is 0, every B-F difference is >0):
+//  1. Runs = 4-connected components of F.  Until the first pop of a B pixel that receives a
+//     positive label (call it b1, at time T1), runs do not interact: a B pixel popped as -1 changes
+//     nothing any other pixel reads.  Each run is flooded on its own lane ("phase 1"); B
+//     neighbours are recorded as leaf items (when, at what level and by whom they are queued).
+//  2. The global pop order of the merged runs is a total order computable from per-run data: for
+//     items x, y of different runs, x pops first iff the largest (level, push time) key on x's
+//     push chain is smaller than y's, comparing push times recursively through the pushers
+//     (a pop can only be delayed by a larger key on its own chain).  `before()` implements it.
+//  3. b1 = the earliest B item whose neighbours popped before it agree on one label Y1.  At T1 the
+//     level-0 FIFO holds nothing else, so the flood from b1 runs to completion through every B
+//     pixel not yet queued that it reaches (label Y1: such a pixel has no labelled F neighbour,
+//     which would have queued it) before anything else pops.
+//  4. After that, the unresolved pixels fall into 4-connected components that are again
+//     independent ("phase 2"), each flooded on its own lane from the queue state at T1.  The only
+//     order not known locally is the FIFO order among F pixels queued by the flood from b1 at the
+//     same level; for those the lane checks that every permutation gives the same labels.
+// Anything outside these assumptions (a capacity, a comparison that does not terminate, a tie
+// whose order matters) raises a flag and the caller re-runs the plain sequential flood.
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+
+#include "common.hpp"
+
+namespace nsc {
+
+int ccl_neq(const uint8_t* img, uint8_t key, int32_t* labels, int w, int h, int conn, hipStream_t s);
+int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int32_t* rank, hipStream_t s);
+int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s);
+
+namespace {
+
+constexpr int NQ = 256;
+constexpr int IN_QUEUE = -2, WSHED = -1;
+constexpr uint16_t NOPUSH = 0xFFFF;
+enum { C_FRAME = 0, C_SEED = 1, C_F = 2, C_B = 3 };
+enum { FB_LEAF_CAP = 1, FB_CMP = 2, FB_TIES = 4, FB_HEAP = 8, FB_TIEPERM = 16 };
+constexpr int CMP_MAX_STEPS = 1 << 22;
+
+struct Dev {
+    // image
+    const uint8_t* img;
+    int W, H, N;
+    const int32_t* m0;
+    int32_t* m;
+    const uint8_t* cls;
+    const uint16_t* lvl0;
+    const int32_t* run;  // canonical run id (1..R) for F pixels
+    // F items (indexed by pixel)
+    uint8_t* st;     // 0 none, 1 queued, 2 popped
+    uint8_t* ilvl;
+    int32_t* pseq;
+    int32_t* parent;  // pusher pixel, -1 = initial push
+    int32_t* bnode;   // item id
+    int32_t* popidx;
+    int32_t* lab;     // label decided in the run's flood
+    int32_t* qnext;   // FIFO links, indexed by item id
+    // leaves
+    int32_t* head;    // per B pixel: first leaf slot or -1
+    int32_t* lf_b;
+    int32_t* lf_run;
+    uint8_t* lf_lvl;
+    int32_t* lf_pseq;
+    int32_t* lf_parent;
+    int32_t* lf_bnode;
+    int32_t* lf_popidx;
+    int32_t* lf_next;
+    int* nleaf;
+    int leafcap;
+    int* flags;
+};
+
+__device__ __forceinline__ int c_diff(const uint8_t* a, const uint8_t* b) {
+    const int db = abs((int)a[0] - (int)b[0]);
+    const int dg = abs((int)a[1] - (int)b[1]);
+    const int dr = abs((int)a[2] - (int)b[2]);
+    return max(max(db, dg), dr);
+}
+
+// ---- item accessors: id < N -> F pixel or initially queued B pixel; id >= N -> leaf slot ----
+__device__ __forceinline__ int it_run(const Dev& D, int x) {
+    if (x >= D.N) return D.lf_run[x - D.N];
+    return D.cls[x] == C_F ? D.run[x] : -(x + 2);
+}
+__device__ __forceinline__ int it_lvl(const Dev& D, int x) {
+    if (x >= D.N) return D.lf_lvl[x - D.N];
+    return D.cls[x] == C_F ? D.ilvl[x] : D.lvl0[x];
+}
+__device__ __forceinline__ int it_parent(const Dev& D, int x) {
+    if (x >= D.N) return D.lf_parent[x - D.N];
+    return D.cls[x] == C_F ? D.parent[x] : -1;
+}
+__device__ __forceinline__ int it_bnode(const Dev& D, int x) {
+    if (x >= D.N) return D.lf_bnode[x - D.N];
+    return D.cls[x] == C_F ? D.bnode[x] : x;
+}
+__device__ __forceinline__ int it_popidx(const Dev& D, int x) {
+    if (x >= D.N) return D.lf_popidx[x - D.N];
+    return D.cls[x] == C_F ? D.popidx[x] : 0;
+}
+__device__ __forceinline__ int it_pseq(const Dev& D, int x) {
+    if (x >= D.N) return D.lf_pseq[x - D.N];
+    return D.cls[x] == C_F ? D.pseq[x] : 0;
+}
+
+// x pops before y in the merged phase-1 flood (x != y); sets FB_CMP if the walk is too long
+__device__ bool before(const Dev& D, int x, int y) {
+    for (int step = 0; step < CMP_MAX_STEPS; ++step) {
+        if (x == y) return false;
+        const int rx = it_run(D, x), ry = it_run(D, y);
+        if (rx == ry) return it_popidx(D, x) < it_popidx(D, y);
+        const int a = it_bnode(D, x), b = it_bnode(D, y);
+        const int la = it_lvl(D, a), lb = it_lvl(D, b);
+        if (la != lb) return la < lb;
+        const int pa = it_parent(D, a), pb = it_parent(D, b);
+        if (pa < 0 && pb < 0) return a < b;  // both queued in the initial raster scan
+        if (pa < 0) return true;
+        if (pb < 0) return false;
+        x = pa;
+        y = pb;
+    }
+    atomicOr(D.flags, FB_CMP);
+    return false;
+}
+
+// ---- classification ----
+__global__ void k_classify(Dev D, uint8_t* cls, uint16_t* lvl0, uint8_t* fmask, int* nF) {
+    const int W = D.W, H = D.H;
+    int local = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < D.N; i += gridDim.x * blockDim.x) {
+        const int x = i % W, y = i / W;
+        const bool frame = (x == 0 || y == 0 || x == W - 1 || y == H - 1);
+        const int v = D.m0[i];
+        uint8_t c;
+        uint16_t L = NOPUSH;
+        if (frame) c = C_FRAME;
+        else if (v > 0) c = C_SEED;
+        else {
+            const uint8_t* p = D.img + 3 * (size_t)i;
+            c = (p[0] == 0 && p[1] == 0 && p[2] == 0) ? C_B : C_F;
+            int best = 256;
+            auto pos = [&](int j, int xx, int yy) {
+                return !(xx == 0 || yy == 0 || xx == W - 1 || yy == H - 1) && D.m0[j] > 0;
+            };
+            if (pos(i - 1, x - 1, y)) best = min(best, c_diff(p, p - 3));
+            if (pos(i + 1, x + 1, y)) best = min(best, c_diff(p, p + 3));
+            if (pos(i - W, x, y - 1)) best = min(best, c_diff(p, p - 3 * (size_t)W));
+            if (pos(i + W, x, y + 1)) best = min(best, c_diff(p, p + 3 * (size_t)W));
+            if (best < 256) L = (uint16_t)best;
+        }
+        cls[i] = c;
+        lvl0[i] = L;
+        fmask[i] = (c == C_F) ? 1 : 0;
+        D.st[i] = 0;
+        D.head[i] = -1;
+        D.m[i] = frame ? WSHED : (v < 0 ? 0 : v);
+        local += (c == C_F);
+    }
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(nF, local);
+}
+
+// ---- phase 1: one lane floods one run ----
+__global__ void k_phase1(Dev D, const int32_t* __restrict__ box5, int nrun, int* __restrict__ qh, int* __restrict__ qt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrun) return;
+    const int W = D.W;
+    const int id = r + 1;
+    int* head = qh + (size_t)r * NQ;
+    int* tail = qt + (size_t)r * NQ;
+    for (int l = 0; l < NQ; ++l) { head[l] = -1; tail[l] = -1; }
+    int active = NQ, seq = 0, popc = 0;
+    auto enq = [&](int l, int x) {
+        D.qnext[x] = -1;
+        if (tail[l] >= 0) D.qnext[tail[l]] = x;
+        else head[l] = x;
+        tail[l] = x;
+        if (l < active) active = l;
+    };
+    // key of an F item or leaf of this run: (level, pseq)
+    auto key_less = [&](int a, int b) {
+        const int la = it_lvl(D, a), lb = it_lvl(D, b);
+        if (la != lb) return la < lb;
+        return it_pseq(D, a) < it_pseq(D, b);
+    };
+    const int minx = box5[nrun + r], maxx = box5[2 * nrun + r];
+    const int miny = box5[3 * nrun + r], maxy = box5[4 * nrun + r];
+    for (int y = miny; y <= maxy; ++y)
+        for (int x = minx; x <= maxx; ++x) {
+            const int i = y * W + x;
+            if (D.cls[i] == C_F && D.run[i] == id && D.lvl0[i] != NOPUSH) {
+                D.ilvl[i] = (uint8_t)D.lvl0[i];
+                D.pseq[i] = seq++;
+                D.parent[i] = -1;
+                D.bnode[i] = i;
+                D.st[i] = 1;
+                enq(D.lvl0[i], i);
+            }
+        }
+    const int dirs[4] = {-1, 1, -W, W};
+    for (;;) {
+        if (active >= NQ) break;
+        if (head[active] < 0) {
+            int l = active + 1;
+            while (l < NQ && head[l] < 0) ++l;
+            if (l == NQ) break;
+            active = l;
+        }
+        const int x = head[active];
+        const int nx = D.qnext[x];
+        head[active] = nx;
+        if (nx < 0) tail[active] = -1;
+        if (x >= D.N) {  // leaf: record when it pops, nothing else
+            D.lf_popidx[x - D.N] = popc++;
+            continue;
+        }
+        const int p = x;
+        D.popidx[p] = popc++;
+        int lab = 0;
+        for (int d = 0; d < 4; ++d) {
+            const int s = p + dirs[d];
+            int t = 0;
+            const uint8_t cs = D.cls[s];
+            if (cs == C_SEED) t = D.m0[s];
+            else if (cs == C_F && D.st[s] == 2) t = D.lab[s];
+            if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+        }
+        D.lab[p] = lab;
+        D.st[p] = 2;
+        if (lab == WSHED) continue;
+        const uint8_t* ptr = D.img + 3 * (size_t)p;
+        const int bp = D.bnode[p];
+        for (int d = 0; d < 4; ++d) {
+            const int s = p + dirs[d];
+            const uint8_t cs = D.cls[s];
+            if (cs == C_F) {
+                if (D.st[s] != 0) continue;
+                const int l = c_diff(ptr, D.img + 3 * (size_t)s);
+                D.ilvl[s] = (uint8_t)l;
+                D.pseq[s] = seq++;
+                D.parent[s] = p;
+                D.bnode[s] = key_less(bp, s) ? s : bp;
+                D.st[s] = 1;
+                enq(l, s);
+            } else if (cs == C_B && D.lvl0[s] == NOPUSH) {
+                bool done = false;  // queued already by an earlier labelled neighbour of this run
+                for (int e = 0; e < 4; ++e) {
+                    const int f2 = s + dirs[e];
+                    if (f2 != p && D.cls[f2] == C_F && D.run[f2] == id && D.st[f2] == 2 && D.lab[f2] > 0) done = true;
+                }
+                if (done) continue;
+                const int slot = atomicAdd(D.nleaf, 1);
+                if (slot >= D.leafcap) { atomicOr(D.flags, FB_LEAF_CAP); continue; }
+                const int l = c_diff(ptr, D.img + 3 * (size_t)s);
+                D.lf_b[slot] = s;
+                D.lf_run[slot] = id;
+                D.lf_lvl[slot] = (uint8_t)l;
+                D.lf_pseq[slot] = seq++;
+                D.lf_parent[slot] = p;
+                D.lf_popidx[slot] = -1;
+                const int lid = D.N + slot;
+                D.lf_bnode[slot] = key_less(bp, lid) ? lid : bp;
+                D.lf_next[slot] = atomicExch(&D.head[s], slot);
+                enq(l, lid);
+            }
+        }
+    }
+}
+
+// ---- B items: the real queue entry of each B pixel and its label decision at pop time ----
+__global__ void k_bresolve(Dev D, int32_t* __restrict__ real, int* __restrict__ cand, int* __restrict__ ncand) {
+    const int W = D.W;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < D.N; b += gridDim.x * blockDim.x) {
+        real[b] = -1;
+        if (D.cls[b] != C_B) continue;
+        int r = -1;
+        if (D.lvl0[b] != NOPUSH) r = b;
+        else {
+            for (int sl = D.head[b]; sl >= 0; sl = D.lf_next[sl]) {
+                const int lid = D.N + sl;
+                if (r < 0 || before(D, D.lf_parent[sl], it_parent(D, r))) r = lid;
+            }
+        }
+        real[b] = r;
+        if (r < 0) continue;
+        const int dirs[4] = {-1, 1, -W, W};
+        int lab = 0;
+        for (int d = 0; d < 4; ++d) {
+            const int s = b + dirs[d];
+            int t = 0;
+            const uint8_t cs = D.cls[s];
+            if (cs == C_SEED) t = D.m0[s];
+            else if (cs == C_F && D.st[s] == 2 && D.lab[s] > 0 && before(D, s, r)) t = D.lab[s];
+            if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+        }
+        if (lab > 0) cand[atomicAdd(ncand, 1)] = b;
+    }
+}
+
+// info[0] = b1 pixel (-1 none), info[1] = b1 item, info[2] = Y1
+__global__ __launch_bounds__(1024) void k_b1_reduce(Dev D, const int32_t* __restrict__ real, const int* __restrict__ cand,
+                                                    const int* __restrict__ ncand, int* __restrict__ info) {
+    __shared__ int best[1024];
+    const int n = *ncand;
+    int mine = -1;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int b = cand[k];
+        if (mine < 0 || before(D, real[b], real[mine])) mine = b;
+    }
+    best[threadIdx.x] = mine;
+    __syncthreads();
+    for (int stride = 512; stride > 0; stride >>= 1) {
+        if ((int)threadIdx.x < stride) {
+            const int a = best[threadIdx.x], c = best[threadIdx.x + stride];
+            if (a < 0 || (c >= 0 && before(D, real[c], real[a]))) best[threadIdx.x] = c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int b1 = best[0];
+        info[0] = b1;
+        info[1] = b1 >= 0 ? real[b1] : -1;
+        int lab = 0;
+        if (b1 >= 0) {
+            const int W = D.W;
+            const int dirs[4] = {-1, 1, -W, W};
+            const int r = real[b1];
+            for (int d = 0; d < 4; ++d) {
+                const int s = b1 + dirs[d];
+                int t = 0;
+                const uint8_t cs = D.cls[s];
+                if (cs == C_SEED) t = D.m0[s];
+                else if (cs == C_F && D.st[s] == 2 && D.lab[s] > 0 && before(D, s, r)) t = D.lab[s];
+                if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+            }
+        }
+        info[2] = lab;
+        // a level-0 b1 could share the level-0 FIFO with other items: outside the decomposition
+        if (b1 >= 0 && it_lvl(D, real[b1]) == 0) atomicOr(D.flags, FB_TIES);
+    }
+}
+
+// queue state at T1 (after b1's pop); P pixels (B, never queued) -> pmask
+__global__ void k_t1_state(Dev D, const int32_t* __restrict__ real, const int* __restrict__ info, uint8_t* __restrict__ pmask,
+                           uint8_t* __restrict__ kind) {
+    const int b1 = info[0], b1i = info[1], Y1 = info[2];
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < D.N; p += gridDim.x * blockDim.x) {
+        uint8_t pm = 0, kd = 0;
+        const uint8_t c = D.cls[p];
+        if (c == C_F) {
+            int v = 0;
+            if (D.st[p] == 2 && (b1 < 0 || before(D, p, b1i))) v = D.lab[p];
+            else if (D.st[p] >= 1 && (D.parent[p] < 0 || b1 < 0 || before(D, D.parent[p], b1i))) { v = IN_QUEUE; kd = 1; }
+            D.m[p] = v;
+        } else if (c == C_B) {
+            const int r = real[p];
+            int v = 0;
+            if (p == b1) v = Y1;
+            else if (r >= 0) {
+                const int par = it_parent(D, r);
+                const bool queued = par < 0 || b1 < 0 || before(D, par, b1i);
+                if (queued) {
+                    if (b1 < 0 || before(D, r, b1i)) v = WSHED;
+                    else { v = IN_QUEUE; kd = 1; }
+                }
+            }
+            if (v == 0) pm = 1;
+            D.m[p] = v;
+        }
+        pmask[p] = pm;
+        kind[p] = kd;
+    }
+}
+
+// flood from b1: the never-queued B pixels it reaches are exactly those at finite geodesic
+// distance from b1 through never-queued B pixels; they take Y1.  An F pixel still unqueued next to
+// one of them (or to b1) is queued by that flood (kind 2) from its neighbour nearest to b1: the
+// FIFO position of that push is ordered first by this distance (the flood's layer).
+__global__ void k_geo_init(Dev D, const int* __restrict__ info, uint8_t* __restrict__ pmask, unsigned* __restrict__ dist) {
+    const int b1 = info[0];
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < D.N; p += gridDim.x * blockDim.x) {
+        const bool src = (p == b1);
+        if (src) pmask[p] = 1;
+        dist[p] = src ? 0u : 0xFFFFFFFFu;
+    }
+}
+__global__ void k_bfs_mark(Dev D, const int* __restrict__ info, const unsigned* __restrict__ dist, uint8_t* __restrict__ kind,
+                           int32_t* __restrict__ pdist) {
+    const int b1 = info[0], Y1 = info[2];
+    const int W = D.W;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < D.N; p += gridDim.x * blockDim.x) {
+        if (b1 < 0 || Y1 <= 0) continue;
+        if (D.cls[p] != C_F || D.m[p] != 0) continue;
+        const int dirs[4] = {-1, 1, -W, W};
+        unsigned best = 0xFFFFFFFFu;
+        for (int d = 0; d < 4; ++d) {
+            const int s = p + dirs[d];
+            if (D.cls[s] == C_B) best = min(best, dist[s]);
+        }
+        if (best != 0xFFFFFFFFu) { kind[p] = 2; pdist[p] = (int)best; }
+    }
+}
+__global__ void k_bfs_commit(Dev D, const int* __restrict__ info, const unsigned* __restrict__ dist, const uint8_t* __restrict__ kind,
+                             uint8_t* __restrict__ umask) {
+    const int b1 = info[0], Y1 = info[2];
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < D.N; p += gridDim.x * blockDim.x) {
+        int v = D.m[p];
+        const uint8_t c = D.cls[p];
+        if (b1 >= 0 && Y1 > 0 && c == C_B && v == 0 && dist[p] != 0xFFFFFFFFu) { v = Y1; D.m[p] = v; }
+        if (kind[p] == 2) { v = IN_QUEUE; D.m[p] = v; }
+        umask[p] = (c != C_FRAME && c != C_SEED && (v == 0 || v == IN_QUEUE)) ? 1 : 0;
+    }
+}
+
+// exclusive scan of n ints, single block
+__global__ __launch_bounds__(1024) void k_scan_block(const int32_t* __restrict__ in, int32_t* __restrict__ out, int n) {
+    __shared__ int part[1024];
+    const int per = (n + 1023) / 1024;
+    const int start = threadIdx.x * per;
+    int acc = 0;
+    for (int k = 0; k < per && start + k < n; ++k) acc += in[start + k];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int k = 0; k < 1024; ++k) { const int v = part[k]; part[k] = run; run += v; }
+    }
+    __syncthreads();
+    int run = part[threadIdx.x];
+    for (int k = 0; k < per && start + k < n; ++k) { out[start + k] = run; run += in[start + k]; }
+}
+
+// ---- phase 2: one lane floods one component of unresolved pixels ----
+struct P2 {
+    const int32_t* comp;
+    const int32_t* box5;
+    const int32_t* area;
+    const int32_t* off;
+    const uint8_t* kind;
+    int32_t* scratch;  // 3 * area per component: item list, heap, saved labels
+    int ncomp;
+};
+
+// order of two queued items of the same level by push time: phase-1 items by pusher order,
+// items queued by the flood from b1 after them (their mutual order is not known: returns
+// false both ways, the caller checks that it does not matter)
+__device__ bool push_before(const Dev& D, const int32_t* real, const uint8_t* kind, int a, int b) {
+    const bool ba = kind[a] == 2, bb = kind[b] == 2;
+    if (ba != bb) return bb;
+    if (ba) {  // the b1 flood's layer first; equal layers: canonical raster order (order unknown)
+        if (D.lab[a] != D.lab[b]) return D.lab[a] < D.lab[b];
+        return a < b;
+    }
+    const int ia = D.cls[a] == C_B ? real[a] : a;
+    const int ib = D.cls[b] == C_B ? real[b] : b;
+    const int pa = it_parent(D, ia), pb = it_parent(D, ib);
+    if (pa < 0 && pb < 0) return ia < ib;
+    if (pa < 0) return true;
+    if (pb < 0) return false;
+    if (pa == pb) return it_pseq(D, ia) < it_pseq(D, ib);
+    return before(D, pa, pb);
+}
+
+__device__ int item_level(const Dev& D, const int32_t* real, const uint8_t* kind, int p) {
+    if (kind[p] == 2) {
+        const uint8_t* c = D.img + 3 * (size_t)p;
+        return max(max((int)c[0], (int)c[1]), (int)c[2]);
+    }
+    return D.cls[p] == C_B ? it_lvl(D, real[p]) : D.ilvl[p];
+}
+
+// binary heap on (level << 24 | seq) packed keys, values = pixel ids
+__device__ __forceinline__ void heap_push(long long* hk, int* hv, int& n, long long k, int v) {
+    int i = n++;
+    while (i > 0) {
+        const int pi = (i - 1) >> 1;
+        if (hk[pi] <= k) break;
+        hk[i] = hk[pi]; hv[i] = hv[pi]; i = pi;
+    }
+    hk[i] = k; hv[i] = v;
+}
+__device__ __forceinline__ int heap_pop(long long* hk, int* hv, int& n) {
+    const int top = hv[0];
+    const long long k = hk[--n];
+    const int v = hv[n];
+    int i = 0;
+    while (true) {
+        int c = 2 * i + 1;
+        if (c >= n) break;
+        if (c + 1 < n && hk[c + 1] < hk[c]) ++c;
+        if (hk[c] >= k) break;
+        hk[i] = hk[c]; hv[i] = hv[c]; i = c;
+    }
+    if (n > 0) { hk[i] = k; hv[i] = v; }
+    return top;
+}
+
+constexpr int MAXG = 8, MAXGL = 5, MAXCOMBO = 4096;
+struct TieRec {
+    int c, ni, ntie, ncombo;
+    long long base;  // offset (ints) of the record's region in the tie pool
+    int gfirst[MAXG], glen[MAXG];
+};
+
+__device__ __forceinline__ int fact(int n) { int f = 1; for (int k = 2; k <= n; ++k) f *= k; return f; }
+
+// flood one component from its sorted queue `order` (ni items); the state of the component's
+// pixels lives either in D.m (st == nullptr) or in a private bbox-indexed array st.  Returns
+// false on heap overflow.
+__device__ bool flood_component(const Dev& D, const P2& Q, int c, const int* order, int ni, int* st, int* hv,
+                                long long* hk, int A, const int32_t* real) {
+    const int W = D.W, id = c + 1;
+    const int minx = Q.box5[Q.ncomp + c], maxx = Q.box5[2 * Q.ncomp + c];
+    const int miny = Q.box5[3 * Q.ncomp + c];
+    const int bw = maxx - minx + 1;
+    auto loc = [&](int i) -> int* {  // private cell of a component pixel, nullptr otherwise
+        if (!st) return nullptr;
+        const int x = i % W, y = i / W;
+        if (x < minx || x > maxx || y < miny || Q.comp[i] != id) return nullptr;
+        return st + (y - miny) * bw + (x - minx);
+    };
+    auto get = [&](int i) { int* q = loc(i); return q ? *q : D.m[i]; };
+    auto put = [&](int i, int v) { int* q = loc(i); if (q) *q = v; else D.m[i] = v; };
+    const int dirs[4] = {-1, 1, -W, W};
+    int hn = 0;
+    long long seqc = 0;
+    for (int k = 0; k < ni; ++k)
+        heap_push(hk, hv, hn, ((long long)item_level(D, real, Q.kind, order[k]) << 40) | (seqc++), order[k]);
+    while (hn > 0) {
+        const int p = heap_pop(hk, hv, hn);
+        int lab = 0;
+        for (int d = 0; d < 4; ++d) {
+            const int t = get(p + dirs[d]);
+            if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+        }
+        put(p, lab);
+        if (lab == WSHED) continue;
+        const uint8_t* ptr = D.img + 3 * (size_t)p;
+        for (int d = 0; d < 4; ++d) {
+            const int s2 = p + dirs[d];
+            if (get(s2) == 0) {
+                if (hn >= A) return false;
+                const int l = c_diff(ptr, D.img + 3 * (size_t)s2);
+                heap_push(hk, hv, hn, ((long long)l << 40) | (seqc++), s2);
+                put(s2, IN_QUEUE);
+            }
+        }
+    }
+    return true;
+}
+
+__global__ void k_phase2(Dev D, P2 Q, const int32_t* __restrict__ real, long long* __restrict__ heapk, TieRec* ties,
+                         int* nties, int tiecap, long long* pool_used, long long poolcap) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Q.ncomp) return;
+    const int W = D.W;
+    const int id = c + 1;
+    const int A = Q.area[c];
+    int32_t* items = Q.scratch + 3 * (size_t)Q.off[c];
+    int* hv = items + A;
+    long long* hk = heapk + (size_t)Q.off[c];
+    const int minx = Q.box5[Q.ncomp + c], maxx = Q.box5[2 * Q.ncomp + c];
+    const int miny = Q.box5[3 * Q.ncomp + c], maxy = Q.box5[4 * Q.ncomp + c];
+    // collect queued items
+    int ni = 0;
+    for (int y = miny; y <= maxy; ++y)
+        for (int x = minx; x <= maxx; ++x) {
+            const int i = y * W + x;
+            if (Q.comp[i] == id && D.m[i] == IN_QUEUE) items[ni++] = i;
+        }
+    // insertion sort by (level, push time)
+    for (int a = 1; a < ni; ++a) {
+        const int v = items[a];
+        const int lv = item_level(D, real, Q.kind, v);
+        int b = a - 1;
+        while (b >= 0) {
+            const int u = items[b];
+            const int lu = item_level(D, real, Q.kind, u);
+            const bool gt = (lu > lv) || (lu == lv && push_before(D, real, Q.kind, v, u));
+            if (!gt) break;
+            items[b + 1] = u;
+            --b;
+        }
+        items[b + 1] = v;
+    }
+    // groups of items queued by the b1 flood at the same level and layer: order unknown
+    TieRec R;
+    R.c = c; R.ni = ni; R.ntie = 0;
+    long long ncombo = 1;
+    for (int a = 0; a < ni;) {
+        const int v = items[a];
+        if (Q.kind[v] != 2) { ++a; continue; }
+        const int lv = item_level(D, real, Q.kind, v);
+        int e = a + 1;
+        while (e < ni && Q.kind[items[e]] == 2 && item_level(D, real, Q.kind, items[e]) == lv &&
+               D.lab[items[e]] == D.lab[v])
+            ++e;
+        if (e - a >= 2) {
+            if (R.ntie == MAXG || e - a > MAXGL) { atomicOr(D.flags, FB_TIES); return; }
+            R.gfirst[R.ntie] = a;
+            R.glen[R.ntie] = e - a;
+            ++R.ntie;
+            ncombo *= fact(e - a);
+            if (ncombo > MAXCOMBO) { atomicOr(D.flags, FB_TIES); return; }
+        }
+        a = e;
+    }
+    if (R.ntie == 0) {
+        if (!flood_component(D, Q, c, items, ni, nullptr, hv, hk, A, real)) atomicOr(D.flags, FB_HEAP);
+        return;
+    }
+    // defer to the parallel permutation check (one wave per component)
+    R.ncombo = (int)ncombo;
+    const int B = (maxx - minx + 1) * (maxy - miny + 1);
+    const long long per = (long long)B + ni + A + 2LL * A + 4;
+    const long long need = 64 * per + B + 64;
+    const long long base = atomicAdd((unsigned long long*)pool_used, (unsigned long long)need);
+    const int k = atomicAdd(nties, 1);
+    if (k >= tiecap) { atomicOr(D.flags, FB_TIES); return; }
+    if (base + need > poolcap) {  // slot taken but no room: leave an empty record
+        atomicOr(D.flags, FB_TIES);
+        R.c = -1;
+    }
+    R.base = base;
+    ties[k] = R;
+}
+
+// one wave per deferred component: lane t floods the arrangements t, t+64, ... of the tie groups
+// on a private copy; every result must equal arrangement 0, which is then committed
+__global__ __launch_bounds__(64) void k_phase2_ties(Dev D, P2 Q, const int32_t* __restrict__ real, const TieRec* __restrict__ ties,
+                                                   const int* __restrict__ nties, int* pool) {
+    const int k = blockIdx.x;
+    if (k >= min(*nties, (int)gridDim.x)) return;
+    const TieRec R = ties[k];
+    if (R.c < 0) return;
+    const int c = R.c, id = c + 1, W = D.W, lane = threadIdx.x;
+    const int A = Q.area[c];
+    const int minx = Q.box5[Q.ncomp + c], maxx = Q.box5[2 * Q.ncomp + c];
+    const int miny = Q.box5[3 * Q.ncomp + c], maxy = Q.box5[4 * Q.ncomp + c];
+    const int bw = maxx - minx + 1, bh = maxy - miny + 1, B = bw * bh;
+    const int ni = R.ni;
+    const int32_t* items = Q.scratch + 3 * (size_t)Q.off[c];
+    const long long per = (long long)B + ni + A + 2LL * A + 4;
+    int* mine = pool + R.base + lane * per;
+    int* st = mine;
+    int* order = st + B;
+    int* hv = order + ni;
+    long long* hk = reinterpret_cast<long long*>(hv + A + ((reinterpret_cast<uintptr_t>(hv + A) & 7) ? 1 : 0));
+    int* canon = pool + R.base + 64 * per;
+    __shared__ int bad;
+    if (lane == 0) bad = 0;
+    __syncthreads();
+    const int rounds = (R.ncombo + 63) / 64;
+    for (int r = 0; r < rounds; ++r) {
+        const int combo = r * 64 + lane;
+        const bool act = combo < R.ncombo;
+        if (act) {
+            for (int q = 0; q < ni; ++q) order[q] = items[q];
+            int code = combo;
+            for (int g = 0; g < R.ntie; ++g) {  // arrangement = factorial-base digits per group
+                const int L = R.glen[g];
+                const int f = fact(L);
+                int digit = code % f;
+                code /= f;
+                int pool_el[MAXGL];
+                for (int q = 0; q < L; ++q) pool_el[q] = items[R.gfirst[g] + q];
+                for (int q = 0; q < L; ++q) {
+                    const int fq = fact(L - 1 - q);
+                    const int idx = digit / fq;
+                    digit %= fq;
+                    order[R.gfirst[g] + q] = pool_el[idx];
+                    for (int z = idx; z < L - 1 - q; ++z) pool_el[z] = pool_el[z + 1];
+                }
+            }
+            for (int y = 0; y < bh; ++y)
+                for (int x = 0; x < bw; ++x) st[y * bw + x] = 0;
+            for (int q = 0; q < ni; ++q) {
+                const int i = items[q];
+                st[(i / W - miny) * bw + (i % W - minx)] = IN_QUEUE;
+            }
+            if (!flood_component(D, Q, c, order, ni, st, hv, hk, A, real)) atomicOr(&bad, 1);
+        }
+        if (r == 0) {
+            if (lane == 0)
+                for (int q = 0; q < B; ++q) canon[q] = st[q];
+            __syncthreads();
+        }
+        if (act) {
+            for (int y = 0; y < bh; ++y)
+                for (int x = 0; x < bw; ++x) {
+                    const int i = (miny + y) * W + (minx + x);
+                    if (Q.comp[i] == id && st[y * bw + x] != canon[y * bw + x]) atomicOr(&bad, 1);
+                }
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        if (bad) { atomicOr(D.flags, FB_TIEPERM); return; }
+        for (int y = 0; y < bh; ++y)
+            for (int x = 0; x < bw; ++x) {
+                const int i = (miny + y) * W + (minx + x);
+                if (Q.comp[i] == id) D.m[i] = canon[y * bw + x];
+            }
+    }
+}
+
+inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u)); }
+
+}  // namespace
+
+// Returns NSC_OK with *ok = 1 when the decomposition produced the exact result in m, *ok = 0
+// when the caller must run the sequential flood instead (m is then undefined).
+void ws_last_flags_record(int flags);
+
+int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t s);
+
+int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, int* ok, hipStream_t s) {
+    const size_t N = (size_t)W * H;
+    const bool dbg = getenv("NSC_WS_DEBUG") != nullptr;
+    hipEvent_t ev0 = nullptr;
+    auto tick = [&](const char* what, const int* dcount) {
+        if (!dbg) return;
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
+        (void)hipEventSynchronize(e);
+        float ms = 0.f;
+        if (ev0) (void)hipEventElapsedTime(&ms, ev0, e);
+        int c = -1;
+        if (dcount) (void)hipMemcpy(&c, dcount, sizeof(int), hipMemcpyDeviceToHost);
+        fprintf(stderr, "[ws] %-12s %8.3f ms  count %d\n", what, ms, c);
+        if (ev0) (void)hipEventDestroy(ev0);
+        ev0 = e;
+    };
+    tick("start", nullptr);
+    *ok = 0;
+    if (N >= (size_t)INT_MAX / 8) return NSC_OK;
+    Carver measure(nullptr);
+    auto carve = [&](Carver& c, uint8_t*& cls, uint16_t*& lvl0, uint8_t*& fmask, int32_t*& run, uint8_t*& st,
+                     uint8_t*& ilvl, int32_t*& pseq, int32_t*& parent, int32_t*& bnode, int32_t*& popidx, int32_t*& lab,
+                     int32_t*& head, int32_t*& real, int32_t*& lab2, uint8_t*& kind, int*& small) {
+        cls = c.take<uint8_t>(N);
+        lvl0 = c.take<uint16_t>(N);
+        fmask = c.take<uint8_t>(N);
+        run = c.take<int32_t>(N);
+        st = c.take<uint8_t>(N);
+        ilvl = c.take<uint8_t>(N);
+        pseq = c.take<int32_t>(N);
+        parent = c.take<int32_t>(N);
+        bnode = c.take<int32_t>(N);
+        popidx = c.take<int32_t>(N);
+        lab = c.take<int32_t>(N);
+        head = c.take<int32_t>(N);
+        real = c.take<int32_t>(N);
+        lab2 = c.take<int32_t>(N);
+        kind = c.take<uint8_t>(N);
+        small = c.take<int>(64);
+    };
+    uint8_t *cls, *fmask, *st, *ilvl, *kind;
+    uint16_t* lvl0;
+    int32_t *run, *pseq, *parent, *bnode, *popidx, *lab, *head, *real, *lab2;
+    int* small;
+    carve(measure, cls, lvl0, fmask, run, st, ilvl, pseq, parent, bnode, popidx, lab, head, real, lab2, kind, small);
+    Scratch sc;
+    NSC_CHECK(sc.alloc(measure.size(), s));
+    Carver cut(sc.p);
+    carve(cut, cls, lvl0, fmask, run, st, ilvl, pseq, parent, bnode, popidx, lab, head, real, lab2, kind, small);
+    // small[]: 0 nF, 1 nrun, 2 nleaf, 3 flags, 4 ncand, 8..10 info, 16 ncomp, 20 deferred tie components,
+    // 24..25 tie pool bump pointer (long long)
+    NSC_HIP(hipMemsetAsync(small, 0, 64 * sizeof(int), s));
+
+    Dev D{};
+    D.img = img; D.W = W; D.H = H; D.N = (int)N; D.m0 = m0; D.m = m; D.cls = cls; D.lvl0 = lvl0; D.run = run;
+    D.st = st; D.ilvl = ilvl; D.pseq = pseq; D.parent = parent; D.bnode = bnode; D.popidx = popidx; D.lab = lab;
+    D.head = head; D.nleaf = &small[2]; D.flags = &small[3];
+    hipLaunchKernelGGL(k_classify, grid1(N), dim3(256), 0, s, D, cls, lvl0, fmask, &small[0]);
+    tick("classify", &small[0]);
+    NSC_CHECK(ccl_neq(fmask, 0, run, W, H, 4, s));
+    NSC_CHECK(relabel_impl(run, W, H, -1, &small[1], lab2, s));
+    tick("runs", &small[1]);
+    int hostc[2];
+    NSC_HIP(hipMemcpyAsync(hostc, small, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    NSC_HIP(hipStreamSynchronize(s));
+    const int nF = hostc[0], nrun = hostc[1];
+    // phase-1 scratch: run boxes, per-run FIFO heads/tails, queue links, leaf table
+    const int leafcap = std::max(1024, std::min((int)std::min(N, (size_t)INT_MAX / 4), 4 * nF + 1024));
+    Carver m2(nullptr);
+    auto carve2 = [&](Carver& c, int32_t*& box5, int*& qh, int*& qt, int32_t*& qnext, int32_t*& lf) {
+        box5 = c.take<int32_t>(5 * (size_t)std::max(nrun, 1));
+        qh = c.take<int>((size_t)std::max(nrun, 1) * NQ);
+        qt = c.take<int>((size_t)std::max(nrun, 1) * NQ);
+        qnext = c.take<int32_t>(N + (size_t)leafcap);
+        lf = c.take<int32_t>((size_t)leafcap * 8);
+    };
+    int32_t *box5, *qnext, *lf;
+    int *qh, *qt;
+    carve2(m2, box5, qh, qt, qnext, lf);
+    Scratch sc2;
+    NSC_CHECK(sc2.alloc(m2.size(), s));
+    Carver cut2(sc2.p);
+    carve2(cut2, box5, qh, qt, qnext, lf);
+    D.qnext = qnext;
+    D.leafcap = leafcap;
+    D.lf_b = lf;
+    D.lf_run = lf + (size_t)leafcap;
+    D.lf_pseq = lf + 2 * (size_t)leafcap;
+    D.lf_parent = lf + 3 * (size_t)leafcap;
+    D.lf_bnode = lf + 4 * (size_t)leafcap;
+    D.lf_popidx = lf + 5 * (size_t)leafcap;
+    D.lf_next = lf + 6 * (size_t)leafcap;
+    D.lf_lvl = reinterpret_cast<uint8_t*>(lf + 7 * (size_t)leafcap);
+    if (nrun > 0) {
+        NSC_CHECK(bbox_impl(run, W, H, nrun, box5, nullptr, s));
+        hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, s, D, box5, nrun, qh, qt);
+    }
+    tick("phase1", &small[2]);
+    // B resolution and b1
+    int* cand = lab2;  // reuse (N ints)
+    hipLaunchKernelGGL(k_bresolve, grid1(N), dim3(256), 0, s, D, real, cand, &small[4]);
+    tick("bresolve", &small[4]);
+    hipLaunchKernelGGL(k_b1_reduce, dim3(1), dim3(1024), 0, s, D, real, cand, &small[4], &small[8]);
+    tick("b1", &small[8]);
+    hipLaunchKernelGGL(k_t1_state, grid1(N), dim3(256), 0, s, D, real, &small[8], fmask, kind);
+    // flood from b1 over the never-queued B pixels (fmask marks them): geodesic distance from b1
+    {
+        unsigned* dist = reinterpret_cast<unsigned*>(head);  // head[] is dead after k_bresolve
+        hipLaunchKernelGGL(k_geo_init, grid1(N), dim3(256), 0, s, D, &small[8], fmask, dist);
+        NSC_CHECK(geodist_impl(fmask, dist, W, H, s));
+        tick("geodist", nullptr);
+        hipLaunchKernelGGL(k_bfs_mark, grid1(N), dim3(256), 0, s, D, &small[8], dist, kind, lab /* pdist; lab[] is dead */);
+        hipLaunchKernelGGL(k_bfs_commit, grid1(N), dim3(256), 0, s, D, &small[8], dist, kind, fmask);
+    }
+    // phase 2
+    NSC_CHECK(ccl_neq(fmask, 0, lab2, W, H, 4, s));
+    NSC_CHECK(relabel_impl(lab2, W, H, -1, &small[16], head /* rank scratch: head[] is dead after k_bresolve */, s));
+    int ncomp = 0;
+    NSC_HIP(hipMemcpyAsync(&ncomp, &small[16], sizeof(int), hipMemcpyDeviceToHost, s));
+    NSC_HIP(hipStreamSynchronize(s));
+    if (ncomp > 0) {
+        Scratch sc3;
+        Carver m3(nullptr);
+        auto carve3 = [&](Carver& c, int32_t*& b5, int32_t*& area, int32_t*& off, int32_t*& scr, long long*& hk) {
+            b5 = c.take<int32_t>(5 * (size_t)ncomp);
+            area = c.take<int32_t>(ncomp);
+            off = c.take<int32_t>(ncomp);
+            scr = c.take<int32_t>(3 * N);
+            hk = c.take<long long>(N);
+        };
+        int32_t *b5, *area, *off, *scr;
+        long long* hk;
+        carve3(m3, b5, area, off, scr, hk);
+        NSC_CHECK(sc3.alloc(m3.size(), s));
+        Carver cut3(sc3.p);
+        carve3(cut3, b5, area, off, scr, hk);
+        NSC_CHECK(bbox_impl(lab2, W, H, ncomp, b5, area, s));
+        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(1024), 0, s, area, off, ncomp);
+        P2 Q{lab2, b5, area, off, kind, scr, ncomp};
+        tick("p2 setup", &small[16]);
+        const int tiecap = 4096;
+        const long long poolcap = std::max<long long>(16ll << 20, 4ll * (long long)N);  // ints
+        Scratch sc4;
+        NSC_CHECK(sc4.alloc(sizeof(TieRec) * tiecap + sizeof(int) * poolcap + 256, s));
+        TieRec* ties = reinterpret_cast<TieRec*>(sc4.p);
+        int* pool = reinterpret_cast<int*>(reinterpret_cast<char*>(sc4.p) + ((sizeof(TieRec) * tiecap + 255) & ~size_t(255)));
+        long long* pool_used = reinterpret_cast<long long*>(&small[24]);  // 8-byte aligned: small is 256-aligned
+        hipLaunchKernelGGL(k_phase2, dim3(cdiv(ncomp, 64)), dim3(64), 0, s, D, Q, real, hk, ties, &small[20], tiecap,
+                           pool_used, poolcap);
+        tick("phase2", &small[20]);
+        hipLaunchKernelGGL(k_phase2_ties, dim3(tiecap), dim3(64), 0, s, D, Q, real, ties, &small[20], pool);
+        tick("ties", &small[3]);
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    int flags = 0;
+    NSC_HIP(hipGetLastError());
+    NSC_HIP(hipMemcpyAsync(&flags, &small[3], sizeof(int), hipMemcpyDeviceToHost, s));
+    NSC_HIP(hipStreamSynchronize(s));
+    *ok = (flags == 0) ? 1 : 0;
+    ws_last_flags_record(flags);
+    return NSC_OK;
+}
+
+}  // namespace nsc

@@ -245,3 +245,10 @@ def segment_nuclei(bgr, params=None, ws=None, stages=(), timing=False):
         from nscale_amd._lib import check
         check(st, "nsc_segment_nuclei")
     return st, labels, cc.value, bufs, {i: ms[i] for i in range(32) if ms[i] > 0}
+
+
+def watershed_stats(with_reason=False):
+    """(decomposed-flood calls, sequential-fallback calls[, last fallback reason bits])."""
+    a, b, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    lib().nsc_watershed_stats(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return (a.value, b.value, c.value) if with_reason else (a.value, b.value)

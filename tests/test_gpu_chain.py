@@ -69,3 +69,21 @@ def test_chain_rbc_branch():
     assert np.array_equal(bufs[2], obufs[2])
     if st == 0:
         assert np.array_equal(lab, olab)
+
+
+def test_watershed_decomposition_used_and_equal_to_sequential(monkeypatch):
+    """The chain's watershed runs on the decomposed parallel flood (no fallback), and the
+    sequential per-component flood gives the same bytes on the same input."""
+    from nscale_amd import gpu
+    from nscale_amd.synth import synth_he
+    bgr = torch.from_numpy(synth_he(1024, 1024, 11)).cuda()
+    f0, b0 = gpu.watershed_stats()
+    st, lab, cc, bufs, _ = gpu.segment_nuclei(bgr, stages=(20,))
+    torch.cuda.synchronize()
+    f1, b1 = gpu.watershed_stats()
+    assert f1 == f0 + 1 and b1 == b0, "decomposed flood not used"
+    monkeypatch.setenv("NSC_WATERSHED", "seq")
+    st2, lab2, cc2, bufs2, _ = gpu.segment_nuclei(bgr, stages=(20,))
+    torch.cuda.synchronize()
+    assert st == st2 == 0
+    assert torch.equal(bufs[20], bufs2[20]) and torch.equal(lab, lab2)

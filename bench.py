@@ -7,9 +7,13 @@ chain call on that tile with the input already resident in HBM.
 Prints ONE JSON line: segmented Mpix/s (input pixels / wall time), p50/p99 tile latency, the
 roofline of the imreconstruct<uchar> kernel inside the chain (3 B/px algorithmic: read marker,
 read mask, write output) and the CPU baseline (the oracle chain on the host cores, bounded
-sample).  --mode batched runs B copies of the tile through the batched entry point.
+sample).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size S] [--no-cpu]
+--workload recon times imreconstruct<uchar> alone (8-connectivity) on the 4096x4096 grayscale
+ellipse tile (background 220, ellipses U[60,120], noise sigma 8, marker = mask - 10 saturating):
+the profiling workload for the reconstruction kernel.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size S] [--no-cpu] [--workload chain|recon]
 """
 import argparse
 import ctypes
@@ -67,14 +71,67 @@ def cpu_baseline(size, seconds_budget=20.0):
             "one_core_mpix_s": round(one_core, 3)}
 
 
+def bench_recon(args):
+    import torch
+    from nscale_amd import gpu
+    from nscale_amd.synth import synth_mask, synth_marker
+
+    S = args.size
+    mask_h = synth_mask(S, S, 2026)
+    mask = torch.from_numpy(mask_h).cuda()
+    marker = torch.from_numpy(synth_marker(mask_h)).cuda()
+    for _ in range(args.warmup):
+        out = gpu.imreconstruct(marker, mask, 8)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        out = gpu.imreconstruct(marker, mask, 8)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = sorted(a.elapsed_time(b) for a, b in evs)
+    avg = sum(ms) / len(ms)
+    achieved = 3 * S * S / (avg * 1e-3) / 1e9
+    out = {"metric": "imreconstruct<uchar> Mpix/s", "value": round(args.steps * S * S / wall / 1e6, 3),
+           "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+           "config": {"workload": "imreconstruct<uchar> conn 8, %dx%d ellipse mask, marker = mask - 10" % (S, S)},
+           "p50_ms": round(statistics.median(ms), 4),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_ms": round(avg, 4)}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_copy(args):
+    """Calibration leg for HBM byte counters: a plain device copy of a known size (64 MiB)."""
+    import torch
+    n = 64 << 20
+    a = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print(json.dumps({"metric": "device copy GB/s", "value": round(2 * n * args.steps / wall / 1e9, 1), "unit": "GB/s",
+                      "bytes_read_per_call": n, "bytes_written_per_call": n, "steps": args.steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--cpu-size", type=int, default=2048)
+    ap.add_argument("--cpu-size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("chain", "recon", "copy"), default="chain")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the segmentation path has no multi-GPU component in this build "
@@ -85,6 +142,10 @@ def main():
     from nscale_amd import gpu
     from nscale_amd.synth import synth_he
 
+    if args.workload == "recon":
+        return bench_recon(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     S = args.size
     bgr = torch.from_numpy(synth_he(S, S, 2026)).cuda()
     ws = gpu.Workspace(S, S)

@@ -48,3 +48,31 @@ def test_errors_are_reported_not_thrown():
     rc = L.nsc_imreconstruct_u8(None, None, None, 0, 0, 8, None)
     assert rc == -10
     assert b"invalid argument" in L.nsc_last_error()
+
+
+def test_cpp_operator_surface_compiles_and_links(tmp_path):
+    """include/nscale_amd.hpp (the nscale:: C++ names over the C ABI) compiles with the host
+    compiler and links against the library; the program makes no GPU call."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None or not os.path.isdir("/opt/rocm/include"):
+        import pytest
+        pytest.skip("host compiler or ROCm headers absent")
+    src = tmp_path / "shim.cpp"
+    src.write_text(
+        '#include "nscale_amd.hpp"\n#include <cstdio>\n'
+        "int main() {\n"
+        "  nsc_seg_params p = nscale::hip::HistologicalEntities::defaults();\n"
+        "  nscale::hip::Image<const uint8_t> a(nullptr, 0, 0);\n"
+        "  nscale::hip::Image<uint8_t> o(nullptr, 0, 0);\n"
+        "  try { nscale::hip::imreconstruct<uint8_t>(a, a, o, 8, nullptr); return 2; }\n"
+        "  catch (const nscale::hip::Error& e) { std::printf(\"%s|%d|%d\\n\", nsc_version(), e.code, p.minSize); }\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "shim"
+    libdir = os.path.join(ROOT, "nscale_amd")
+    subprocess.run(["g++", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+                    "-I/opt/rocm/include", str(src), "-o", str(exe), "-L", libdir, "-lnscale_hip",
+                    "-Wl,-rpath," + libdir, "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.strip().split("|")
+    assert out[1] == "-10" and int(out[2]) > 0

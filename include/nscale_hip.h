@@ -157,6 +157,51 @@ int nsc_workspace_destroy(nsc_workspace* ws);
 int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p, int32_t* labels,
                        int* compcount, nsc_workspace* ws, void** stage, double* stage_ms, void* stream);
 
+/* ---- continuous-batching tile server --------------------------------------------------- */
+/* Replaces the reference's one-tile-per-MPI-process scale-out of segmentNuclei
+   (nu-segment.cpp:274-433, nu-segmentTest.cpp:207): requests of any tile size are admitted when a
+   slot (worker + stream + device workspace for max_w x max_h pixels, either orientation) is free
+   and their device-memory footprint (nsc_request_bytes) fits the HBM budget left after the slot
+   workspaces; admitted requests run concurrently on their own streams.  Outputs go to the
+   caller's device buffers; the caller keeps inputs and outputs alive until the request is
+   waited on or polled complete. */
+typedef struct nsc_server nsc_server;
+typedef struct nsc_server_config {
+    int slots;            /* concurrent chains, 1..64 */
+    size_t hbm_budget;    /* device bytes for slot workspaces plus in-flight requests */
+    int max_w, max_h;     /* largest tile (w*h pixels; either orientation) */
+} nsc_server_config;
+typedef struct nsc_request {
+    const uint8_t* bgr;   /* BGR tile: device pointer, or host pointer when host_input != 0 */
+    int w, h;
+    int host_input;
+    int32_t* labels;      /* device int32 w*h: the chain's labels (smallest raster index, bg -1) */
+    int canonical;        /* != 0: relabel to 1..N in raster order of first pixel, bg 0 */
+    int32_t* bbox5;       /* optional device n x 5 [label|minx|maxx|miny|maxy] (implies canonical) */
+    int32_t* area;        /* optional device per-label pixel count (with bbox5) */
+    int max_labels;       /* capacity of bbox5 / area in labels */
+    const nsc_seg_params* params; /* NULL = the reference's defaults */
+} nsc_request;
+typedef struct nsc_result {
+    int status;           /* chain status code */
+    int compcount;        /* nuclei found */
+    int error;            /* NSC_OK or NSC_ERR_* */
+    int slot;             /* the slot that ran it */
+    double queue_ms, run_ms, total_ms;  /* submit -> start -> end, host clock */
+    double start_ms, end_ms;            /* start and end, ms since the server was created */
+} nsc_result;
+size_t nsc_request_bytes(int w, int h);
+int nsc_server_create(const nsc_server_config* cfg, nsc_server** server);
+int nsc_server_submit(nsc_server* server, const nsc_request* req, int64_t* ticket);
+/* block until the request is done; returns its error code (NSC_OK) and fills *res */
+int nsc_server_wait(nsc_server* server, int64_t ticket, nsc_result* res);
+/* 1 while pending; otherwise like nsc_server_wait */
+int nsc_server_poll(nsc_server* server, int64_t ticket, nsc_result* res);
+int nsc_server_stats(nsc_server* server, int* queued, int* running, size_t* inflight_bytes,
+                     size_t* request_budget);
+/* drains every submitted request, then frees the slots */
+int nsc_server_destroy(nsc_server* server);
+
 #ifdef __cplusplus
 }
 #endif

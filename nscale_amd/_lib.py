@@ -27,6 +27,21 @@ class SegParams(ctypes.Structure):
                 ("fillHolesConnectivity", _I), ("reconConnectivity", _I), ("watershedConnectivity", _I)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", _I), ("hbm_budget", ctypes.c_size_t), ("max_w", _I), ("max_h", _I)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("bgr", _P), ("w", _I), ("h", _I), ("host_input", _I), ("labels", _P), ("canonical", _I),
+                ("bbox5", _P), ("area", _P), ("max_labels", _I), ("params", _P)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", _I), ("compcount", _I), ("error", _I), ("slot", _I),
+                ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("start_ms", ctypes.c_double), ("end_ms", ctypes.c_double)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -38,6 +53,15 @@ def lib():
         L.nsc_synth_mask.argtypes = [_P, _I, _I, ctypes.c_uint64]
         L.nsc_synth_he.argtypes = [_P, _I, _I, ctypes.c_uint64]
         L.nsc_synth_marker.argtypes = [_P, _P, ctypes.c_size_t]
+        L.nsc_request_bytes.restype = ctypes.c_size_t
+        L.nsc_request_bytes.argtypes = [_I, _I]
+        L.nsc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]
+        L.nsc_server_submit.argtypes = [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_int64)]
+        L.nsc_server_wait.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(Result)]
+        L.nsc_server_poll.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(Result)]
+        L.nsc_server_stats.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_size_t),
+                                       ctypes.POINTER(ctypes.c_size_t)]
+        L.nsc_server_destroy.argtypes = [_P]
         _lib = L
     return _lib
 

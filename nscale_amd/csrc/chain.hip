@@ -128,6 +128,7 @@ __global__ void k_nonoverlap(const uint8_t* __restrict__ seg_big, const int32_t*
 // ------------------------------------------------------------------------------------------
 struct Workspace {
     int max_w = 0, max_h = 0;
+    size_t n_cap = 0, big_bytes = 0;  // pixel capacity of the planes, bytes of the shared scratch
     void* base = nullptr;
     size_t bytes = 0;
     // u8 planes
@@ -140,12 +141,24 @@ struct Workspace {
     int* counts;  // [0] background, [1] compcount S1, [2] compcount final-area, [3] ncomp
 };
 
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+static size_t big_scratch(int w, int h) { return al256(std::max(local_max_scratch(w, h), ws2_scratch(w, h))); }
+
+size_t workspace_bytes(int mw, int mh) {
+    const size_t n = (size_t)mw * mh;
+    return 23 * al256(n) + al256(3 * n) + 8 * al256(4 * n) + big_scratch(mw, mh) + 256;
+}
+
+// a tile fits a workspace when its planes and its shared scratch fit (orientation-free)
+bool workspace_fits(const nsc_workspace* ws, int w, int h);
+
 static int ws_alloc(Workspace& W, int mw, int mh) {
     const size_t n = (size_t)mw * mh;
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t u8n = al(n), b3 = al(3 * n), i32 = al(4 * n);
-    const size_t big = al(std::max(local_max_scratch(mw, mh), ws2_scratch(mw, mh)));
-    W.bytes = 23 * u8n + b3 + 3 * i32 + 5 * i32 + big + 256;
+    const size_t u8n = al256(n), b3 = al256(3 * n), i32 = al256(4 * n);
+    const size_t big = big_scratch(mw, mh);
+    W.bytes = workspace_bytes(mw, mh);
+    W.n_cap = n;
+    W.big_bytes = big;
     NSC_HIP(hipMalloc(&W.base, W.bytes));
     uint8_t* p = static_cast<uint8_t*>(W.base);
     uint8_t** u8s[] = {&W.bgm, &W.bw1, &W.bw2, &W.r2b, &W.rc, &W.rc_open, &W.rc_recon, &W.diff, &W.t1, &W.t2,
@@ -171,6 +184,10 @@ static int ws_alloc(Workspace& W, int mw, int mh) {
 }  // namespace nsc
 
 struct nsc_workspace : nsc::Workspace {};
+
+bool nsc::workspace_fits(const nsc_workspace* ws, int w, int h) {
+    return ws && (size_t)w * h <= ws->n_cap && big_scratch(w, h) <= ws->big_bytes;
+}
 
 using namespace nsc;
 
@@ -207,7 +224,7 @@ int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p
                        nsc_workspace* ws, void** stage, double* stage_ms, void* stream) {
     if (!bgr || w <= 0 || h <= 0) return NSC_SEG_INVALID_IMAGE;
     NSC_ARG(labels && prm && ws, "segment_nuclei");
-    NSC_ARG(w <= ws->max_w && h <= ws->max_h && (size_t)w * h <= (size_t)ws->max_w * ws->max_h, "tile larger than the workspace");
+    NSC_ARG(workspace_fits(ws, w, h), "tile larger than the workspace");
     NSC_ARG(valid_conn(prm->fillHolesConnectivity) && valid_conn(prm->reconConnectivity) &&
                 valid_conn(prm->watershedConnectivity), "connectivity");
     NSC_CHECK(ensure_init());

@@ -1,3 +1,4 @@
+#include <atomic>
 // cv::watershed (OpenCV 2.4.9 imgproc/segmentation.cpp) on a BGR image with int32 markers, and
 // nscale::watershed2 (MorphologicOperations.cpp:3717-3760) with NeighborOperations::border
 // (NeighborOperations.cpp:17-44).
@@ -161,7 +162,7 @@ inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u
 int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, int* ok, hipStream_t s);
 
 // how often the decomposed flood applied / fell back (process-wide counters, for reporting)
-static int g_ws_fast = 0, g_ws_fallback = 0, g_ws_last_flags = 0;
+static std::atomic<int> g_ws_fast{0}, g_ws_fallback{0}, g_ws_last_flags{0};
 void ws_stats_record(int ok) { if (ok) ++g_ws_fast; else ++g_ws_fallback; }
 void ws_last_flags_record(int flags) { g_ws_last_flags = flags; }
 

@@ -7,7 +7,7 @@ import ctypes
 
 import torch
 
-from nscale_amd._lib import SegParams, call, lib
+from nscale_amd._lib import Request, Result, SegParams, ServerConfig, call, check, lib
 
 _P = ctypes.c_void_p
 
@@ -252,3 +252,72 @@ def watershed_stats(with_reason=False):
     a, b, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
     lib().nsc_watershed_stats(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
     return (a.value, b.value, c.value) if with_reason else (a.value, b.value)
+
+
+class Server:
+    """Continuous-batching segmentNuclei server (nsc_server_*): requests are admitted when a slot is
+    free and their footprint fits the HBM budget, and run concurrently on their own streams.
+
+    submit() takes a device BGR tile (kept alive by the server object until the request is waited
+    on) and returns a ticket; wait() returns (status, compcount, labels, bbox5, area, result)."""
+
+    def __init__(self, slots=8, hbm_budget=200 << 30, max_w=4096, max_h=8192):
+        self.handle = _P()
+        cfg = ServerConfig(slots, hbm_budget, max_w, max_h)
+        call("nsc_server_create", ctypes.byref(cfg), ctypes.byref(self.handle))
+        self._live = {}
+
+    def submit(self, bgr, canonical=False, max_labels=0, params=None):
+        h, w = int(bgr.shape[0]), int(bgr.shape[1])
+        assert bgr.is_contiguous() and bgr.dtype == torch.uint8 and bgr.dim() == 3 and bgr.shape[2] == 3
+        dev = bgr.device if bgr.is_cuda else torch.device("cuda", torch.cuda.current_device())
+        # outputs always live on the device, also for host inputs (the worker copies those in)
+        labels = torch.empty(h, w, dtype=torch.int32, device=dev)
+        bbox = area = None
+        if max_labels:
+            bbox = torch.empty(5 * max_labels, dtype=torch.int32, device=dev)
+            area = torch.empty(max_labels, dtype=torch.int32, device=dev)
+        req = Request(_P(bgr.data_ptr()), w, h, 0 if bgr.is_cuda else 1, _P(labels.data_ptr()), 1 if canonical else 0,
+                      _P(bbox.data_ptr()) if bbox is not None else None,
+                      _P(area.data_ptr()) if area is not None else None, max_labels,
+                      ctypes.cast(ctypes.byref(params), _P) if params is not None else None)
+        # inputs must be ready on the device before another stream reads them
+        if bgr.is_cuda:
+            torch.cuda.current_stream().synchronize()
+        t = ctypes.c_int64(0)
+        call("nsc_server_submit", self.handle, ctypes.byref(req), ctypes.byref(t))
+        self._live[t.value] = (bgr, labels, bbox, area)
+        return t.value
+
+    def wait(self, ticket):
+        res = Result()
+        rc = lib().nsc_server_wait(self.handle, ticket, ctypes.byref(res))
+        bgr, labels, bbox, area = self._live.pop(ticket)
+        check(rc, "nsc_server_wait")
+        n = res.compcount
+        if bbox is not None:
+            bbox = bbox[:5 * n].view(5, n) if n else bbox[:0].view(5, 0)
+            area = area[:n]
+        return res.status, n, labels, bbox, area, res
+
+    def stats(self):
+        q, r = ctypes.c_int(0), ctypes.c_int(0)
+        b, bud = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        call("nsc_server_stats", self.handle, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b), ctypes.byref(bud))
+        return q.value, r.value, b.value, bud.value
+
+    def close(self):
+        if self.handle:
+            lib().nsc_server_destroy(self.handle)
+            self.handle = _P()
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def request_bytes(w, h):
+    return int(lib().nsc_request_bytes(w, h))

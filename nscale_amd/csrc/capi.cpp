@@ -18,6 +18,11 @@ void set_error(const char* fmt, ...) {
 }
 void clear_error() { g_err[0] = 0; }
 
+SideStream& side_stream() {
+    static thread_local SideStream ss;
+    return ss;
+}
+
 int ensure_init() {
     static std::once_flag once;
     static int status = NSC_OK;

@@ -93,6 +93,31 @@ inline int valid_conn(int c) { return c == 4 || c == 8; }
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// ---- fork/join onto a side stream ---------------------------------------------------------
+// Independent launches (e.g. long-latency work on a few runs beside many short ones) can overlap
+// on a per-thread side stream (default priority); join() makes the caller's stream wait for it.
+struct SideStream {
+    hipStream_t side = nullptr;
+    hipEvent_t forked = nullptr, joined = nullptr;
+    int fork(hipStream_t s) {
+        if (!side) {
+            NSC_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+            NSC_HIP(hipEventCreateWithFlags(&forked, hipEventDisableTiming));
+            NSC_HIP(hipEventCreateWithFlags(&joined, hipEventDisableTiming));
+        }
+        NSC_HIP(hipEventRecord(forked, s));
+        NSC_HIP(hipStreamWaitEvent(side, forked, 0));
+        return NSC_OK;
+    }
+    int join(hipStream_t s) {
+        NSC_HIP(hipEventRecord(joined, side));
+        NSC_HIP(hipStreamWaitEvent(s, joined, 0));
+        return NSC_OK;
+    }
+};
+// one per host thread: concurrent callers never share a side stream
+SideStream& side_stream();
+
 // ---- device helpers ----------------------------------------------------------------------
 // max/min with the selection rule of std::max / std::min (first argument on ties)
 template <typename R>

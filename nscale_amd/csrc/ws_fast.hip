@@ -169,13 +169,15 @@ __global__ void k_classify(Dev D, uint8_t* cls, uint16_t* lvl0, uint8_t* fmask, 
 }
 
 // ---- phase 1: one lane floods one run ----
-__global__ void k_phase1(Dev D, const int32_t* __restrict__ box5, int nrun, int* __restrict__ qh, int* __restrict__ qt) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrun) return;
+__global__ void k_phase1(Dev D, const int32_t* __restrict__ box5, int nrun, const int* __restrict__ list,
+                         const int* __restrict__ count, int* __restrict__ qh, int* __restrict__ qt) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= *count) return;
+    const int r = list[idx];
     const int W = D.W;
     const int id = r + 1;
-    int* head = qh + (size_t)r * NQ;
-    int* tail = qt + (size_t)r * NQ;
+    int* head = qh + (size_t)idx * NQ;
+    int* tail = qt + (size_t)idx * NQ;
     for (int l = 0; l < NQ; ++l) { head[l] = -1; tail[l] = -1; }
     int active = NQ, seq = 0, popc = 0;
     auto enq = [&](int l, int x) {
@@ -267,12 +269,231 @@ __global__ void k_phase1(Dev D, const int32_t* __restrict__ box5, int nrun, int*
                 D.lf_parent[slot] = p;
                 D.lf_popidx[slot] = -1;
                 const int lid = D.N + slot;
-                D.lf_bnode[slot] = key_less(bp, lid) ? lid : bp;
-                D.lf_next[slot] = atomicExch(&D.head[s], slot);
+                D.lf_bnode[slot] = key_less(bp, lid) ? lid : bp;  // linked into head[s] by k_link_leaves
                 enq(l, lid);
             }
         }
     }
+}
+
+// ---- phase 1 with the run's neighbourhood staged in LDS ----
+// Same flood as k_phase1, for runs whose bounding box plus a 1-pixel margin has at most LCAP
+// pixels.  The wave stages the box (class, in-run flag, seed labels, push levels, colours) into
+// LDS and numbers the initial items in raster order with ballots; lane 0 then runs the flood with
+// every read in LDS, writing the per-item records to global memory as it goes.  Leaf slots are
+// reserved per run up front (one per B pixel of the box that could become a leaf), so no atomic
+// sits on the flood's critical path; unused slots keep lf_b = -1.
+constexpr uint16_t NIL16 = 0xFFFF;
+enum { K_INRUN = 4, K_ST1 = 8, K_ST2 = 16 };
+
+__device__ __forceinline__ int cdiff_packed(uint32_t a, uint32_t b) {
+    const int d0 = abs((int)(a & 255) - (int)(b & 255));
+    const int d1 = abs((int)((a >> 8) & 255) - (int)((b >> 8) & 255));
+    const int d2 = abs((int)((a >> 16) & 255) - (int)((b >> 16) & 255));
+    return max(max(d0, d1), d2);
+}
+
+template <int LCAP>
+__global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restrict__ box5, int nrun,
+                                                   const int* __restrict__ list, const int* __restrict__ count) {
+    __shared__ uint8_t code[LCAP];
+    __shared__ int32_t lk[LCAP];     // known label: seed label, label of a popped F pixel, else 0
+    __shared__ uint16_t lv0[LCAP];
+    __shared__ uint32_t col[LCAP];
+    __shared__ uint8_t ilv[LCAP];
+    __shared__ uint16_t psq[LCAP];
+    __shared__ uint16_t bnd[LCAP];
+    __shared__ uint16_t qn[2 * LCAP];  // FIFO links: F item k, leaf of B pixel k at LCAP + k
+    __shared__ uint16_t lfj[LCAP];
+    __shared__ uint16_t qhead[NQ], qtail[NQ];
+    __shared__ int sh_base;
+
+    if ((int)blockIdx.x >= *count) return;
+    const int r = list[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int W = D.W, id = r + 1;
+    const int ox = box5[nrun + r] - 1, oy = box5[3 * nrun + r] - 1;
+    const int lw = box5[2 * nrun + r] - ox + 2, lh = box5[4 * nrun + r] - oy + 2;
+    const int L = lw * lh;  // <= LCAP (bucketed by the host); the box lies inside the frame
+    auto gidx = [&](int k) { return (oy + k / lw) * W + ox + k % lw; };
+
+    for (int k = lane; k < L; k += 64) {
+        const int g = gidx(k);
+        const uint8_t c = D.cls[g];
+        uint8_t cd = c;
+        if (c == C_F && D.run[g] == id) cd |= K_INRUN;
+        code[k] = cd;
+        lk[k] = (c == C_SEED) ? D.m0[g] : 0;
+        lv0[k] = D.lvl0[g];
+        const uint8_t* q = D.img + 3 * (size_t)g;
+        col[k] = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16);
+    }
+    for (int l = lane; l < NQ; l += 64) { qhead[l] = NIL16; qtail[l] = NIL16; }
+    __syncthreads();
+    // leaf candidates (B, never initially queued, 4-adjacent to the run) and initial items in raster order
+    int cnt = 0, ninit = 0;
+    for (int k0 = 0; k0 < L; k0 += 64) {
+        const int k = k0 + lane;
+        bool cand = false, init = false;
+        if (k < L) {
+            const uint8_t cd = code[k];
+            const int kx = k % lw, ky = k / lw;
+            if ((cd & 3) == C_B && lv0[k] == NOPUSH) {
+                cand = (kx > 0 && (code[k - 1] & K_INRUN)) || (kx < lw - 1 && (code[k + 1] & K_INRUN)) ||
+                       (ky > 0 && (code[k - lw] & K_INRUN)) || (ky < lh - 1 && (code[k + lw] & K_INRUN));
+            }
+            init = (cd & K_INRUN) && lv0[k] != NOPUSH;
+        }
+        const unsigned long long bc = __ballot(cand), bi = __ballot(init);
+        if (init) qn[LCAP + ninit + __popcll(bi & ((1ull << lane) - 1ull))] = (uint16_t)k;
+        cnt += __popcll(bc);
+        ninit += __popcll(bi);
+    }
+    if (lane == 0) sh_base = atomicAdd(D.nleaf, cnt);
+    __syncthreads();
+    const int base = sh_base;
+    // every reserved slot below the capacity reads as unused until a leaf fills it
+    for (int j = base + lane; j < min(base + cnt, D.leafcap); j += 64) D.lf_b[j] = -1;
+    if (base + cnt > D.leafcap) {
+        if (lane == 0) atomicOr(D.flags, FB_LEAF_CAP);
+        return;
+    }
+    // initial items: records in parallel
+    for (int q = lane; q < ninit; q += 64) {
+        const int k = qn[LCAP + q];
+        const int g = gidx(k);
+        ilv[k] = (uint8_t)lv0[k];
+        psq[k] = (uint16_t)q;
+        bnd[k] = (uint16_t)k;
+        code[k] |= K_ST1;
+        D.ilvl[g] = (uint8_t)lv0[k];
+        D.pseq[g] = q;
+        D.parent[g] = -1;
+        D.bnode[g] = g;
+        D.st[g] = 1;
+    }
+    __syncthreads();
+    if (lane != 0) return;
+
+    int active = NQ;
+    auto enq = [&](int l, int x) {  // FIFO of level l; the flood resumes at the lowest queued level
+        qn[x] = NIL16;
+        const uint16_t t = qtail[l];
+        if (t != NIL16) qn[t] = (uint16_t)x;
+        else qhead[l] = (uint16_t)x;
+        qtail[l] = (uint16_t)x;
+        if (l < active) active = l;
+    };
+    for (int q = 0; q < ninit; ++q) {
+        const int k = qn[LCAP + q];
+        enq(lv0[k], k);
+    }
+    int seq = ninit, popc = 0, nlf = 0;
+    const int ld[4] = {-1, 1, -lw, lw};
+    const int gd[4] = {-1, 1, -W, W};
+    for (;;) {
+        if (active >= NQ) break;
+        if (qhead[active] == NIL16) {
+            int l = active + 1;
+            while (l < NQ && qhead[l] == NIL16) ++l;
+            if (l == NQ) break;
+            active = l;
+        }
+        const int x = qhead[active];
+        const uint16_t nx = qn[x];
+        qhead[active] = nx;
+        if (nx == NIL16) qtail[active] = NIL16;
+        if (x >= LCAP) {  // leaf: record when it pops
+            D.lf_popidx[base + lfj[x - LCAP]] = popc++;
+            continue;
+        }
+        const int p = x;
+        const int gp = gidx(p);
+        D.popidx[gp] = popc++;
+        int lab = 0;
+        for (int d = 0; d < 4; ++d) {
+            const int t = lk[p + ld[d]];
+            if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+        }
+        lk[p] = lab;
+        code[p] = (uint8_t)((code[p] & ~K_ST1) | K_ST2);
+        D.lab[gp] = lab;
+        D.st[gp] = 2;
+        if (lab == WSHED) continue;
+        const uint32_t cp = col[p];
+        const int bp = bnd[p];
+        const int bl = ilv[bp], bs = psq[bp];
+        for (int d = 0; d < 4; ++d) {
+            const int sk = p + ld[d];
+            const uint8_t cs = code[sk];
+            if ((cs & 3) == C_F) {
+                if (cs & (K_ST1 | K_ST2)) continue;
+                const int l = cdiff_packed(cp, col[sk]);
+                const int gs = gp + gd[d];
+                const bool newer = (bl < l) || (bl == l && bs < seq);  // key(bnode[p]) < key(s)
+                const int b = newer ? sk : bp;
+                ilv[sk] = (uint8_t)l;
+                psq[sk] = (uint16_t)seq;
+                bnd[sk] = (uint16_t)b;
+                code[sk] = cs | K_ST1;
+                D.ilvl[gs] = (uint8_t)l;
+                D.pseq[gs] = seq;
+                D.parent[gs] = gp;
+                D.bnode[gs] = gidx(b);
+                D.st[gs] = 1;
+                ++seq;
+                enq(l, sk);
+            } else if ((cs & 3) == C_B && lv0[sk] == NOPUSH) {
+                const int sx = sk % lw, sy = sk / lw;
+                bool done = false;  // queued already by an earlier labelled neighbour of this run
+                for (int e = 0; e < 4; ++e) {
+                    const int fx = sx + (e == 0 ? -1 : e == 1 ? 1 : 0), fy = sy + (e == 2 ? -1 : e == 3 ? 1 : 0);
+                    if (fx < 0 || fx >= lw || fy < 0 || fy >= lh) continue;
+                    const int f2 = fy * lw + fx;
+                    if (f2 != p && (code[f2] & K_INRUN) && (code[f2] & K_ST2) && lk[f2] > 0) done = true;
+                }
+                if (done) continue;
+                if (nlf >= cnt) { atomicOr(D.flags, FB_LEAF_CAP); continue; }
+                const int j = nlf++;
+                const int slot = base + j;
+                lfj[sk] = (uint16_t)j;
+                const int l = cdiff_packed(cp, col[sk]);
+                const bool newer = (bl < l) || (bl == l && bs < seq);
+                D.lf_b[slot] = gp + gd[d];
+                D.lf_run[slot] = id;
+                D.lf_lvl[slot] = (uint8_t)l;
+                D.lf_pseq[slot] = seq;
+                D.lf_parent[slot] = gp;
+                D.lf_popidx[slot] = -1;
+                D.lf_bnode[slot] = newer ? D.N + slot : gidx(bp);
+                ++seq;
+                enq(l, LCAP + sk);
+            }
+        }
+        if (seq >= 65535) { atomicOr(D.flags, FB_LEAF_CAP); return; }
+    }
+}
+
+// every used leaf slot joins the list of its B pixel (list order is irrelevant to its readers)
+__global__ void k_link_leaves(Dev D) {
+    const int n = min(*D.nleaf, D.leafcap);
+    for (int sl = blockIdx.x * blockDim.x + threadIdx.x; sl < n; sl += gridDim.x * blockDim.x) {
+        const int b = D.lf_b[sl];
+        if (b >= 0 && b < D.N) D.lf_next[sl] = atomicExch(&D.head[b], sl);
+    }
+}
+
+// runs by the size of their LDS frame (box + 1-pixel margin): bucket b holds frames of at most
+// kRunTier[b] pixels; the last bucket floods from global memory
+constexpr int kRunTiers = 5;
+__constant__ int kRunTier[kRunTiers - 1] = {512, 1536, 4096, 7040};
+__global__ void k_bucket_runs(const int32_t* __restrict__ box5, int nrun, int* __restrict__ lists, int* __restrict__ counts) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrun) return;
+    const long long L = (long long)(box5[2 * nrun + r] - box5[nrun + r] + 3) * (box5[4 * nrun + r] - box5[3 * nrun + r] + 3);
+    int b = 0;
+    while (b < kRunTiers - 1 && L > kRunTier[b]) ++b;
+    lists[(size_t)b * nrun + atomicAdd(&counts[b], 1)] = r;
 }
 
 // ---- B items: the real queue entry of each B pixel and its label decision at pop time ----
@@ -305,14 +526,40 @@ __global__ void k_bresolve(Dev D, int32_t* __restrict__ real, int* __restrict__ 
     }
 }
 
-// info[0] = b1 pixel (-1 none), info[1] = b1 item, info[2] = Y1
-__global__ __launch_bounds__(1024) void k_b1_reduce(Dev D, const int32_t* __restrict__ real, const int* __restrict__ cand,
-                                                    const int* __restrict__ ncand, int* __restrict__ info) {
-    __shared__ int best[1024];
+// earliest candidate per block (candidates: B pixels whose earlier-popped neighbours agree on a
+// positive label), -1 for a block without candidates
+constexpr int B1_BLOCKS = 1024;
+__global__ __launch_bounds__(256) void k_b1_partial(Dev D, const int32_t* __restrict__ real, const int* __restrict__ cand,
+                                                    const int* __restrict__ ncand, int* __restrict__ partial) {
+    __shared__ int best[256];
     const int n = *ncand;
+    int mine = -1;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += B1_BLOCKS * 256) {
+        const int b = cand[k];
+        if (mine < 0 || before(D, real[b], real[mine])) mine = b;
+    }
+    best[threadIdx.x] = mine;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if ((int)threadIdx.x < stride) {
+            const int a = best[threadIdx.x], c = best[threadIdx.x + stride];
+            if (a < 0 || (c >= 0 && before(D, real[c], real[a]))) best[threadIdx.x] = c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = best[0];
+}
+
+// info[0] = b1 pixel (-1 none), info[1] = b1 item, info[2] = Y1; `cand` holds the B1_BLOCKS
+// per-block winners (-1 = none)
+__global__ __launch_bounds__(1024) void k_b1_reduce(Dev D, const int32_t* __restrict__ real, const int* __restrict__ cand,
+                                                    int* __restrict__ info) {
+    __shared__ int best[1024];
+    const int n = B1_BLOCKS;
     int mine = -1;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const int b = cand[k];
+        if (b < 0) continue;
         if (mine < 0 || before(D, real[b], real[mine])) mine = b;
     }
     best[threadIdx.x] = mine;
@@ -762,7 +1009,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         real = c.take<int32_t>(N);
         lab2 = c.take<int32_t>(N);
         kind = c.take<uint8_t>(N);
-        small = c.take<int>(64);
+        small = c.take<int>(64 + B1_BLOCKS);  // counters, then the per-block b1 winners
     };
     uint8_t *cls, *fmask, *st, *ilvl, *kind;
     uint16_t* lvl0;
@@ -774,7 +1021,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     Carver cut(sc.p);
     carve(cut, cls, lvl0, fmask, run, st, ilvl, pseq, parent, bnode, popidx, lab, head, real, lab2, kind, small);
     // small[]: 0 nF, 1 nrun, 2 nleaf, 3 flags, 4 ncand, 8..10 info, 16 ncomp, 20 deferred tie components,
-    // 24..25 tie pool bump pointer (long long)
+    // 24..25 tie pool bump pointer (long long), 26..30 run bucket sizes
     NSC_HIP(hipMemsetAsync(small, 0, 64 * sizeof(int), s));
 
     Dev D{};
@@ -795,7 +1042,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     Carver m2(nullptr);
     auto carve2 = [&](Carver& c, int32_t*& box5, int*& qh, int*& qt, int32_t*& qnext, int32_t*& lf) {
         box5 = c.take<int32_t>(5 * (size_t)std::max(nrun, 1));
-        qh = c.take<int>((size_t)std::max(nrun, 1) * NQ);
+        qh = c.take<int>((size_t)std::max(nrun, 1) * (NQ + kRunTiers));  // + run buckets
         qt = c.take<int>((size_t)std::max(nrun, 1) * NQ);
         qnext = c.take<int32_t>(N + (size_t)leafcap);
         lf = c.take<int32_t>((size_t)leafcap * 8);
@@ -819,14 +1066,32 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     D.lf_lvl = reinterpret_cast<uint8_t*>(lf + 7 * (size_t)leafcap);
     if (nrun > 0) {
         NSC_CHECK(bbox_impl(run, W, H, nrun, box5, nullptr, s));
-        hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, s, D, box5, nrun, qh, qt);
+        int* lists = reinterpret_cast<int*>(qh);  // kRunTiers * nrun ints
+        int* bcount = &small[26];                  // kRunTiers counters
+        int* qh2 = qh + (size_t)kRunTiers * nrun;  // FIFO heads for the global-flood bucket
+        hipLaunchKernelGGL(k_bucket_runs, dim3(cdiv(nrun, 256)), dim3(256), 0, s, box5, nrun, lists, bcount);
+        auto L = [&](int b) { return lists + (size_t)b * nrun; };
+        // the few large runs set the latency: they run on the side stream beside the many small ones
+        SideStream& ss = side_stream();
+        NSC_CHECK(ss.fork(s));
+        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(nrun), dim3(64), 0, ss.side, D, box5, nrun, L(2), bcount + 2);
+        hipLaunchKernelGGL(k_phase1_lds<7040>, dim3(nrun), dim3(64), 0, ss.side, D, box5, nrun, L(3), bcount + 3);
+        hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, ss.side, D, box5, nrun, L(4), bcount + 4, qh2, qt);
+        hipLaunchKernelGGL(k_phase1_lds<1536>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1);
+        hipLaunchKernelGGL(k_phase1_lds<512>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(0), bcount);
+        NSC_CHECK(ss.join(s));
+        hipLaunchKernelGGL(k_link_leaves, grid1(N), dim3(256), 0, s, D);
     }
     tick("phase1", &small[2]);
     // B resolution and b1
     int* cand = lab2;  // reuse (N ints)
     hipLaunchKernelGGL(k_bresolve, grid1(N), dim3(256), 0, s, D, real, cand, &small[4]);
     tick("bresolve", &small[4]);
-    hipLaunchKernelGGL(k_b1_reduce, dim3(1), dim3(1024), 0, s, D, real, cand, &small[4], &small[8]);
+    {
+        int* partial = small + 64;
+        hipLaunchKernelGGL(k_b1_partial, dim3(B1_BLOCKS), dim3(256), 0, s, D, real, cand, &small[4], partial);
+        hipLaunchKernelGGL(k_b1_reduce, dim3(1), dim3(1024), 0, s, D, real, partial, &small[8]);
+    }
     tick("b1", &small[8]);
     hipLaunchKernelGGL(k_t1_state, grid1(N), dim3(256), 0, s, D, real, &small[8], fmask, kind);
     // flood from b1 over the never-queued B pixels (fmask marks them): geodesic distance from b1

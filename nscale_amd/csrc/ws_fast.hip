@@ -804,8 +804,21 @@ __device__ bool flood_component(const Dev& D, const P2& Q, int c, const int* ord
     return true;
 }
 
+// components deferred to the LDS flood: a record per component, a work item per (record, arrangement)
+constexpr int FL_SMALL = 1024, FL_BIG = 4096;  // frame pixels (box + 1-pixel margin) of the two LDS tiers
+constexpr int FL_MIN_AREA = 24;                // smaller tie-free components flood in their lane
+struct FloodQ {
+    TieRec* recs;
+    int* nrec;
+    int reccap;
+    int* work[2];
+    int* nwork;  // [2]
+    int workcap;
+    int* res;    // per-arrangement results of tie components (pool)
+};
+
 __global__ void k_phase2(Dev D, P2 Q, const int32_t* __restrict__ real, long long* __restrict__ heapk, TieRec* ties,
-                         int* nties, int tiecap, long long* pool_used, long long poolcap) {
+                         int* nties, int tiecap, long long* pool_used, long long poolcap, FloodQ F) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= Q.ncomp) return;
     const int W = D.W;
@@ -860,12 +873,30 @@ __global__ void k_phase2(Dev D, P2 Q, const int32_t* __restrict__ real, long lon
         }
         a = e;
     }
+    R.ncombo = (int)ncombo;
+    const int L = (maxx - minx + 3) * (maxy - miny + 3);
+    if (L <= FL_BIG && (R.ntie > 0 || A > FL_MIN_AREA)) {  // LDS flood, one wave per arrangement
+        long long base = 0;
+        if (ncombo > 1) {
+            const long long need = ncombo * (long long)L;
+            base = atomicAdd((unsigned long long*)pool_used, (unsigned long long)need);
+            if (base + need > poolcap) { atomicOr(D.flags, FB_TIES); return; }
+        }
+        R.base = base;
+        const int rec = atomicAdd(F.nrec, 1);
+        if (rec >= F.reccap) { atomicOr(D.flags, FB_TIES); return; }
+        F.recs[rec] = R;
+        const int tier = L <= FL_SMALL ? 0 : 1;
+        const int w0 = atomicAdd(&F.nwork[tier], (int)ncombo);
+        if (w0 + ncombo > F.workcap) { atomicOr(D.flags, FB_TIES); return; }
+        for (int k = 0; k < (int)ncombo; ++k) F.work[tier][w0 + k] = rec * MAXCOMBO + k;
+        return;
+    }
     if (R.ntie == 0) {
         if (!flood_component(D, Q, c, items, ni, nullptr, hv, hk, A, real)) atomicOr(D.flags, FB_HEAP);
         return;
     }
     // defer to the parallel permutation check (one wave per component)
-    R.ncombo = (int)ncombo;
     const int B = (maxx - minx + 1) * (maxy - miny + 1);
     const long long per = (long long)B + ni + A + 2LL * A + 4;
     const long long need = 64 * per + B + 64;
@@ -956,6 +987,151 @@ __global__ __launch_bounds__(64) void k_phase2_ties(Dev D, P2 Q, const int32_t* 
                 const int i = (miny + y) * W + (minx + x);
                 if (Q.comp[i] == id) D.m[i] = canon[y * bw + x];
             }
+    }
+}
+
+// one wave floods one arrangement of one deferred component with its frame (box + 1-pixel
+// margin) in LDS: the same flood as flood_component (lowest level first, FIFO within a level,
+// neighbours in the order left, right, up, down), lane 0 driving it
+template <int LCAP>
+__global__ __launch_bounds__(64) void k_flood_lds(Dev D, P2 Q, const int32_t* __restrict__ real, FloodQ F, int tier) {
+    __shared__ int32_t st[LCAP];
+    __shared__ uint32_t col[LCAP];
+    __shared__ uint16_t qn[LCAP];
+    __shared__ uint8_t inc[LCAP];
+    __shared__ uint16_t ord[LCAP];
+    __shared__ uint8_t olv[LCAP];
+    __shared__ uint16_t qhead[NQ], qtail[NQ];
+    __shared__ int bad;
+    const int nw = min(F.nwork[tier], F.workcap);
+    for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
+    const int w = F.work[tier][wi];
+    const TieRec R = F.recs[w / MAXCOMBO];
+    const int combo = w % MAXCOMBO;
+    const int c = R.c, id = c + 1, W = D.W, lane = threadIdx.x;
+    const int ox = Q.box5[Q.ncomp + c] - 1, oy = Q.box5[3 * Q.ncomp + c] - 1;
+    const int lw = Q.box5[2 * Q.ncomp + c] - ox + 2, lh = Q.box5[4 * Q.ncomp + c] - oy + 2;
+    const int L = lw * lh;  // <= LCAP; the frame lies inside the image (components avoid the frame)
+    const int ni = R.ni;
+    const int32_t* items = Q.scratch + 3 * (size_t)Q.off[c];
+    auto gidx = [&](int k) { return (oy + k / lw) * W + ox + k % lw; };
+    if (lane == 0) bad = 0;
+    for (int k = lane; k < L; k += 64) {
+        const int g = gidx(k);
+        inc[k] = Q.comp[g] == id ? 1 : 0;
+        st[k] = D.m[g];
+        const uint8_t* q = D.img + 3 * (size_t)g;
+        col[k] = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16);
+    }
+    for (int l = lane; l < NQ; l += 64) { qhead[l] = NIL16; qtail[l] = NIL16; }
+    for (int q = lane; q < ni; q += 64) {
+        const int v = items[q];
+        ord[q] = (uint16_t)((v / W - oy) * lw + (v % W - ox));
+        olv[q] = (uint8_t)item_level(D, real, Q.kind, v);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        // arrangement `combo`: factorial-base digits, one per tie group (members share their level)
+        int code = combo;
+        for (int g = 0; g < R.ntie; ++g) {
+            const int Lg = R.glen[g], f0 = R.gfirst[g];
+            const int f = fact(Lg);
+            int digit = code % f;
+            code /= f;
+            uint16_t pool_el[MAXGL];
+            for (int q = 0; q < Lg; ++q) pool_el[q] = ord[f0 + q];
+            for (int q = 0; q < Lg; ++q) {
+                const int fq = fact(Lg - 1 - q);
+                const int idx = digit / fq;
+                digit %= fq;
+                ord[f0 + q] = pool_el[idx];
+                for (int z = idx; z < Lg - 1 - q; ++z) pool_el[z] = pool_el[z + 1];
+            }
+        }
+        int active = NQ;
+        auto enq = [&](int l, int x) {
+            qn[x] = NIL16;
+            const uint16_t t = qtail[l];
+            if (t != NIL16) qn[t] = (uint16_t)x;
+            else qhead[l] = (uint16_t)x;
+            qtail[l] = (uint16_t)x;
+            if (l < active) active = l;
+        };
+        for (int q = 0; q < ni; ++q) enq(olv[q], ord[q]);
+        const int ld[4] = {-1, 1, -lw, lw};
+        for (;;) {
+            if (active >= NQ) break;
+            if (qhead[active] == NIL16) {
+                int l = active + 1;
+                while (l < NQ && qhead[l] == NIL16) ++l;
+                if (l == NQ) break;
+                active = l;
+            }
+            const int p = qhead[active];
+            const uint16_t nx = qn[p];
+            qhead[active] = nx;
+            if (nx == NIL16) qtail[active] = NIL16;
+            int lab = 0;
+            for (int d = 0; d < 4; ++d) {
+                const int t = st[p + ld[d]];
+                if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+            }
+            st[p] = lab;
+            if (lab == WSHED) continue;
+            const uint32_t cp = col[p];
+            for (int d = 0; d < 4; ++d) {
+                const int s2 = p + ld[d];
+                if (st[s2] != 0) continue;
+                if (!inc[s2]) { bad = 1; continue; }  // an unresolved pixel outside the component
+                st[s2] = IN_QUEUE;
+                enq(cdiff_packed(cp, col[s2]), s2);
+            }
+        }
+        if (bad) atomicOr(D.flags, FB_HEAP);
+    }
+    __syncthreads();
+    if (R.ncombo == 1) {
+        for (int k = lane; k < L; k += 64)
+            if (inc[k]) D.m[gidx(k)] = st[k];
+    } else {
+        int* out = F.res + R.base + (long long)combo * L;
+        for (int k = lane; k < L; k += 64) out[k] = st[k];
+    }
+    __syncthreads();  // the LDS frame is reused by the next work item
+    }
+}
+
+// tie components flooded in LDS: every arrangement must give arrangement 0's labels, which are committed
+__global__ __launch_bounds__(256) void k_tie_commit(Dev D, P2 Q, FloodQ F) {
+    __shared__ int bad;
+    const int nr = min(*F.nrec, F.reccap);
+    for (int ri = blockIdx.x; ri < nr; ri += gridDim.x) {
+    const TieRec R = F.recs[ri];
+    if (R.ncombo == 1) continue;
+    const int c = R.c, id = c + 1, W = D.W;
+    const int ox = Q.box5[Q.ncomp + c] - 1, oy = Q.box5[3 * Q.ncomp + c] - 1;
+    const int lw = Q.box5[2 * Q.ncomp + c] - ox + 2, lh = Q.box5[4 * Q.ncomp + c] - oy + 2;
+    const int L = lw * lh;
+    const int* res = F.res + R.base;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        const int g = (oy + k / lw) * W + ox + k % lw;
+        if (Q.comp[g] != id) continue;
+        const int v0 = res[k];
+        for (int a = 1; a < R.ncombo; ++a)
+            if (res[(long long)a * L + k] != v0) { bad = 1; break; }
+    }
+    __syncthreads();
+    if (bad) {
+        if (threadIdx.x == 0) atomicOr(D.flags, FB_TIEPERM);
+    } else {
+        for (int k = threadIdx.x; k < L; k += blockDim.x) {
+            const int g = (oy + k / lw) * W + ox + k % lw;
+            if (Q.comp[g] == id) D.m[g] = res[k];
+        }
+    }
+    __syncthreads();  // `bad` is reset for the next record
     }
 }
 
@@ -1136,9 +1312,26 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         TieRec* ties = reinterpret_cast<TieRec*>(sc4.p);
         int* pool = reinterpret_cast<int*>(reinterpret_cast<char*>(sc4.p) + ((sizeof(TieRec) * tiecap + 255) & ~size_t(255)));
         long long* pool_used = reinterpret_cast<long long*>(&small[24]);  // 8-byte aligned: small is 256-aligned
+        // records and work lists of the LDS flood
+        const int workcap = ncomp + (1 << 18);
+        Scratch sc5;
+        NSC_CHECK(sc5.alloc(sizeof(TieRec) * (size_t)ncomp + 2 * sizeof(int) * (size_t)workcap + 512, s));
+        FloodQ F{};
+        F.recs = reinterpret_cast<TieRec*>(sc5.p);
+        F.reccap = ncomp;
+        F.work[0] = reinterpret_cast<int*>(reinterpret_cast<char*>(sc5.p) + ((sizeof(TieRec) * (size_t)ncomp + 255) & ~size_t(255)));
+        F.work[1] = F.work[0] + workcap;
+        F.workcap = workcap;
+        F.nrec = &small[32];
+        F.nwork = &small[33];  // [33], [34]
+        F.res = pool;
         hipLaunchKernelGGL(k_phase2, dim3(cdiv(ncomp, 64)), dim3(64), 0, s, D, Q, real, hk, ties, &small[20], tiecap,
-                           pool_used, poolcap);
+                           pool_used, poolcap, F);
         tick("phase2", &small[20]);
+        hipLaunchKernelGGL(k_flood_lds<FL_BIG>, dim3(1024), dim3(64), 0, s, D, Q, real, F, 1);
+        hipLaunchKernelGGL(k_flood_lds<FL_SMALL>, dim3(4096), dim3(64), 0, s, D, Q, real, F, 0);
+        hipLaunchKernelGGL(k_tie_commit, dim3(1024), dim3(256), 0, s, D, Q, F);
+        tick("lds flood", &small[32]);
         hipLaunchKernelGGL(k_phase2_ties, dim3(tiecap), dim3(64), 0, s, D, Q, real, ties, &small[20], pool);
         tick("ties", &small[3]);
     }

@@ -1274,7 +1274,12 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     {
         unsigned* dist = reinterpret_cast<unsigned*>(head);  // head[] is dead after k_bresolve
         hipLaunchKernelGGL(k_geo_init, grid1(N), dim3(256), 0, s, D, &small[8], fmask, dist);
-        NSC_CHECK(geodist_impl(fmask, dist, W, H, s));
+        const int grc = geodist_impl(fmask, dist, W, H, s);
+        if (grc == NSC_ERR_LIMIT) {  // outside the tile search's range: the sequential flood decides
+            ws_last_flags_record(FB_CMP);
+            return NSC_OK;
+        }
+        NSC_CHECK(grc);
         tick("geodist", nullptr);
         hipLaunchKernelGGL(k_bfs_mark, grid1(N), dim3(256), 0, s, D, &small[8], dist, kind, lab /* pdist; lab[] is dead */);
         hipLaunchKernelGGL(k_bfs_commit, grid1(N), dim3(256), 0, s, D, &small[8], dist, kind, fmask);

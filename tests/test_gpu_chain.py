@@ -71,12 +71,14 @@ def test_chain_rbc_branch():
         assert np.array_equal(lab, olab)
 
 
-def test_watershed_decomposition_used_and_equal_to_sequential(monkeypatch):
+@pytest.mark.parametrize("size,seed", [(1024, 11), (1024, 2026), (2048, 7)])
+def test_watershed_decomposition_used_and_equal_to_sequential(monkeypatch, size, seed):
     """The chain's watershed runs on the decomposed parallel flood (no fallback), and the
     sequential per-component flood gives the same bytes on the same input."""
     from nscale_amd import gpu
     from nscale_amd.synth import synth_he
-    bgr = torch.from_numpy(synth_he(1024, 1024, 11)).cuda()
+    print("watershed decomposition", size, seed, flush=True)
+    bgr = torch.from_numpy(synth_he(size, size, seed)).cuda()
     f0, b0 = gpu.watershed_stats()
     st, lab, cc, bufs, _ = gpu.segment_nuclei(bgr, stages=(20,))
     torch.cuda.synchronize()

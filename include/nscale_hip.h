@@ -64,6 +64,21 @@ int nsc_imreconstruct_f32(const float* marker, const float* mask, float* out, in
 /* nscale::imreconstruct<int>(seeds, image, conn)  MorphologicOperations.h:34, .cpp:1730-1903 */
 int nsc_imreconstruct_i32(const int32_t* marker, const int32_t* mask, int32_t* out, int w, int h, int conn,
                           void* stream);
+/* Batched reconstruction: n independent (marker, mask) pairs of any sizes and connectivity in
+   one launch per pass (the reference calls imreconstruct once per tile, nu-segment.cpp:175).
+   Results are identical to n single calls.  marker/mask/out are device pointers of the element
+   type of the entry point (uint8 or float). */
+typedef struct nsc_recon_desc {
+    const void* marker;
+    const void* mask;
+    void* out;
+    int w, h, conn;
+} nsc_recon_desc;
+int nsc_imreconstruct_u8_batched(const nsc_recon_desc* descs, int n, void* stream);
+int nsc_imreconstruct_f32_batched(const nsc_recon_desc* descs, int n, void* stream);
+/* nscale::imhmin<float>(image, h, conn) on n images (MorphologicOperations.cpp:3342-3364):
+   descs[i].mask = input image, descs[i].out = output, descs[i].marker unused */
+int nsc_imhmin_f32_batched(const nsc_recon_desc* descs, int n, float hval, void* stream);
 /* nscale::imreconstructBinary<uchar>(seeds, image, conn)  MorphologicOperations.h:61, .cpp:2768-2909 */
 int nsc_imreconstruct_binary_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int w, int h,
                                 int conn, void* stream);

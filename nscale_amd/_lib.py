@@ -36,6 +36,10 @@ class Request(ctypes.Structure):
                 ("bbox5", _P), ("area", _P), ("max_labels", _I), ("params", _P)]
 
 
+class ReconDesc(ctypes.Structure):
+    _fields_ = [("marker", _P), ("mask", _P), ("out", _P), ("w", _I), ("h", _I), ("conn", _I)]
+
+
 class Result(ctypes.Structure):
     _fields_ = [("status", _I), ("compcount", _I), ("error", _I), ("slot", _I),
                 ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
@@ -62,6 +66,9 @@ def lib():
         L.nsc_server_stats.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_size_t),
                                        ctypes.POINTER(ctypes.c_size_t)]
         L.nsc_server_destroy.argtypes = [_P]
+        L.nsc_imreconstruct_u8_batched.argtypes = [ctypes.POINTER(ReconDesc), _I, _P]
+        L.nsc_imreconstruct_f32_batched.argtypes = [ctypes.POINTER(ReconDesc), _I, _P]
+        L.nsc_imhmin_f32_batched.argtypes = [ctypes.POINTER(ReconDesc), _I, ctypes.c_float, _P]
         _lib = L
     return _lib
 

@@ -67,6 +67,18 @@ int nsc_imreconstruct_f32(const float* marker, const float* mask, float* out, in
     return recon_impl<float>(marker, mask, out, w, h, conn, as_stream(stream));
 }
 
+int nsc_imreconstruct_u8_batched(const nsc_recon_desc* descs, int n, void* stream) {
+    NSC_ARG(descs && n > 0, "imreconstruct_u8_batched");
+    NSC_CHECK(ensure_init());
+    return recon_batched_impl<uint8_t>(descs, n, as_stream(stream));
+}
+
+int nsc_imreconstruct_f32_batched(const nsc_recon_desc* descs, int n, void* stream) {
+    NSC_ARG(descs && n > 0, "imreconstruct_f32_batched");
+    NSC_CHECK(ensure_init());
+    return recon_batched_impl<float>(descs, n, as_stream(stream));
+}
+
 int nsc_imreconstruct_i32(const int32_t* marker, const int32_t* mask, int32_t* out, int w, int h, int conn,
                           void* stream) {
     NSC_ARG(marker && mask && out && w > 0 && h > 0 && valid_conn(conn), "imreconstruct_i32");

@@ -154,5 +154,7 @@ int ccl_mask(const uint8_t* fg, const float* vals, int32_t* labels, int w, int h
 
 template <typename T>
 int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s);
+template <typename T>
+int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s);
 
 }  // namespace nsc

@@ -9,7 +9,9 @@
 //   outside the image count as -FLT_MAX, the reference's padding).
 // The flood fills become: CCL over (candidate, equal value) + OR of the "has a greater neighbour"
 // flag per component root.
+#include <algorithm>
 #include <cfloat>
+#include <vector>
 
 #include "common.hpp"
 
@@ -89,6 +91,40 @@ inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u
 
 }  // namespace
 
+// batched h-minima: pixels of all images in one flat index space (image found by binary search
+// over the offsets)
+struct HB {
+    const float* img;
+    float* mask;
+    float* marker;
+    float* out;
+    long long off, n;
+};
+__device__ __forceinline__ int hb_find(const HB* __restrict__ b, int nb, long long i) {
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b[mid].off <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+__global__ void k_hmin_prep_b(const HB* __restrict__ b, int nb, long long total, float hval) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const HB& e = b[hb_find(b, nb, i)];
+        const long long k = i - e.off;
+        const float m = -e.img[k];
+        e.mask[k] = m;
+        e.marker[k] = m - hval;
+    }
+}
+__global__ void k_hmin_post_b(const HB* __restrict__ b, int nb, long long total) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const HB& e = b[hb_find(b, nb, i)];
+        const long long k = i - e.off;
+        e.out[k] = -e.out[k];
+    }
+}
+
 int imhmin_impl(const float* img, float* out, int w, int h, float hval, int conn, float* s1, float* s2, hipStream_t s) {
     const size_t n = (size_t)w * h;
     hipLaunchKernelGGL(k_hmin_prep, grid1(n), dim3(256), 0, s, img, hval, s1, s2, n);
@@ -129,6 +165,42 @@ int local_max_impl(const float* img, uint8_t* out, int w, int h, int conn, bool 
 using namespace nsc;
 
 extern "C" {
+
+int nsc_imhmin_f32_batched(const nsc_recon_desc* descs, int n, float hval, void* stream) {
+    NSC_ARG(descs && n > 0, "imhmin_f32_batched");
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    std::vector<HB> hb(n);
+    std::vector<nsc_recon_desc> rd(n);
+    long long total = 0;
+    for (int i = 0; i < n; ++i) {
+        NSC_ARG(descs[i].mask && descs[i].out && descs[i].w > 0 && descs[i].h > 0 && valid_conn(descs[i].conn),
+                "imhmin_f32_batched: descriptor");
+        hb[i].img = static_cast<const float*>(descs[i].mask);
+        hb[i].out = static_cast<float*>(descs[i].out);
+        hb[i].off = total;
+        hb[i].n = (long long)descs[i].w * descs[i].h;
+        total += hb[i].n;
+    }
+    Scratch planes, tab;
+    NSC_CHECK(planes.alloc(2 * sizeof(float) * (size_t)total + 512 * (size_t)n, s));
+    char* p = static_cast<char*>(planes.p);
+    for (int i = 0; i < n; ++i) {
+        const size_t b = ((sizeof(float) * (size_t)hb[i].n + 255) / 256) * 256;
+        hb[i].mask = reinterpret_cast<float*>(p); p += b;
+        hb[i].marker = reinterpret_cast<float*>(p); p += b;
+        rd[i] = nsc_recon_desc{hb[i].marker, hb[i].mask, descs[i].out, descs[i].w, descs[i].h, descs[i].conn};
+    }
+    NSC_CHECK(tab.alloc(sizeof(HB) * n, s));
+    NSC_HIP(hipMemcpyAsync(tab.p, hb.data(), sizeof(HB) * n, hipMemcpyHostToDevice, s));
+    const unsigned g = (unsigned)std::min<long long>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_hmin_prep_b, dim3(g), dim3(256), 0, s, tab.as<HB>(), n, total, hval);
+    NSC_CHECK(recon_batched_impl<float>(rd.data(), n, s));
+    hipLaunchKernelGGL(k_hmin_post_b, dim3(g), dim3(256), 0, s, tab.as<HB>(), n, total);
+    NSC_HIP(hipGetLastError());
+    NSC_HIP(hipStreamSynchronize(s));  // host tables above are released on return
+    return NSC_OK;
+}
 
 int nsc_imhmin_f32(const float* img, float* out, int w, int h, float hval, int conn, void* stream) {
     NSC_ARG(img && out && w > 0 && h > 0 && valid_conn(conn), "imhmin_f32");

@@ -20,6 +20,9 @@
 //    the output: 3 bytes/px for uint8, the algorithmic minimum.  A tile whose owned border
 //    changed marks the neighbours whose halo it feeds; later passes re-run only marked tiles
 //    (reading out + mask) until no tile is marked.
+#include <algorithm>
+#include <vector>
+
 #include "common.hpp"
 
 namespace nsc {
@@ -43,25 +46,38 @@ __device__ __forceinline__ void mark_tile(int t, unsigned next_pass, unsigned* s
     }
 }
 
-template <typename T, int CONN>
-__global__ __launch_bounds__(64) void recon_tiles(const T* marker, const T* __restrict__ mask,
-                                                  T* out, int w, int h, int tiles_x, int tiles_y,
-                                                  const int* __restrict__ list, const int* __restrict__ list_count,
-                                                  int* __restrict__ next_list, int* __restrict__ next_count,
-                                                  unsigned* __restrict__ stamp, int pass) {
-    using R = typename RegOf<T>::type;
-    __shared__ T lm[TH][TW + 4];  // mask per window pixel (frozen pixels: mask = value)
-    __shared__ T tr[TH][TW + 1];  // transpose buffer (values fit the storage type)
-    __shared__ R init_l[TH], init_r[TH];
-    __shared__ unsigned chg_bits;
+// one image of a reconstruction launch: its planes, size, connectivity and first global tile id
+template <typename T>
+struct RImg {
+    const T* marker;
+    const T* mask;
+    T* out;
+    int w, h, tiles_x, tiles_y, conn, base;
+};
 
-    int tile;
-    if (pass == 0) {
-        tile = blockIdx.x;
-    } else {
-        if ((int)blockIdx.x >= *list_count) return;
-        tile = list[blockIdx.x];
-    }
+template <typename T>
+struct RShared {
+    using R = typename RegOf<T>::type;
+    T lm[TH][TW + 4];  // mask per window pixel (frozen pixels: mask = value)
+    T tr[TH][TW + 1];  // transpose buffer (values fit the storage type)
+    R init_l[TH], init_r[TH];
+    unsigned chg_bits;
+};
+
+// one tile of one image: load window, sweep to stability, write owned pixels, mark neighbours
+template <typename T, int CONN>
+__device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass, RShared<T>& sh, int* __restrict__ next_list,
+                                           int* __restrict__ next_count, unsigned* __restrict__ stamp) {
+    using R = typename RegOf<T>::type;
+    auto& lm = sh.lm;
+    auto& tr = sh.tr;
+    auto& init_l = sh.init_l;
+    auto& init_r = sh.init_r;
+    auto& chg_bits = sh.chg_bits;
+    const T* marker = im.marker;
+    const T* __restrict__ mask = im.mask;
+    T* out = im.out;
+    const int w = im.w, h = im.h, tiles_x = im.tiles_x, tiles_y = im.tiles_y;
     const int tx = tile % tiles_x, ty = tile / tiles_x;
     const int lane = threadIdx.x;
     const int gx = tx * OW - 1 + lane;
@@ -207,17 +223,38 @@ __global__ __launch_bounds__(64) void recon_tiles(const T* marker, const T* __re
                             (dx == -1 && (b & CH_LEFT)) || (dx == 1 && (b & CH_RIGHT));
                 const int nx = tx + dx, ny = ty + dy;
                 if (need && nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y)
-                    mark_tile(ny * tiles_x + nx, np, stamp, next_list, next_count);
+                    mark_tile(im.base + ny * tiles_x + nx, np, stamp, next_list, next_count);
             }
+    }
+}
+
+// Pass 0 runs every tile of every image; pass k > 0 runs the tiles listed by pass k - 1.  Blocks
+// stride over their work; a batch's images are found through tile_img (global tile -> image).
+// CONN 0 = per-image connectivity.
+template <typename T, int CONN>
+__global__ __launch_bounds__(64) void recon_tiles(const RImg<T>* __restrict__ imgs, RImg<T> one, const int* __restrict__ tile_img,
+                                                  int ntiles, const int* __restrict__ list, const int* __restrict__ list_count,
+                                                  int* __restrict__ next_list, int* __restrict__ next_count,
+                                                  unsigned* __restrict__ stamp, int pass) {
+    __shared__ RShared<T> sh;
+    const int n = (pass == 0) ? ntiles : *list_count;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int g = (pass == 0) ? k : list[k];
+        const RImg<T> im = imgs ? imgs[tile_img[g]] : one;
+        const int local = g - im.base;
+        if (CONN == 8 || (CONN == 0 && im.conn == 8))
+            recon_tile<T, 8>(im, local, pass, sh, next_list, next_count, stamp);
+        else
+            recon_tile<T, 4>(im, local, pass, sh, next_list, next_count, stamp);
+        __syncthreads();  // the shared window is reused by the next tile
     }
 }
 
 }  // namespace
 
+// Passes over a set of images until no tile is marked.  imgs_dev = nullptr: the single image `one`.
 template <typename T>
-int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s) {
-    const int tiles_x = (int)cdiv(w, OW), tiles_y = (int)cdiv(h, OH);
-    const int nt = tiles_x * tiles_y;
+static int recon_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, int conn, hipStream_t s) {
     const int max_pass = 4 * nt + 8;
     Scratch sc;
     const size_t bytes = sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2);
@@ -228,13 +265,17 @@ int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, h
     int* counts = listB + nt;
     NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt, s));
     NSC_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (max_pass + 2), s));
+    const unsigned grid = (unsigned)std::min(nt, 1 << 16);
     auto launch = [&](int pass, const int* lst, const int* cnt, int* nl, int* nc) {
         if (conn == 8)
-            hipLaunchKernelGGL((recon_tiles<T, 8>), dim3(nt), dim3(64), 0, s, marker, mask, out, w, h, tiles_x,
-                               tiles_y, lst, cnt, nl, nc, stamp, pass);
+            hipLaunchKernelGGL((recon_tiles<T, 8>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nt, lst, cnt, nl,
+                               nc, stamp, pass);
+        else if (conn == 4)
+            hipLaunchKernelGGL((recon_tiles<T, 4>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nt, lst, cnt, nl,
+                               nc, stamp, pass);
         else
-            hipLaunchKernelGGL((recon_tiles<T, 4>), dim3(nt), dim3(64), 0, s, marker, mask, out, w, h, tiles_x,
-                               tiles_y, lst, cnt, nl, nc, stamp, pass);
+            hipLaunchKernelGGL((recon_tiles<T, 0>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nt, lst, cnt, nl,
+                               nc, stamp, pass);
     };
     launch(0, nullptr, nullptr, listA, &counts[0]);
     NSC_HIP(hipGetLastError());
@@ -259,9 +300,57 @@ int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, h
     return NSC_OK;
 }
 
+template <typename T>
+int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s) {
+    RImg<T> one{marker, mask, out, w, h, (int)cdiv(w, OW), (int)cdiv(h, OH), conn, 0};
+    return recon_run<T>(nullptr, one, nullptr, one.tiles_x * one.tiles_y, conn, s);
+}
+
+// A batch of images in one launch per pass: the image table and the tile -> image map live in
+// stream-ordered scratch.  conn: 4 or 8 for the whole batch, 0 = per image.
+template <typename T>
+int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
+    std::vector<RImg<T>> imgs(n);
+    long long nt = 0;
+    int conn_all = d[0].conn;
+    for (int i = 0; i < n; ++i) {
+        NSC_ARG(d[i].marker && d[i].mask && d[i].out && d[i].w > 0 && d[i].h > 0 && (d[i].conn == 4 || d[i].conn == 8),
+                "imreconstruct batched: descriptor");
+        RImg<T>& r = imgs[i];
+        r.marker = static_cast<const T*>(d[i].marker);
+        r.mask = static_cast<const T*>(d[i].mask);
+        r.out = static_cast<T*>(d[i].out);
+        r.w = d[i].w;
+        r.h = d[i].h;
+        r.tiles_x = (int)cdiv(r.w, OW);
+        r.tiles_y = (int)cdiv(r.h, OH);
+        r.conn = d[i].conn;
+        r.base = (int)nt;
+        nt += (long long)r.tiles_x * r.tiles_y;
+        if (d[i].conn != conn_all) conn_all = 0;
+    }
+    NSC_ARG(nt < (1ll << 30), "imreconstruct batched: too many tiles");
+    std::vector<int> tile_img((size_t)nt);
+    for (int i = 0; i < n; ++i)
+        std::fill(tile_img.begin() + imgs[i].base, tile_img.begin() + imgs[i].base + imgs[i].tiles_x * imgs[i].tiles_y, i);
+    Scratch tab;
+    const size_t ib = ((sizeof(RImg<T>) * n + 255) / 256) * 256;
+    NSC_CHECK(tab.alloc(ib + sizeof(int) * (size_t)nt, s));
+    RImg<T>* imgs_dev = tab.as<RImg<T>>();
+    int* tile_dev = reinterpret_cast<int*>(static_cast<char*>(tab.p) + ib);
+    NSC_HIP(hipMemcpyAsync(imgs_dev, imgs.data(), sizeof(RImg<T>) * n, hipMemcpyHostToDevice, s));
+    NSC_HIP(hipMemcpyAsync(tile_dev, tile_img.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice, s));
+    const int rc = recon_run<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, s);
+    // the host tables must outlive the copies (pageable copies complete before returning; be explicit)
+    NSC_HIP(hipStreamSynchronize(s));
+    return rc;
+}
+
 template int recon_impl<uint8_t>(const uint8_t*, const uint8_t*, uint8_t*, int, int, int, hipStream_t);
 template int recon_impl<int32_t>(const int32_t*, const int32_t*, int32_t*, int, int, int, hipStream_t);
 template int recon_impl<float>(const float*, const float*, float*, int, int, int, hipStream_t);
+template int recon_batched_impl<uint8_t>(const nsc_recon_desc*, int, hipStream_t);
+template int recon_batched_impl<float>(const nsc_recon_desc*, int, hipStream_t);
 
 }  // namespace nsc
 

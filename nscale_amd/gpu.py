@@ -7,7 +7,7 @@ import ctypes
 
 import torch
 
-from nscale_amd._lib import Request, Result, SegParams, ServerConfig, call, check, lib
+from nscale_amd._lib import ReconDesc, Request, Result, SegParams, ServerConfig, call, check, lib
 
 _P = ctypes.c_void_p
 
@@ -33,6 +33,34 @@ def imreconstruct(marker, mask, conn=8):
           torch.int32: "nsc_imreconstruct_i32"}[mask.dtype]
     call(fn, _ptr(marker), _ptr(mask), _ptr(out), w, h, conn, _stream())
     return out
+
+
+def imreconstruct_batched(pairs, conns):
+    """n independent reconstructions (uint8 or float32 (marker, mask) pairs of any sizes) in one
+    launch per pass; conns: one connectivity per pair.  Returns the outputs."""
+    n = len(pairs)
+    outs = [torch.empty_like(mask) for _, mask in pairs]
+    arr = (ReconDesc * n)()
+    dt = pairs[0][1].dtype
+    for i, ((marker, mask), o) in enumerate(zip(pairs, outs)):
+        assert mask.dtype == dt and marker.shape == mask.shape
+        h, w = _hw(mask)
+        arr[i] = ReconDesc(_ptr(marker), _ptr(mask), _ptr(o), w, h, int(conns[i]))
+    fn = {torch.uint8: "nsc_imreconstruct_u8_batched", torch.float32: "nsc_imreconstruct_f32_batched"}[dt]
+    call(fn, arr, n, _stream())
+    return outs
+
+
+def imhmin_batched(imgs, hval, conns):
+    """nscale::imhmin<float> on n float32 images in one batch."""
+    n = len(imgs)
+    outs = [torch.empty_like(im) for im in imgs]
+    arr = (ReconDesc * n)()
+    for i, (im, o) in enumerate(zip(imgs, outs)):
+        h, w = _hw(im)
+        arr[i] = ReconDesc(None, _ptr(im), _ptr(o), w, h, int(conns[i]))
+    call("nsc_imhmin_f32_batched", arr, n, ctypes.c_float(hval), _stream())
+    return outs
 
 
 def _u8out(t):

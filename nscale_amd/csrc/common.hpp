@@ -156,5 +156,7 @@ template <typename T>
 int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s);
 template <typename T>
 int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s);
+// per-thread reconstruction schedule: -1 default, 0 one launch per pass, 1 one cooperative launch
+void set_thread_recon_schedule(int schedule);
 
 }  // namespace nsc

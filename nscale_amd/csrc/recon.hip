@@ -20,7 +20,11 @@
 //    the output: 3 bytes/px for uint8, the algorithmic minimum.  A tile whose owned border
 //    changed marks the neighbours whose halo it feeds; later passes re-run only marked tiles
 //    (reading out + mask) until no tile is marked.
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
+#include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -250,6 +254,39 @@ __global__ __launch_bounds__(64) void recon_tiles(const RImg<T>* __restrict__ im
     }
 }
 
+// The whole reconstruction in one cooperative launch: every pass runs over a grid-stride loop and
+// ends at a grid-wide barrier (co-residency of all blocks is guaranteed by the cooperative launch),
+// so no host round trip sits between passes.  All blocks read the same pass count after the
+// barrier, hence leave the loop together.
+template <typename T, int CONN>
+__global__ __launch_bounds__(64) void recon_persistent(const RImg<T>* __restrict__ imgs, RImg<T> one,
+                                                       const int* __restrict__ tile_img, int ntiles, int* listA, int* listB,
+                                                       int* counts, unsigned* stamp, int max_pass, int* status) {
+    __shared__ RShared<T> sh;
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    for (int pass = 0; pass <= max_pass; ++pass) {
+        const int* cur = (pass & 1) ? listB : listA;  // written by the previous pass
+        int* nxt = (pass & 1) ? listA : listB;
+        const int n = pass == 0 ? ntiles : __hip_atomic_load(&counts[pass - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (n == 0) return;
+        if (pass == max_pass) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(status, 1);
+            return;
+        }
+        for (int k = blockIdx.x; k < n; k += gridDim.x) {
+            const int g = (pass == 0) ? k : cur[k];
+            const RImg<T> im = imgs ? imgs[tile_img[g]] : one;
+            const int local = g - im.base;
+            if (CONN == 8 || (CONN == 0 && im.conn == 8))
+                recon_tile<T, 8>(im, local, pass, sh, nxt, &counts[pass], stamp);
+            else
+                recon_tile<T, 4>(im, local, pass, sh, nxt, &counts[pass], stamp);
+            __syncthreads();
+        }
+        grid.sync();
+    }
+}
+
 }  // namespace
 
 // Passes over a set of images until no tile is marked.  imgs_dev = nullptr: the single image `one`.
@@ -300,10 +337,82 @@ static int recon_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* til
     return NSC_OK;
 }
 
+static std::mutex& coop_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+// Same passes in one cooperative launch (see recon_persistent).
+template <typename T, int CONN>
+static int recon_coop_launch(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, hipStream_t s) {
+    static int max_blocks[2] = {0, 0};  // co-resident blocks of this kernel on the device (per process)
+    int& mb = max_blocks[0];
+    if (mb == 0) {
+        int dev = 0, per_cu = 0, cus = 0;
+        NSC_HIP(hipGetDevice(&dev));
+        NSC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        NSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, recon_persistent<T, CONN>, 64, 0));
+        mb = std::max(1, per_cu) * cus;
+    }
+    const int max_pass = 4 * nt + 8;
+    Scratch sc;
+    const size_t bytes = sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2) + sizeof(int);
+    NSC_CHECK(sc.alloc(bytes, s));
+    unsigned* stamp = sc.as<unsigned>();
+    int* listA = reinterpret_cast<int*>(stamp + nt);
+    int* listB = listA + nt;
+    int* counts = listB + nt;
+    int* status = counts + max_pass + 2;
+    NSC_HIP(hipMemsetAsync(stamp, 0, bytes, s));
+    dim3 grid((unsigned)std::min(nt, mb)), block(64);
+    RImg<T> one_arg = one;
+    int ntiles = nt, mp = max_pass;
+    void* args[] = {(void*)&imgs_dev, (void*)&one_arg, (void*)&tile_img, (void*)&ntiles, (void*)&listA, (void*)&listB,
+                    (void*)&counts, (void*)&stamp, (void*)&mp, (void*)&status};
+    int st = 0;
+    {
+        // two grid-barrier kernels sharing the device could each hold CUs the other waits for: the
+        // process runs one at a time, each to completion
+        std::lock_guard<std::mutex> lk(coop_mutex());
+        NSC_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(recon_persistent<T, CONN>), grid, block, args, 0, s));
+        NSC_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
+        NSC_HIP(hipStreamSynchronize(s));
+    }
+    if (st) {
+        set_error("imreconstruct: no convergence after %d passes", max_pass);
+        return NSC_ERR_LIMIT;
+    }
+    return NSC_OK;
+}
+
+template <typename T>
+static int recon_coop(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, int conn, hipStream_t s) {
+    if (conn == 8) return recon_coop_launch<T, 8>(imgs_dev, one, tile_img, nt, s);
+    if (conn == 4) return recon_coop_launch<T, 4>(imgs_dev, one, tile_img, nt, s);
+    return recon_coop_launch<T, 0>(imgs_dev, one, tile_img, nt, s);
+}
+
+// schedule: 0 = one launch per pass (host checks every 4 passes), 1 = one cooperative launch
+// (NSC_RECON_SCHEDULE=coop).  Measured on MI355X the per-pass launches are faster (the grid
+// barrier costs more than a launch, and the cooperative grid is capped at the co-resident block
+// count), so they are the default.  Threads that run many chains side by side (the server's
+// workers) always use 0 so that their reconstructions overlap instead of queueing for the
+// cooperative launch.
+static thread_local int tl_schedule = -1;
+static int recon_schedule() {
+    static int v = [] {
+        const char* e = getenv("NSC_RECON_SCHEDULE");
+        return (e && e[0] == 'c') ? 1 : 0;
+    }();
+    return tl_schedule >= 0 ? tl_schedule : v;
+}
+
 template <typename T>
 int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s) {
     RImg<T> one{marker, mask, out, w, h, (int)cdiv(w, OW), (int)cdiv(h, OH), conn, 0};
-    return recon_run<T>(nullptr, one, nullptr, one.tiles_x * one.tiles_y, conn, s);
+    const int nt = one.tiles_x * one.tiles_y;
+    if (recon_schedule() == 1) return recon_coop<T>(nullptr, one, nullptr, nt, conn, s);
+    return recon_run<T>(nullptr, one, nullptr, nt, conn, s);
 }
 
 // A batch of images in one launch per pass: the image table and the tile -> image map live in
@@ -340,7 +449,8 @@ int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
     int* tile_dev = reinterpret_cast<int*>(static_cast<char*>(tab.p) + ib);
     NSC_HIP(hipMemcpyAsync(imgs_dev, imgs.data(), sizeof(RImg<T>) * n, hipMemcpyHostToDevice, s));
     NSC_HIP(hipMemcpyAsync(tile_dev, tile_img.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice, s));
-    const int rc = recon_run<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, s);
+    const int rc = recon_schedule() == 1 ? recon_coop<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, s)
+                                         : recon_run<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, s);
     // the host tables must outlive the copies (pageable copies complete before returning; be explicit)
     NSC_HIP(hipStreamSynchronize(s));
     return rc;
@@ -349,6 +459,8 @@ int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
 template int recon_impl<uint8_t>(const uint8_t*, const uint8_t*, uint8_t*, int, int, int, hipStream_t);
 template int recon_impl<int32_t>(const int32_t*, const int32_t*, int32_t*, int, int, int, hipStream_t);
 template int recon_impl<float>(const float*, const float*, float*, int, int, int, hipStream_t);
+void set_thread_recon_schedule(int schedule) { tl_schedule = schedule; }
+
 template int recon_batched_impl<uint8_t>(const nsc_recon_desc*, int, hipStream_t);
 template int recon_batched_impl<float>(const nsc_recon_desc*, int, hipStream_t);
 

@@ -123,6 +123,7 @@ struct nsc_server {
     void worker(int k) {
         Slot& S = slots[k];
         (void)hipSetDevice(device);
+        nsc::set_thread_recon_schedule(0);
         for (;;) {
             Req* q = nullptr;
             {

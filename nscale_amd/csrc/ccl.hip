@@ -45,12 +45,29 @@ __device__ __forceinline__ bool joins(const Px& P, size_t i, size_t j) {
     return P.fv[i] == P.fv[j];
 }
 
-__device__ __forceinline__ int lfind(const int* L, int x) {
-    while (true) {
-        int p = L[x];
-        if (p == x) return x;
+// find with intermediate pointer jumping (each visited node is re-pointed to its grandparent).
+// Parents are always smaller than children, and a non-root never becomes a root again, so
+// re-pointing a non-root to any ancestor keeps every set intact; a concurrent union that lowers
+// the same pointer continues its own loop with the value it replaced (see unite), so a lost
+// update only shortens a path.
+// relaxed atomic accesses: other threads update the forest concurrently, so no access may be
+// cached or reordered by the compiler
+__device__ __forceinline__ int ld_rel(const int* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st_rel(int* a, int v) { __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ int ld_wg(const int* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+__device__ __forceinline__ void st_wg(int* a, int v) { __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+
+__device__ __forceinline__ int lfind(int* L, int x) {
+    int p = ld_wg(&L[x]);
+    if (p == x) return x;
+    int q = ld_wg(&L[p]);
+    while (p != q) {
+        st_wg(&L[x], q);
         x = p;
+        p = q;
+        q = ld_wg(&L[p]);
     }
+    return p;
 }
 
 __device__ void lunite(int* L, int a, int b) {
@@ -65,12 +82,17 @@ __device__ void lunite(int* L, int a, int b) {
     }
 }
 
-__device__ __forceinline__ int gfind(const int32_t* L, int x) {
-    while (true) {
-        int p = __builtin_nontemporal_load(&L[x]);
-        if (p == x) return x;
+__device__ __forceinline__ int gfind(int32_t* L, int x) {
+    int p = ld_rel(&L[x]);
+    if (p == x) return x;
+    int q = ld_rel(&L[p]);
+    while (p != q) {
+        st_rel(&L[x], q);
         x = p;
+        p = q;
+        q = ld_rel(&L[p]);
     }
+    return p;
 }
 
 __device__ void gunite(int32_t* L, int a, int b) {
@@ -163,9 +185,11 @@ __global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval) {
             if (bgval != -1) L[i] = bgval;
             continue;
         }
+        // read-only walk: a path-jumping find here could re-point a pixel whose final root another
+        // thread has already written back to an intermediate ancestor
         int r = p;
         while (true) {
-            int q = L[r];
+            const int q = ld_rel(&L[r]);
             if (q == r) break;
             r = q;
         }

@@ -198,3 +198,23 @@ def test_watershed2_reference_image():
     print("watershed2 reference image", flush=True)
     got = host(G().watershed2(dev(bgr), dev(img), 8))
     assert np.array_equal(got, o_watershed2(bgr, img, 8))
+
+
+def test_ccl_family_repeatable_on_chain_images():
+    """Union-find under contention: the same label/area/fill results on every repetition, equal to
+    the oracle, on the chain's own masks (huge background component, many nuclei)."""
+    from nscale_amd.synth import synth_he
+    from tests._lib import o_segment
+    bgr = synth_he(1024, 1024, 2026)
+    st, _, b = o_segment(bgr, stages=(21, 23))
+    nonov, seg = b[21], b[23]
+    exp_lab8 = o_cc_label(seg, 8)
+    exp_open = o_bwareaopen2(nonov, 21, 1000, 4)
+    inv = (255 - seg).astype(np.uint8)
+    exp_inv4 = o_cc_label(inv, 4)
+    print("ccl repeatable", flush=True)
+    for rep in range(12):
+        assert np.array_equal(host(G().cc_label(dev(seg), 8)), exp_lab8), rep
+        assert np.array_equal(host(G().cc_label(dev(inv), 4)), exp_inv4), rep
+        got, _ = G().bwareaopen2(dev(nonov), 21, 1000, 4)
+        assert np.array_equal(host(got), exp_open[0] if isinstance(exp_open, tuple) else exp_open), rep

@@ -8,7 +8,7 @@
 // Labelling is union-find with the smallest raster index as root (the reference's merge links
 // the larger root under the smaller, ConnComponents.cpp:34-43, so its labels are also the
 // smallest index of each component):
-//   1. ccl_local: 64x16 tiles in LDS, unions with atomicMin on local indices (local raster
+//   1. ccl_local: 64x64 tiles in LDS, unions with atomicMin on local indices (local raster
 //      order = global raster order inside a tile), then each pixel stores the global index of
 //      its tile-local root;
 //   2. ccl_seams: pixels on tile seams unite with their cross-tile neighbours in global memory
@@ -17,6 +17,7 @@
 // Pixel classes: FG_NEQ  -> foreground = value != key; neighbours join when both foreground
 //                FG_EQ   -> foreground = value != 0; neighbours join when values are equal
 //                FG_FEQ  -> foreground = cls != 0; neighbours join when float values are equal
+#include <algorithm>
 #include <climits>
 
 #include "common.hpp"
@@ -24,7 +25,7 @@
 namespace nsc {
 namespace {
 
-constexpr int CT_W = 64, CT_H = 16, CT_N = CT_W * CT_H;
+constexpr int CT_W = 64, CT_H = 64, CT_N = CT_W * CT_H;
 enum { FG_NEQ = 0, FG_EQ = 1, FG_FEQ = 2 };
 
 struct Px {
@@ -373,17 +374,43 @@ __global__ void k_bbox_init(int32_t* out5, int32_t* area, int n) {
     out5[4 * n + k] = -1;
     if (area) area[k] = 0;
 }
+// Each thread scans 16 consecutive pixels and issues the atomics once per run of equal labels
+// within a row (a run's x extent is its first and last pixel), instead of once per pixel.
+constexpr int BB_RUN = 16;
 __global__ void k_bbox(const int32_t* __restrict__ L, int w, int h, int n, int32_t* out5, int32_t* area) {
-    size_t total = (size_t)w * h;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-        int l = L[i];
-        if (l <= 0 || l > n) continue;
-        int k = l - 1, x = (int)(i % w), y = (int)(i / w);
-        atomicMin(&out5[n + k], x);
-        atomicMax(&out5[2 * n + k], x);
-        atomicMin(&out5[3 * n + k], y);
-        atomicMax(&out5[4 * n + k], y);
-        if (area) atomicAdd(&area[k], 1);
+    const size_t total = (size_t)w * h;
+    const size_t chunks = (total + BB_RUN - 1) / BB_RUN;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks; c += (size_t)gridDim.x * blockDim.x) {
+        const size_t i0 = c * BB_RUN, i1 = std::min(total, i0 + BB_RUN);
+        int x = (int)(i0 % w), y = (int)(i0 / w);
+        int cur = 0, x0 = 0, x1 = 0, cnt = 0;
+        auto flush = [&]() {
+            if (cur <= 0 || cur > n) return;
+            const int k = cur - 1;
+            atomicMin(&out5[n + k], x0);
+            atomicMax(&out5[2 * n + k], x1);
+            atomicMin(&out5[3 * n + k], y);
+            atomicMax(&out5[4 * n + k], y);
+            if (area) atomicAdd(&area[k], cnt);
+        };
+        for (size_t i = i0; i < i1; ++i) {
+            const int l = L[i];
+            if (l != cur) {
+                flush();
+                cur = l;
+                x0 = x;
+                cnt = 0;
+            }
+            x1 = x;
+            ++cnt;
+            if (++x == w) {  // row ends: the run ends with it
+                flush();
+                cur = 0;
+                x = 0;
+                ++y;
+            }
+        }
+        flush();
     }
 }
 
@@ -465,7 +492,7 @@ int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int3
 int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s) {
     if (n <= 0) return NSC_OK;
     hipLaunchKernelGGL(k_bbox_init, dim3(cdiv(n, 256)), dim3(256), 0, s, out5, area, n);
-    hipLaunchKernelGGL(k_bbox, grid1((size_t)w * h), dim3(256), 0, s, canon, w, h, n, out5, area);
+    hipLaunchKernelGGL(k_bbox, grid1(cdiv((size_t)w * h, BB_RUN)), dim3(256), 0, s, canon, w, h, n, out5, area);
     NSC_HIP(hipGetLastError());
     return NSC_OK;
 }

@@ -18,6 +18,37 @@ void set_error(const char* fmt, ...) {
 }
 void clear_error() { g_err[0] = 0; }
 
+namespace {
+struct SyncState {
+    bool blocking = false;
+    hipEvent_t ev = nullptr;
+    ~SyncState() {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+};
+SyncState& sync_state() {
+    static thread_local SyncState st;
+    return st;
+}
+}  // namespace
+
+void set_thread_blocking_sync(bool on) { sync_state().blocking = on; }
+
+hipError_t stream_sync(hipStream_t s) {
+    SyncState& st = sync_state();
+    if (!st.blocking) return hipStreamSynchronize(s);
+    if (!st.ev) {
+        const hipError_t e = hipEventCreateWithFlags(&st.ev, hipEventBlockingSync | hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipEventRecord(st.ev, s);
+    if (e != hipSuccess) return e;
+    e = hipEventSynchronize(st.ev);
+    if (e != hipSuccess) return e;
+    // the work is done; this returns at once and completes any pageable host copy bookkeeping
+    return hipStreamSynchronize(s);
+}
+
 SideStream& side_stream() {
     static thread_local SideStream ss;
     return ss;

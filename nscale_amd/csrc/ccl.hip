@@ -524,7 +524,7 @@ int nsc_cc_relabel(int32_t* labels, int w, int h, int32_t bgval, int* count, voi
     NSC_CHECK(relabel_impl(labels, w, h, bgval, dcount, rank, s));
     if (count) {
         NSC_HIP(hipMemcpyAsync(count, dcount, sizeof(int), hipMemcpyDeviceToHost, s));
-        NSC_HIP(hipStreamSynchronize(s));
+        NSC_HIP(stream_sync(s));
     }
     return NSC_OK;
 }
@@ -543,7 +543,7 @@ int nsc_bwareaopen2_u8(const uint8_t* img, uint8_t* out, int w, int h, int minSi
     NSC_CHECK(bwareaopen_impl(img, out, w, h, minSize, maxSize, conn, dcount, labels, area, s));
     if (count) {
         NSC_HIP(hipMemcpyAsync(count, dcount, sizeof(int), hipMemcpyDeviceToHost, s));
-        NSC_HIP(hipStreamSynchronize(s));
+        NSC_HIP(stream_sync(s));
     }
     return NSC_OK;
 }

@@ -267,7 +267,7 @@ int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p
     NSC_HIP(hipGetLastError());
     int bgArea = 0;
     NSC_HIP(hipMemcpyAsync(&bgArea, &W.counts[0], sizeof(int), hipMemcpyDeviceToHost, s));
-    NSC_HIP(hipStreamSynchronize(s));
+    NSC_HIP(stream_sync(s));
     const float ratio = (float)bgArea / (float)(w * h);  // HE.cpp:125
     mark(1);
     if (ratio >= 0.99) { finish_timing(); return NSC_SEG_BACKGROUND; }
@@ -355,7 +355,7 @@ int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p
     NSC_HIP(hipGetLastError());
     int c[4] = {0, 0, 0, 0};
     NSC_HIP(hipMemcpyAsync(c, W.counts, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-    NSC_HIP(hipStreamSynchronize(s));
+    NSC_HIP(stream_sync(s));
     finish_timing();
     if (c[1] == 0) return NSC_SEG_NO_CANDIDATES_LEFT;  // HE.cpp:272-274
     if (c[2] == 0) return NSC_SEG_NO_CANDIDATES_LEFT;  // HE.cpp:554-556

@@ -118,6 +118,11 @@ struct SideStream {
 // one per host thread: concurrent callers never share a side stream
 SideStream& side_stream();
 
+// Wait for the stream.  By default the thread spins (lowest latency); a thread that has asked for
+// blocking waits (the server's workers, many of which wait at once) sleeps on an event instead.
+hipError_t stream_sync(hipStream_t s);
+void set_thread_blocking_sync(bool on);
+
 // ---- device helpers ----------------------------------------------------------------------
 // max/min with the selection rule of std::max / std::min (first argument on ties)
 template <typename R>

@@ -198,7 +198,7 @@ int nsc_imhmin_f32_batched(const nsc_recon_desc* descs, int n, float hval, void*
     NSC_CHECK(recon_batched_impl<float>(rd.data(), n, s));
     hipLaunchKernelGGL(k_hmin_post_b, dim3(g), dim3(256), 0, s, tab.as<HB>(), n, total);
     NSC_HIP(hipGetLastError());
-    NSC_HIP(hipStreamSynchronize(s));  // host tables above are released on return
+    NSC_HIP(stream_sync(s));  // host tables above are released on return
     return NSC_OK;
 }
 

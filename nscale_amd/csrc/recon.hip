@@ -327,7 +327,7 @@ static int recon_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* til
         NSC_HIP(hipGetLastError());
         int c = 0;
         NSC_HIP(hipMemcpyAsync(&c, &counts[pass - 1], sizeof(int), hipMemcpyDeviceToHost, s));
-        NSC_HIP(hipStreamSynchronize(s));
+        NSC_HIP(stream_sync(s));
         if (c == 0) break;
         if (pass > max_pass) {
             set_error("imreconstruct: no convergence after %d passes", max_pass);
@@ -376,7 +376,7 @@ static int recon_coop_launch(const RImg<T>* imgs_dev, const RImg<T>& one, const 
         std::lock_guard<std::mutex> lk(coop_mutex());
         NSC_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(recon_persistent<T, CONN>), grid, block, args, 0, s));
         NSC_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
-        NSC_HIP(hipStreamSynchronize(s));
+        NSC_HIP(stream_sync(s));
     }
     if (st) {
         set_error("imreconstruct: no convergence after %d passes", max_pass);
@@ -452,7 +452,7 @@ int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
     const int rc = recon_schedule() == 1 ? recon_coop<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, s)
                                          : recon_run<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, s);
     // the host tables must outlive the copies (pageable copies complete before returning; be explicit)
-    NSC_HIP(hipStreamSynchronize(s));
+    NSC_HIP(stream_sync(s));
     return rc;
 }
 
@@ -645,7 +645,7 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
         NSC_HIP(hipGetLastError());
         int c = 0;
         NSC_HIP(hipMemcpyAsync(&c, &counts[p - 1], sizeof(int), hipMemcpyDeviceToHost, s));
-        NSC_HIP(hipStreamSynchronize(s));
+        NSC_HIP(stream_sync(s));
         if (c == 0) break;
         if (p > max_pass) {
             set_error("geodesic distance: no convergence after %d passes", max_pass);

@@ -114,7 +114,7 @@ struct nsc_server {
             if (r.bbox5 && n > 0) {
                 if (n > r.max_labels) { res.error = NSC_ERR_LIMIT; return; }
                 rc = nsc_bounding_boxes(r.labels, r.w, r.h, n, r.bbox5, r.area, S.stream);
-                if (rc == NSC_OK && hipStreamSynchronize(S.stream) != hipSuccess) rc = NSC_ERR_HIP;
+                if (rc == NSC_OK && nsc::stream_sync(S.stream) != hipSuccess) rc = NSC_ERR_HIP;
                 if (rc != NSC_OK) { res.error = rc; return; }
             }
         }
@@ -124,6 +124,7 @@ struct nsc_server {
         Slot& S = slots[k];
         (void)hipSetDevice(device);
         nsc::set_thread_recon_schedule(0);
+        nsc::set_thread_blocking_sync(true);
         for (;;) {
             Req* q = nullptr;
             {

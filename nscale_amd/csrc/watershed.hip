@@ -201,7 +201,7 @@ int cvws_impl(const uint8_t* img, int32_t* m, int W, int H, void* scratch, hipSt
     NSC_CHECK(relabel_impl(comp, W, H, -1, cnt.as<int>(), rank, s));
     int ncomp = 0;
     NSC_HIP(hipMemcpyAsync(&ncomp, cnt.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    NSC_HIP(hipStreamSynchronize(s));
+    NSC_HIP(stream_sync(s));
     if (ncomp == 0) return NSC_OK;
     Scratch q;
     NSC_CHECK(q.alloc(sizeof(int32_t) * ((size_t)ncomp * (5 + 2 * NQ)), s));

@@ -1211,7 +1211,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     tick("runs", &small[1]);
     int hostc[2];
     NSC_HIP(hipMemcpyAsync(hostc, small, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-    NSC_HIP(hipStreamSynchronize(s));
+    NSC_HIP(stream_sync(s));
     const int nF = hostc[0], nrun = hostc[1];
     // phase-1 scratch: run boxes, per-run FIFO heads/tails, queue links, leaf table
     const int leafcap = std::max(1024, std::min((int)std::min(N, (size_t)INT_MAX / 4), 4 * nF + 1024));
@@ -1289,7 +1289,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     NSC_CHECK(relabel_impl(lab2, W, H, -1, &small[16], head /* rank scratch: head[] is dead after k_bresolve */, s));
     int ncomp = 0;
     NSC_HIP(hipMemcpyAsync(&ncomp, &small[16], sizeof(int), hipMemcpyDeviceToHost, s));
-    NSC_HIP(hipStreamSynchronize(s));
+    NSC_HIP(stream_sync(s));
     if (ncomp > 0) {
         Scratch sc3;
         Carver m3(nullptr);
@@ -1344,7 +1344,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     int flags = 0;
     NSC_HIP(hipGetLastError());
     NSC_HIP(hipMemcpyAsync(&flags, &small[3], sizeof(int), hipMemcpyDeviceToHost, s));
-    NSC_HIP(hipStreamSynchronize(s));
+    NSC_HIP(stream_sync(s));
     *ok = (flags == 0) ? 1 : 0;
     ws_last_flags_record(flags);
     return NSC_OK;

@@ -53,6 +53,10 @@ int nsc_init(void);
 int nsc_synth_mask(uint8_t* out, int w, int h, uint64_t seed);
 int nsc_synth_marker(const uint8_t* mask, uint8_t* marker, size_t n);
 int nsc_synth_he(uint8_t* bgr, int w, int h, uint64_t seed);
+/* Tile (x0, y0, w, h) of a procedural whole-slide H&E image (DEVICE output, BGR): every pixel is
+   a function of (x, y, seed) only, so overlapping tiles agree; stain intensity varies by +-15%
+   over a 2000-px gradient-noise field.  Stream-ordered. */
+int nsc_synth_slide_tile(uint8_t* bgr, int x0, int y0, int w, int h, uint64_t seed, void* stream);
 
 /* ---- morphological reconstruction ------------------------------------------------------- */
 /* nscale::imreconstruct<uchar>(seeds, image, conn)  MorphologicOperations.h:34, .cpp:1730-1903 */

@@ -57,6 +57,7 @@ def lib():
         L.nsc_synth_mask.argtypes = [_P, _I, _I, ctypes.c_uint64]
         L.nsc_synth_he.argtypes = [_P, _I, _I, ctypes.c_uint64]
         L.nsc_synth_marker.argtypes = [_P, _P, ctypes.c_size_t]
+        L.nsc_synth_slide_tile.argtypes = [_P, _I, _I, _I, _I, ctypes.c_uint64, _P]
         L.nsc_request_bytes.restype = ctypes.c_size_t
         L.nsc_request_bytes.argtypes = [_I, _I]
         L.nsc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]

@@ -63,6 +63,14 @@ def imhmin_batched(imgs, hval, conns):
     return outs
 
 
+def synth_slide_tile(x0, y0, w, h, seed=2026, out=None):
+    """Tile (x0, y0, w, h) of the procedural whole-slide H&E image, rendered on the device (BGR)."""
+    if out is None:
+        out = torch.empty(h, w, 3, dtype=torch.uint8, device="cuda")
+    call("nsc_synth_slide_tile", _ptr(out), int(x0), int(y0), int(w), int(h), int(seed), _stream())
+    return out
+
+
 def _u8out(t):
     return torch.empty(t.shape[0], t.shape[1], dtype=torch.uint8, device=t.device)
 

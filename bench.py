@@ -290,21 +290,26 @@ def bench_slide(args):
     nuclei, px, errors, lat = 0, 0, 0, []
     ws0 = gpu.watershed_stats()
     t0 = time.perf_counter()
-    for i, (x, y, w, h) in enumerate(tiles):
-        if len(inflight) == len(ring):
-            st, n, lab, bbox, area, res = srv.wait(inflight.pop(0))
-            nuclei += n
-            lat.append(res.total_ms)
-            errors += res.error != 0
-        buf = ring[i % len(ring)].view(-1)[:h * w * 3].view(h, w, 3)
-        gpu.synth_slide_tile(x, y, w, h, 2026, buf)
-        inflight.append(srv.submit(buf, canonical=True, max_labels=65536))
-        px += w * h
-    for tk in inflight:
+    runs = []
+
+    def collect(entry):
+        nonlocal nuclei, errors
+        tk, xy = entry
         st, n, lab, bbox, area, res = srv.wait(tk)
         nuclei += n
         lat.append(res.total_ms)
+        runs.append((round(res.run_ms, 1), round(res.queue_ms, 1), xy))
         errors += res.error != 0
+
+    for i, (x, y, w, h) in enumerate(tiles):
+        if len(inflight) == len(ring):
+            collect(inflight.pop(0))
+        buf = ring[i % len(ring)].view(-1)[:h * w * 3].view(h, w, 3)
+        gpu.synth_slide_tile(x, y, w, h, 2026, buf)
+        inflight.append((srv.submit(buf, canonical=True, max_labels=65536), (x, y)))
+        px += w * h
+    for e in inflight:
+        collect(e)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     srv.close()
@@ -315,7 +320,8 @@ def bench_slide(args):
                       "tiles": len(tiles), "slots": args.slots},
            "seconds": round(wall, 2), "nuclei": nuclei, "errors": errors,
            "watershed_sequential_fallbacks": gpu.watershed_stats()[1] - ws0[1],
-           "p50_ms": round(pct(lat, 50), 1), "p99_ms": round(pct(lat, 99), 1)}
+           "p50_ms": round(pct(lat, 50), 1), "p99_ms": round(pct(lat, 99), 1),
+           "slowest_runs_ms_queue_ms_xy": sorted(runs, reverse=True)[:4]}
     print(json.dumps(out), flush=True)
     if errors:
         sys.exit(1)

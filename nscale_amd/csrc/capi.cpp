@@ -34,6 +34,11 @@ SyncState& sync_state() {
 
 void set_thread_blocking_sync(bool on) { sync_state().blocking = on; }
 
+Arena*& thread_arena() {
+    static thread_local Arena* a = nullptr;
+    return a;
+}
+
 hipError_t stream_sync(hipStream_t s) {
     SyncState& st = sync_state();
     if (!st.blocking) return hipStreamSynchronize(s);

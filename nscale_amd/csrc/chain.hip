@@ -139,7 +139,12 @@ struct Workspace {
     float *dist, *distance, *hmask, *hmarker, *d2;
     void* big;  // shared by local minima and the watershed (max of their scratch)
     int* counts;  // [0] background, [1] compcount S1, [2] compcount final-area, [3] ncomp
+    Arena arena;  // operator scratch during a chain call (separate allocation)
 };
+
+// operator scratch the chain needs at most, per pixel of the largest tile (measured peak of the
+// watershed's decomposition plus margin); beyond it operators fall back to the stream pool
+constexpr size_t kArenaPerPx = 144;
 
 static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 static size_t big_scratch(int w, int h) { return al256(std::max(local_max_scratch(w, h), ws2_scratch(w, h))); }
@@ -148,6 +153,8 @@ size_t workspace_bytes(int mw, int mh) {
     const size_t n = (size_t)mw * mh;
     return 23 * al256(n) + al256(3 * n) + 8 * al256(4 * n) + big_scratch(mw, mh) + 256;
 }
+
+size_t workspace_arena_bytes(int mw, int mh) { return kArenaPerPx * (size_t)mw * mh + ((size_t)64 << 20); }
 
 // a tile fits a workspace when its planes and its shared scratch fit (orientation-free)
 bool workspace_fits(const nsc_workspace* ws, int w, int h);
@@ -160,6 +167,12 @@ static int ws_alloc(Workspace& W, int mw, int mh) {
     W.n_cap = n;
     W.big_bytes = big;
     NSC_HIP(hipMalloc(&W.base, W.bytes));
+    W.arena.cap = workspace_arena_bytes(mw, mh);
+    if (hipMalloc(&W.arena.base, W.arena.cap) != hipSuccess) {  // optional: the pool serves instead
+        (void)hipGetLastError();
+        W.arena.base = nullptr;
+        W.arena.cap = 0;
+    }
     uint8_t* p = static_cast<uint8_t*>(W.base);
     uint8_t** u8s[] = {&W.bgm, &W.bw1, &W.bw2, &W.r2b, &W.rc, &W.rc_open, &W.rc_recon, &W.diff, &W.t1, &W.t2,
                        &W.fill1, &W.bw1_t, &W.rbcsel, &W.rbc, &W.seg_norbc, &W.seg_nohole, &W.seg_open, &W.seg_big_t,
@@ -216,6 +229,7 @@ int nsc_workspace_create(int max_w, int max_h, nsc_workspace** ws) {
 int nsc_workspace_destroy(nsc_workspace* ws) {
     if (!ws) return NSC_OK;
     if (ws->base) (void)hipFree(ws->base);
+    if (ws->arena.base) (void)hipFree(ws->arena.base);
     delete ws;
     return NSC_OK;
 }
@@ -232,6 +246,9 @@ int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p
     Workspace& W = *ws;
     const size_t n = (size_t)w * h;
     const nsc_seg_params& p = *prm;
+    W.arena.top = 0;
+    W.arena.depth = 0;
+    ArenaScope arena_scope(&W.arena);
 
     // ---- optional per-stage timing (HIP events on the stream) ----
     std::vector<std::pair<int, hipEvent_t>> marks;

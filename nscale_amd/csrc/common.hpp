@@ -44,11 +44,48 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 int ensure_init();
 
 // ---- stream-ordered scratch memory -------------------------------------------------------
-// Allocations come from the device's default memory pool (hipMallocAsync), whose release
-// threshold is raised at init so steady-state calls do not go back to the driver.
+// Scratch comes from the calling thread's arena when one is active (a chain call activates its
+// workspace's arena: a device buffer allocated once, handed out as a stack -- scopes nest, so
+// frees arrive in reverse order, and a buffer freed out of order is reclaimed when everything
+// above it is gone), else from the device's default memory pool (hipMallocAsync, release
+// threshold raised at init).  Either way the memory is stream-ordered: reuse by a later
+// operator on the same stream is safe without synchronisation.
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t cap = 0, top = 0;
+    struct Entry { size_t off, end; bool live; };
+    Entry stack[64];
+    int depth = 0;
+    void* take(size_t bytes) {
+        const size_t off = (top + 255) & ~(size_t)255;
+        if (!base || depth == 64 || off + bytes > cap) return nullptr;
+        stack[depth++] = Entry{off, off + bytes, true};
+        top = off + bytes;
+        return base + off;
+    }
+    bool give(void* p) {
+        if (!base || p < (void*)base || p >= (void*)(base + cap)) return false;
+        const size_t off = (size_t)((uint8_t*)p - base);
+        for (int k = depth - 1; k >= 0; --k)
+            if (stack[k].off == off && stack[k].live) { stack[k].live = false; break; }
+        while (depth > 0 && !stack[depth - 1].live) --depth;
+        top = depth ? stack[depth - 1].end : 0;
+        return true;
+    }
+};
+Arena*& thread_arena();
+
+// activates an arena for the calling thread for the lifetime of the guard
+struct ArenaScope {
+    Arena* prev;
+    explicit ArenaScope(Arena* a) : prev(thread_arena()) { thread_arena() = a; }
+    ~ArenaScope() { thread_arena() = prev; }
+};
+
 struct Scratch {
     void* p = nullptr;
     hipStream_t s = nullptr;
+    bool from_arena = false;
     Scratch() = default;
     Scratch(const Scratch&) = delete;
     Scratch& operator=(const Scratch&) = delete;
@@ -57,6 +94,12 @@ struct Scratch {
         release();
         s = stream;
         if (bytes == 0) bytes = 256;
+        if (Arena* a = thread_arena()) {
+            if ((p = a->take(bytes)) != nullptr) {
+                from_arena = true;
+                return NSC_OK;
+            }
+        }
         hipError_t e = hipMallocAsync(&p, bytes, stream);
         if (e != hipSuccess) {
             p = nullptr;
@@ -66,8 +109,16 @@ struct Scratch {
         return NSC_OK;
     }
     void release() {
-        if (p) (void)hipFreeAsync(p, s);
+        if (p) {
+            if (from_arena) {
+                Arena* a = thread_arena();
+                if (!a || !a->give(p)) { /* arena gone: nothing to return */ }
+            } else {
+                (void)hipFreeAsync(p, s);
+            }
+        }
         p = nullptr;
+        from_arena = false;
     }
     template <typename T>
     T* as() const { return static_cast<T*>(p); }

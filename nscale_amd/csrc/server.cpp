@@ -22,6 +22,7 @@
 
 namespace nsc {
 size_t workspace_bytes(int max_w, int max_h);
+size_t workspace_arena_bytes(int max_w, int max_h);
 bool workspace_fits(const nsc_workspace* ws, int w, int h);
 }  // namespace nsc
 
@@ -174,7 +175,7 @@ size_t nsc_request_bytes(int w, int h) { return (size_t)w * h * kScratchPerPx + 
 int nsc_server_create(const nsc_server_config* cfg, nsc_server** out) {
     NSC_ARG(cfg && out && cfg->slots > 0 && cfg->slots <= 64 && cfg->max_w > 0 && cfg->max_h > 0, "server_create");
     NSC_CHECK(nsc::ensure_init());
-    const size_t ws_bytes = nsc::workspace_bytes(cfg->max_w, cfg->max_h);
+    const size_t ws_bytes = nsc::workspace_bytes(cfg->max_w, cfg->max_h) + nsc::workspace_arena_bytes(cfg->max_w, cfg->max_h);
     const size_t fixed = ws_bytes * cfg->slots;
     if (cfg->hbm_budget < fixed + nsc_request_bytes(cfg->max_w, cfg->max_h)) {
         set_error("server_create: HBM budget %zu B below the %d slot workspaces (%zu B) plus one largest request",

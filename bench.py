@@ -101,8 +101,19 @@ def bench_recon(args):
            "config": {"workload": "imreconstruct<uchar> conn 8, %dx%d ellipse mask, marker = mask - 10" % (S, S)},
            "p50_ms": round(statistics.median(ms), 4),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_ms": round(avg, 4)}}
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic("recon_c0") if S == 4096 else None, "avg_ms": round(avg, 4)}}
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(key):
+    """HBM bytes per imreconstruct call measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    calibrated on a plain copy) for the 4096x4096 workloads; None if not recorded."""
+    tf = os.path.join(ROOT, "profiles", "recon_u8_4096_traffic.json")
+    try:
+        return json.load(open(tf))[key]["bytes_per_call"]
+    except Exception:
+        return None
 
 
 def checksum(lab):
@@ -442,13 +453,7 @@ def main():
     rt = sum(recon_ms) / len(recon_ms)
     alg_bytes = 3 * S * S
     achieved = alg_bytes / (rt * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "recon_u8_4096_traffic.json")
-    if os.path.exists(tf):
-        try:
-            traffic = json.load(open(tf)).get("bytes_per_call")
-        except Exception:
-            traffic = None
+    traffic = measured_traffic("recon_in_chain") if S == 4096 else None
     del ws
 
     # ---- served path: a batch of B tiles per step through the continuous-batching server ----

@@ -246,6 +246,7 @@ def bench_batched(args):
         return (time.perf_counter() - t0) / reps, r
 
     reps = max(1, args.steps)
+    tuned = gpu.autotune_recon(pairs, conns)  # schedule for this batch (used by both paths below)
     t_plain, outs_p = timed(lambda: [gpu.imreconstruct(mk, m, c) for (mk, m), c in zip(pairs, conns)], reps)
     t_batch, outs_b = timed(lambda: gpu.imreconstruct_batched(pairs, conns), reps)
     exact_u8 = all(torch.equal(a, b) for a, b in zip(outs_p, outs_b))
@@ -265,6 +266,7 @@ def bench_batched(args):
            "config": {"workload": "%d mask/marker pairs, sides log-uniform [512,4096] x aspect U[1,2], conn 4/8" % len(sizes),
                       "megapixels": round(px / 1e6, 1)},
            "plain_mpix_s": round(px / t_plain / 1e6, 1), "batched_vs_plain": round(t_plain / t_batch, 2),
+           "autotuned": tuned,
            "bit_exact": exact_u8 and exact_f32,
            "hminima_f32": {"batched_mpix_s": round(px / t_hbatch / 1e6, 1), "plain_mpix_s": round(px / t_hplain / 1e6, 1),
                            "batched_vs_plain": round(t_hplain / t_hbatch, 2)},

@@ -80,6 +80,13 @@ typedef struct nsc_recon_desc {
 } nsc_recon_desc;
 int nsc_imreconstruct_u8_batched(const nsc_recon_desc* descs, int n, void* stream);
 int nsc_imreconstruct_f32_batched(const nsc_recon_desc* descs, int n, void* stream);
+/* Autotuner of the reconstruction schedule (no reference counterpart; the reference has fixed
+   shapes): times each candidate (windows per workgroup side 1 or 2, 4 or 8 passes between host
+   checks) on this batch -- outputs are written each time -- and keeps the fastest for later
+   calls of the same element type (is_float 0: uint8, 1: float).  region / passes (may be NULL)
+   receive the choice, timings (may be NULL, 4 floats) the measured ms per candidate. */
+int nsc_recon_autotune(const nsc_recon_desc* descs, int n, int is_float, void* stream, int* region, int* passes,
+                       float* timings);
 /* nscale::imhmin<float>(image, h, conn) on n images (MorphologicOperations.cpp:3342-3364):
    descs[i].mask = input image, descs[i].out = output, descs[i].marker unused */
 int nsc_imhmin_f32_batched(const nsc_recon_desc* descs, int n, float hval, void* stream);

@@ -70,6 +70,8 @@ def lib():
         L.nsc_imreconstruct_u8_batched.argtypes = [ctypes.POINTER(ReconDesc), _I, _P]
         L.nsc_imreconstruct_f32_batched.argtypes = [ctypes.POINTER(ReconDesc), _I, _P]
         L.nsc_imhmin_f32_batched.argtypes = [ctypes.POINTER(ReconDesc), _I, ctypes.c_float, _P]
+        L.nsc_recon_autotune.argtypes = [ctypes.POINTER(ReconDesc), _I, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                         ctypes.POINTER(ctypes.c_float)]
         _lib = L
     return _lib
 

@@ -23,6 +23,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <mutex>
 #include <vector>
@@ -57,6 +58,7 @@ struct RImg {
     const T* mask;
     T* out;
     int w, h, tiles_x, tiles_y, conn, base;
+    int rgx, rgy;  // region grid (G x G windows per region) of the launch; base counts regions
 };
 
 template <typename T>
@@ -254,6 +256,236 @@ __global__ __launch_bounds__(64) void recon_tiles(const RImg<T>* __restrict__ im
     }
 }
 
+// ---- regions of G x G windows per workgroup --------------------------------------------
+// One wave per window as above; the windows of a region exchange their shared halos through LDS
+// after every round (so information crosses G windows per pass instead of one), only the region's
+// outer halo is frozen for the pass.  Halo refreshes take the max of the old and the published
+// value (reconstruction by dilation only raises values), so a value published a round late is
+// still a valid bound; the region stops when a whole round changes nothing anywhere.
+template <typename T, int G>
+struct RRegion {
+    using R = typename RegOf<T>::type;
+    T lm[G * G][TH][TW + 4];
+    T tr[G * G][TH][TW + 1];
+    R init_l[G * G][TH], init_r[G * G][TH];
+    T xtop[G * G][TW], xbot[G * G][TW], xleft[G * G][TH], xright[G * G][TH];
+    unsigned chg_bits;
+};
+
+template <typename T, int CONN, int G>
+__device__ __forceinline__ void recon_region(const RImg<T>& im, int region, int pass, RRegion<T, G>& sh,
+                                             int* __restrict__ next_list, int* __restrict__ next_count,
+                                             unsigned* __restrict__ stamp) {
+    using R = typename RegOf<T>::type;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wx = wv % G, wy = wv / G;
+    auto& lm = sh.lm[wv];
+    auto& tr = sh.tr[wv];
+    const T* marker = im.marker;
+    const T* __restrict__ mask = im.mask;
+    T* out = im.out;
+    const int w = im.w, h = im.h;
+    const int rx = region % im.rgx, ry = region / im.rgx;
+    const int tx = rx * G + wx, ty = ry * G + wy;
+    const bool valid = tx < im.tiles_x && ty < im.tiles_y;
+    const bool has_l = valid && wx > 0, has_u = valid && wy > 0;
+    const bool has_r = valid && wx < G - 1 && tx + 1 < im.tiles_x, has_d = valid && wy < G - 1 && ty + 1 < im.tiles_y;
+    const int gx = tx * OW - 1 + lane;
+    const bool col_in = (gx >= 0 && gx < w);
+    const bool col_frozen = (lane == 0 || lane == TW - 1 || !col_in);
+    if (threadIdx.x == 0) sh.chg_bits = 0;
+
+    R v[TH];
+    R init_top = 0, init_bot = 0;
+    if (valid) {
+#pragma unroll
+        for (int y = 0; y < TH; ++y) {
+            const int gy = ty * OH - 1 + y;
+            const bool inb = col_in && gy >= 0 && gy < h;
+            R val = 0, mk = 0;
+            if (inb) {
+                const size_t i = (size_t)gy * w + gx;
+                mk = (R)mask[i];
+                const R a = (pass == 0) ? (R)marker[i] : (R)out[i];
+                val = smin(a, mk);
+            }
+            const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb;
+            v[y] = val;
+            lm[y][lane] = (T)(frozen ? val : mk);
+        }
+        if (lane == 1) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) { sh.init_l[wv][y] = v[y]; sh.xleft[wv][y] = (T)v[y]; }
+        }
+        if (lane == TW - 2) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) { sh.init_r[wv][y] = v[y]; sh.xright[wv][y] = (T)v[y]; }
+        }
+        init_top = v[1];
+        init_bot = v[TH - 2];
+        sh.xtop[wv][lane] = (T)v[1];
+        sh.xbot[wv][lane] = (T)v[TH - 2];
+    }
+    __syncthreads();
+
+    bool owned_change = false;
+    for (int round = 0; round < 65536; ++round) {
+        bool changed = false, swept = false;
+        if (valid) {
+            // shared top/bottom halos from the siblings' last published rows (column layout)
+            if (has_u) {
+                const R n = (R)sh.xbot[wv - G][lane];
+                if (v[0] < n) { v[0] = n; lm[0][lane] = (T)n; changed = true; }
+            }
+            if (has_d) {
+                const R n = (R)sh.xtop[wv + G][lane];
+                if (v[TH - 1] < n) { v[TH - 1] = n; lm[TH - 1][lane] = (T)n; changed = true; }
+            }
+            {
+                R pc = v[0];
+                R pl = from_left(pc, pc), pr = from_right(pc, pc);
+#pragma unroll
+                for (int y = 1; y < TH - 1; ++y) {
+                    R up = pc;
+                    if (CONN == 8) up = smax(up, smax(pl, pr));
+                    const R mk = (R)lm[y][lane];
+                    const R nv = smin(smax(v[y], up), mk);
+                    swept |= (nv != v[y]);
+                    v[y] = nv;
+                    pc = nv;
+                    if (CONN == 8) { pl = from_left(pc, pc); pr = from_right(pc, pc); }
+                }
+            }
+            {
+                R pc = v[TH - 1];
+                R pl = from_left(pc, pc), pr = from_right(pc, pc);
+#pragma unroll
+                for (int y = TH - 2; y >= 1; --y) {
+                    R dn = pc;
+                    if (CONN == 8) dn = smax(dn, smax(pl, pr));
+                    const R mk = (R)lm[y][lane];
+                    const R nv = smin(smax(v[y], dn), mk);
+                    swept |= (nv != v[y]);
+                    v[y] = nv;
+                    pc = nv;
+                    if (CONN == 8) { pl = from_left(pc, pc); pr = from_right(pc, pc); }
+                }
+            }
+#pragma unroll
+            for (int y = 0; y < TH; ++y) tr[y][lane] = (T)v[y];
+        }
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int x = 0; x < TW; ++x) v[x] = (R)tr[lane][x];
+            // shared left/right halos (row layout: lane = row)
+            if (has_l) {
+                const R n = (R)sh.xright[wv - 1][lane];
+                if (v[0] < n) { v[0] = n; lm[lane][0] = (T)n; changed = true; }
+            }
+            if (has_r) {
+                const R n = (R)sh.xleft[wv + 1][lane];
+                if (v[TW - 1] < n) { v[TW - 1] = n; lm[lane][TW - 1] = (T)n; changed = true; }
+            }
+            R prev = v[0];
+#pragma unroll
+            for (int x = 1; x < TW - 1; ++x) {
+                const R mk = (R)lm[lane][x];
+                const R nv = smin(smax(v[x], prev), mk);
+                swept |= (nv != v[x]);
+                v[x] = nv;
+                prev = nv;
+            }
+            prev = v[TW - 1];
+#pragma unroll
+            for (int x = TW - 2; x >= 1; --x) {
+                const R mk = (R)lm[lane][x];
+                const R nv = smin(smax(v[x], prev), mk);
+                swept |= (nv != v[x]);
+                v[x] = nv;
+                prev = nv;
+            }
+            sh.xleft[wv][lane] = (T)v[1];
+            sh.xright[wv][lane] = (T)v[TW - 2];
+#pragma unroll
+            for (int x = 0; x < TW; ++x) tr[lane][x] = (T)v[x];
+        }
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) v[y] = (R)tr[y][lane];
+            sh.xtop[wv][lane] = (T)v[1];
+            sh.xbot[wv][lane] = (T)v[TH - 2];
+        }
+        owned_change |= swept;
+        if (!__syncthreads_or((changed || swept) ? 1 : 0)) break;
+    }
+
+    const bool wave_owned = __any(owned_change);
+    if (valid && (pass == 0 || wave_owned) && !col_frozen) {
+#pragma unroll
+        for (int y = 1; y < TH - 1; ++y) {
+            const int gy = ty * OH - 1 + y;
+            if (gy < h) out[(size_t)gy * w + gx] = (T)v[y];
+        }
+    }
+    // region-border changes -> mark neighbouring regions
+    unsigned bits = 0;
+    if (valid) {
+        const bool edge_l = wx == 0, edge_r = wx == G - 1 || tx == im.tiles_x - 1;
+        const bool edge_u = wy == 0, edge_d = wy == G - 1 || ty == im.tiles_y - 1;
+        if (lane >= 1 && lane <= TW - 2) {
+            if (edge_u && v[1] != init_top) bits |= CH_TOP;
+            if (edge_d && v[TH - 2] != init_bot) bits |= CH_BOT;
+        }
+        if (lane == 1 && edge_l) {
+#pragma unroll
+            for (int y = 1; y < TH - 1; ++y)
+                if (v[y] != sh.init_l[wv][y]) bits |= CH_LEFT;
+        }
+        if (lane == TW - 2 && edge_r) {
+#pragma unroll
+            for (int y = 1; y < TH - 1; ++y)
+                if (v[y] != sh.init_r[wv][y]) bits |= CH_RIGHT;
+        }
+    }
+    if (bits) atomicOr(&sh.chg_bits, bits);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned b = sh.chg_bits;
+        const unsigned np = (unsigned)pass + 1;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) ||
+                            (dx == -1 && (b & CH_LEFT)) || (dx == 1 && (b & CH_RIGHT));
+                const int nx = rx + dx, ny = ry + dy;
+                if (need && nx >= 0 && nx < im.rgx && ny >= 0 && ny < im.rgy)
+                    mark_tile(im.base + ny * im.rgx + nx, np, stamp, next_list, next_count);
+            }
+    }
+}
+
+template <typename T, int CONN, int G>
+__global__ __launch_bounds__(64 * G * G) void recon_regions(const RImg<T>* __restrict__ imgs, RImg<T> one,
+                                                            const int* __restrict__ tile_img, int ntiles,
+                                                            const int* __restrict__ list, const int* __restrict__ list_count,
+                                                            int* __restrict__ next_list, int* __restrict__ next_count,
+                                                            unsigned* __restrict__ stamp, int pass) {
+    __shared__ RRegion<T, G> sh;
+    const int n = (pass == 0) ? ntiles : *list_count;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int g = (pass == 0) ? k : list[k];
+        const RImg<T> im = imgs ? imgs[tile_img[g]] : one;
+        const int local = g - im.base;
+        if (CONN == 8 || (CONN == 0 && im.conn == 8))
+            recon_region<T, 8, G>(im, local, pass, sh, next_list, next_count, stamp);
+        else
+            recon_region<T, 4, G>(im, local, pass, sh, next_list, next_count, stamp);
+        __syncthreads();
+    }
+}
+
 // The whole reconstruction in one cooperative launch: every pass runs over a grid-stride loop and
 // ends at a grid-wide barrier (co-residency of all blocks is guaranteed by the cooperative launch),
 // so no host round trip sits between passes.  All blocks read the same pass count after the
@@ -290,8 +522,39 @@ __global__ __launch_bounds__(64) void recon_persistent(const RImg<T>* __restrict
 }  // namespace
 
 // Passes over a set of images until no tile is marked.  imgs_dev = nullptr: the single image `one`.
+template <typename T, int CONN, int G>
+static void launch_pass(int grid, hipStream_t s, const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt,
+                        const int* lst, const int* cnt, int* nl, int* nc, unsigned* stamp, int pass) {
+    if (G == 1)
+        hipLaunchKernelGGL((recon_tiles<T, CONN>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc,
+                           stamp, pass);
+    else
+        hipLaunchKernelGGL((recon_regions<T, CONN, G>), dim3(grid), dim3(64 * G * G), 0, s, imgs_dev, one, tile_img, nt, lst,
+                           cnt, nl, nc, stamp, pass);
+}
+
+template <typename T, int G>
+static void launch_conn(int conn, int grid, hipStream_t s, const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img,
+                        int nt, const int* lst, const int* cnt, int* nl, int* nc, unsigned* stamp, int pass) {
+    if (conn == 8) launch_pass<T, 8, G>(grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
+    else if (conn == 4) launch_pass<T, 4, G>(grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
+    else launch_pass<T, 0, G>(grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
+}
+
+// largest region side the workgroup's LDS holds for the element type
+template <typename T> constexpr int max_region() { return sizeof(T) == 1 ? 4 : 2; }
+
+// Tunables of the per-pass schedule, per element size (0: u8, 1: 4-byte): region side and the
+// number of passes launched between host checks.  Defaults measured on MI355X; the autotuner
+// (nsc_recon_autotune) replaces them with the fastest setting on a caller's batch.
+struct ReconTune {
+    std::atomic<int> region{0}, batch{4};
+};
+static ReconTune g_tune[2];
+template <typename T> static ReconTune& tune() { return g_tune[sizeof(T) == 1 ? 0 : 1]; }
+
 template <typename T>
-static int recon_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, int conn, hipStream_t s) {
+static int recon_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, int conn, int G, hipStream_t s) {
     const int max_pass = 4 * nt + 8;
     Scratch sc;
     const size_t bytes = sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2);
@@ -302,23 +565,22 @@ static int recon_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* til
     int* counts = listB + nt;
     NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt, s));
     NSC_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (max_pass + 2), s));
-    const unsigned grid = (unsigned)std::min(nt, 1 << 16);
+    const int grid = std::min(nt, 1 << 16);
     auto launch = [&](int pass, const int* lst, const int* cnt, int* nl, int* nc) {
-        if (conn == 8)
-            hipLaunchKernelGGL((recon_tiles<T, 8>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nt, lst, cnt, nl,
-                               nc, stamp, pass);
-        else if (conn == 4)
-            hipLaunchKernelGGL((recon_tiles<T, 4>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nt, lst, cnt, nl,
-                               nc, stamp, pass);
-        else
-            hipLaunchKernelGGL((recon_tiles<T, 0>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nt, lst, cnt, nl,
-                               nc, stamp, pass);
+        switch (G) {
+            case 2: launch_conn<T, 2>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass); break;
+            case 3: launch_conn<T, (max_region<T>() >= 3 ? 3 : 2)>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl,
+                                                                    nc, stamp, pass); break;
+            case 4: launch_conn<T, max_region<T>()>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
+                    break;
+            default: launch_conn<T, 1>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
+        }
     };
     launch(0, nullptr, nullptr, listA, &counts[0]);
     NSC_HIP(hipGetLastError());
     int pass = 1;
     for (;;) {
-        const int batch = 4;
+        const int batch = std::max(1, tune<T>().batch.load());
         for (int b = 0; b < batch && pass <= max_pass; ++b, ++pass) {
             int* cur = (pass & 1) ? listA : listB;
             int* nxt = (pass & 1) ? listB : listA;
@@ -407,12 +669,36 @@ static int recon_schedule() {
     return tl_schedule >= 0 ? tl_schedule : v;
 }
 
+// region side (windows per workgroup side) for element type T: NSC_RECON_REGION overrides
+template <typename T>
+static int region_side() {
+    static int env = [] {
+        const char* e = getenv("NSC_RECON_REGION");
+        return e ? std::max(1, std::min(atoi(e), max_region<T>())) : 0;
+    }();
+    if (env) return env;
+    // measured default 1: 2 ties with it on the 4096^2 workloads, 3 and 4 are slower (a region's
+    // rounds wait on its slowest window)
+    const int t = tune<T>().region.load();
+    return t ? std::min(t, max_region<T>()) : 1;
+}
+
+template <typename T>
+static void set_regions(RImg<T>& r, int G) {
+    r.rgx = (int)cdiv(r.tiles_x, G);
+    r.rgy = (int)cdiv(r.tiles_y, G);
+}
+
 template <typename T>
 int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s) {
-    RImg<T> one{marker, mask, out, w, h, (int)cdiv(w, OW), (int)cdiv(h, OH), conn, 0};
-    const int nt = one.tiles_x * one.tiles_y;
-    if (recon_schedule() == 1) return recon_coop<T>(nullptr, one, nullptr, nt, conn, s);
-    return recon_run<T>(nullptr, one, nullptr, nt, conn, s);
+    RImg<T> one{marker, mask, out, w, h, (int)cdiv(w, OW), (int)cdiv(h, OH), conn, 0, 0, 0};
+    if (recon_schedule() == 1) {
+        set_regions(one, 1);
+        return recon_coop<T>(nullptr, one, nullptr, one.tiles_x * one.tiles_y, conn, s);
+    }
+    const int G = region_side<T>();
+    set_regions(one, G);
+    return recon_run<T>(nullptr, one, nullptr, one.rgx * one.rgy, conn, G, s);
 }
 
 // A batch of images in one launch per pass: the image table and the tile -> image map live in
@@ -422,6 +708,7 @@ int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
     std::vector<RImg<T>> imgs(n);
     long long nt = 0;
     int conn_all = d[0].conn;
+    const int G = recon_schedule() == 1 ? 1 : region_side<T>();
     for (int i = 0; i < n; ++i) {
         NSC_ARG(d[i].marker && d[i].mask && d[i].out && d[i].w > 0 && d[i].h > 0 && (d[i].conn == 4 || d[i].conn == 8),
                 "imreconstruct batched: descriptor");
@@ -435,13 +722,14 @@ int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
         r.tiles_y = (int)cdiv(r.h, OH);
         r.conn = d[i].conn;
         r.base = (int)nt;
-        nt += (long long)r.tiles_x * r.tiles_y;
+        set_regions(r, G);
+        nt += (long long)r.rgx * r.rgy;
         if (d[i].conn != conn_all) conn_all = 0;
     }
     NSC_ARG(nt < (1ll << 30), "imreconstruct batched: too many tiles");
     std::vector<int> tile_img((size_t)nt);
     for (int i = 0; i < n; ++i)
-        std::fill(tile_img.begin() + imgs[i].base, tile_img.begin() + imgs[i].base + imgs[i].tiles_x * imgs[i].tiles_y, i);
+        std::fill(tile_img.begin() + imgs[i].base, tile_img.begin() + imgs[i].base + imgs[i].rgx * imgs[i].rgy, i);
     Scratch tab;
     const size_t ib = ((sizeof(RImg<T>) * n + 255) / 256) * 256;
     NSC_CHECK(tab.alloc(ib + sizeof(int) * (size_t)nt, s));
@@ -450,7 +738,7 @@ int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
     NSC_HIP(hipMemcpyAsync(imgs_dev, imgs.data(), sizeof(RImg<T>) * n, hipMemcpyHostToDevice, s));
     NSC_HIP(hipMemcpyAsync(tile_dev, tile_img.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice, s));
     const int rc = recon_schedule() == 1 ? recon_coop<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, s)
-                                         : recon_run<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, s);
+                                         : recon_run<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, G, s);
     // the host tables must outlive the copies (pageable copies complete before returning; be explicit)
     NSC_HIP(stream_sync(s));
     return rc;
@@ -461,8 +749,66 @@ template int recon_impl<int32_t>(const int32_t*, const int32_t*, int32_t*, int, 
 template int recon_impl<float>(const float*, const float*, float*, int, int, int, hipStream_t);
 void set_thread_recon_schedule(int schedule) { tl_schedule = schedule; }
 
+// Times every (region side, passes per host check) candidate on the caller's batch -- the
+// outputs are written each time, identically -- and keeps the fastest for later calls of the
+// same element type.  timings (optional, host) receives one entry per candidate in ms.
+template <typename T>
+static int recon_autotune(const nsc_recon_desc* d, int n, hipStream_t s, int* region, int* batch, float* timings) {
+    const int regions[2] = {1, 2}, batches[2] = {4, 8};
+    ReconTune& t = tune<T>();
+    const int r0 = t.region.load(), b0 = t.batch.load();
+    hipEvent_t e0, e1;
+    NSC_HIP(hipEventCreate(&e0));
+    NSC_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bi = 0, k = 0;
+    for (int ri = 0; ri < 2; ++ri)
+        for (int b = 0; b < 2; ++b, ++k) {
+            t.region = regions[ri];
+            t.batch = batches[b];
+            int rc = recon_batched_impl<T>(d, n, s);  // warm
+            float ms = 0.f;
+            if (rc == NSC_OK) {
+                (void)hipEventRecord(e0, s);
+                rc = recon_batched_impl<T>(d, n, s);
+                (void)hipEventRecord(e1, s);
+                (void)hipEventSynchronize(e1);
+                (void)hipEventElapsedTime(&ms, e0, e1);
+            }
+            if (rc != NSC_OK) {
+                t.region = r0;
+                t.batch = b0;
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return rc;
+            }
+            if (timings) timings[k] = ms;
+            if (ms < best) { best = ms; bi = k; }
+        }
+    t.region = regions[bi / 2];
+    t.batch = batches[bi % 2];
+    if (region) *region = t.region;
+    if (batch) *batch = t.batch;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return NSC_OK;
+}
+
 template int recon_batched_impl<uint8_t>(const nsc_recon_desc*, int, hipStream_t);
 template int recon_batched_impl<float>(const nsc_recon_desc*, int, hipStream_t);
+
+}  // namespace nsc
+
+extern "C" int nsc_recon_autotune(const nsc_recon_desc* descs, int n, int is_float, void* stream, int* region, int* passes,
+                                  float* timings) {
+    NSC_ARG(descs && n > 0, "recon_autotune");
+    NSC_CHECK(nsc::ensure_init());
+    hipStream_t s = nsc::as_stream(stream);
+    return is_float ? nsc::recon_autotune<float>(descs, n, s, region, passes, timings)
+                    : nsc::recon_autotune<uint8_t>(descs, n, s, region, passes, timings);
+}
+
+namespace nsc {
 
 }  // namespace nsc
 

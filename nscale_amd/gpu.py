@@ -51,6 +51,23 @@ def imreconstruct_batched(pairs, conns):
     return outs
 
 
+def autotune_recon(pairs, conns):
+    """Pick the reconstruction schedule (windows per workgroup side, passes per host check) by timing
+    each candidate on this batch; later calls of the same element type use the choice.
+    Returns {"region": G, "passes": P, "timings_ms": [...]} (candidates (1,4), (1,8), (2,4), (2,8))."""
+    n = len(pairs)
+    arr = (ReconDesc * n)()
+    outs = [torch.empty_like(mask) for _, mask in pairs]
+    for i, ((marker, mask), o) in enumerate(zip(pairs, outs)):
+        h, w = _hw(mask)
+        arr[i] = ReconDesc(_ptr(marker), _ptr(mask), _ptr(o), w, h, int(conns[i]))
+    g, p = ctypes.c_int(0), ctypes.c_int(0)
+    t = (ctypes.c_float * 4)()
+    call("nsc_recon_autotune", arr, n, 1 if pairs[0][1].dtype == torch.float32 else 0, _stream(), ctypes.byref(g),
+         ctypes.byref(p), t)
+    return {"region": g.value, "passes": p.value, "timings_ms": [round(x, 3) for x in t]}
+
+
 def imhmin_batched(imgs, hval, conns):
     """nscale::imhmin<float> on n float32 images in one batch."""
     n = len(imgs)

@@ -73,3 +73,16 @@ def test_batched_single_pair_and_tiny_images():
         assert np.array_equal(o.cpu().numpy(), o_recon(mk, m, 8))
     one = gpu.imreconstruct_batched(dev[3:4], [4])
     assert np.array_equal(one[0].cpu().numpy(), o_recon(pairs[3][0], pairs[3][1], 4))
+
+
+def test_autotuner_choice_keeps_results():
+    from nscale_amd import gpu
+    pairs = _pairs(8, 21)
+    dev = [(torch.from_numpy(mk).cuda(), torch.from_numpy(m).cuda()) for mk, m, _ in pairs]
+    conns = [c for _, _, c in pairs]
+    ref = gpu.imreconstruct_batched(dev, conns)
+    t = gpu.autotune_recon(dev, conns)
+    assert t["region"] in (1, 2) and t["passes"] in (4, 8) and all(x > 0 for x in t["timings_ms"])
+    got = gpu.imreconstruct_batched(dev, conns)
+    torch.cuda.synchronize()
+    assert all(torch.equal(a, b) for a, b in zip(ref, got))

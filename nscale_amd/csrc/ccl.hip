@@ -219,10 +219,23 @@ int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipS
 // ------------------------------------------------------------------------------------------
 // operators on top of the labels
 // ------------------------------------------------------------------------------------------
+// one atomic per run of equal labels within a thread's 16 consecutive pixels (the background
+// component would otherwise take one atomic per pixel on a single address)
 __global__ void k_area(const int32_t* __restrict__ L, int32_t* __restrict__ area, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        int r = L[i];
-        if (r >= 0) atomicAdd(&area[r], 1);
+    const size_t chunks = (n + 15) / 16;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks; c += (size_t)gridDim.x * blockDim.x) {
+        const size_t i0 = c * 16, i1 = i0 + 16 < n ? i0 + 16 : n;
+        int cur = -1, cnt = 0;
+        for (size_t i = i0; i < i1; ++i) {
+            const int r = L[i];
+            if (r != cur) {
+                if (cur >= 0) atomicAdd(&area[cur], cnt);
+                cur = r;
+                cnt = 0;
+            }
+            ++cnt;
+        }
+        if (cur >= 0) atomicAdd(&area[cur], cnt);
     }
 }
 

@@ -49,23 +49,55 @@ const uint8_t kCross3[9] = {0, 1, 0, 1, 1, 1, 0, 1, 0};  // getStructuringElemen
 inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u)); }
 
 // getBackground (HE.cpp:93-109) count, getRBC planes (HE.cpp:33-82), rc = 255 - R (HE.cpp:158)
+__device__ __forceinline__ void s1_pixel(uint8_t B, uint8_t G, uint8_t R, uint8_t blue, uint8_t green, uint8_t red, float t1,
+                                         float t2, uint32_t& bg, uint32_t& b1, uint32_t& b2, uint32_t& rb, uint32_t& rcv,
+                                         int k, int& local) {
+    const bool isbg = (B > blue) && (G > green) && (R > red);
+    local += isbg;
+    const float bd = (float)B + FLT_EPSILON;
+    const float gd = (float)G + FLT_EPSILON;
+    const float rd = (float)R;
+    const float q = rd / gd;  // IEEE division, correctly rounded (no fast-math)
+    bg |= (isbg ? 255u : 0u) << (8 * k);
+    b1 |= ((q > t1) ? 255u : 0u) << (8 * k);
+    b2 |= ((q > t2) ? 255u : 0u) << (8 * k);
+    rb |= ((rd / bd > 1.0f) ? 255u : 0u) << (8 * k);
+    rcv |= (uint32_t)(uint8_t)(255 - R) << (8 * k);
+}
+
+// four pixels per thread: 12 BGR bytes in as three words, one word out per plane
 __global__ void k_s1_pixels(const uint8_t* __restrict__ bgr, size_t n, uint8_t blue, uint8_t green, uint8_t red, float t1,
                             float t2, uint8_t* __restrict__ bgmask, uint8_t* __restrict__ bw1, uint8_t* __restrict__ bw2,
-                            uint8_t* __restrict__ r2b, uint8_t* __restrict__ rc, int* __restrict__ bg_count) {
+                            uint8_t* __restrict__ r2b, uint8_t* __restrict__ rc, int* __restrict__ bg_count, int vec) {
     int local = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const uint8_t B = bgr[3 * i], G = bgr[3 * i + 1], R = bgr[3 * i + 2];
-        const bool bg = (B > blue) && (G > green) && (R > red);
-        local += bg;
-        if (bgmask) bgmask[i] = bg ? 255 : 0;
-        const float bd = (float)B + FLT_EPSILON;
-        const float gd = (float)G + FLT_EPSILON;
-        const float rd = (float)R;
-        const float q = rd / gd;
-        bw1[i] = (q > t1) ? 255 : 0;
-        bw2[i] = (q > t2) ? 255 : 0;
-        r2b[i] = (rd / bd > 1.0f) ? 255 : 0;
-        rc[i] = (uint8_t)(255 - R);
+    const size_t n4 = vec ? n / 4 : 0;  // word access needs a 4-byte aligned input
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(bgr + 12 * q);
+        const uint32_t w0 = src[0], w1 = src[1], w2 = src[2];
+        uint8_t px[12];
+        for (int k = 0; k < 4; ++k) {
+            px[k] = (uint8_t)(w0 >> (8 * k));
+            px[4 + k] = (uint8_t)(w1 >> (8 * k));
+            px[8 + k] = (uint8_t)(w2 >> (8 * k));
+        }
+        uint32_t bg = 0, b1 = 0, b2 = 0, rb = 0, rcv = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s1_pixel(px[3 * k], px[3 * k + 1], px[3 * k + 2], blue, green, red, t1, t2, bg, b1, b2, rb, rcv, k, local);
+        if (bgmask) reinterpret_cast<uint32_t*>(bgmask)[q] = bg;
+        reinterpret_cast<uint32_t*>(bw1)[q] = b1;
+        reinterpret_cast<uint32_t*>(bw2)[q] = b2;
+        reinterpret_cast<uint32_t*>(r2b)[q] = rb;
+        reinterpret_cast<uint32_t*>(rc)[q] = rcv;
+    }
+    // tail pixels (n not a multiple of 4)
+    for (size_t i = 4 * n4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t bg = 0, b1 = 0, b2 = 0, rb = 0, rcv = 0;
+        s1_pixel(bgr[3 * i], bgr[3 * i + 1], bgr[3 * i + 2], blue, green, red, t1, t2, bg, b1, b2, rb, rcv, 0, local);
+        if (bgmask) bgmask[i] = (uint8_t)bg;
+        bw1[i] = (uint8_t)b1;
+        bw2[i] = (uint8_t)b2;
+        r2b[i] = (uint8_t)rb;
+        rc[i] = (uint8_t)rcv;
     }
     // wave reduction then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
@@ -75,7 +107,24 @@ __global__ void k_s1_pixels(const uint8_t* __restrict__ bgr, size_t n, uint8_t b
 // diffIm = sat(rc - rc_recon) (HE.cpp:211); thresholds at G1 / G2 (:227, :279)
 __global__ void k_diff_thr(const uint8_t* __restrict__ rc, const uint8_t* __restrict__ rec, size_t n, uint8_t g1,
                            uint8_t g2, uint8_t* __restrict__ diff, uint8_t* __restrict__ t1, uint8_t* __restrict__ t2) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    // four pixels per thread (the planes are workspace buffers, 256-byte aligned)
+    const size_t n4 = n / 4;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t a = reinterpret_cast<const uint32_t*>(rc)[q], b = reinterpret_cast<const uint32_t*>(rec)[q];
+        uint32_t dv = 0, o1 = 0, o2 = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int d = (int)((a >> (8 * k)) & 255) - (int)((b >> (8 * k)) & 255);
+            const uint32_t v = d > 0 ? (uint32_t)d : 0u;
+            dv |= v << (8 * k);
+            o1 |= (v > g1 ? 255u : 0u) << (8 * k);
+            o2 |= (v > g2 ? 255u : 0u) << (8 * k);
+        }
+        if (diff) reinterpret_cast<uint32_t*>(diff)[q] = dv;
+        reinterpret_cast<uint32_t*>(t1)[q] = o1;
+        reinterpret_cast<uint32_t*>(t2)[q] = o2;
+    }
+    for (size_t i = 4 * n4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int d = (int)rc[i] - (int)rec[i];
         const uint8_t v = d > 0 ? (uint8_t)d : 0;
         if (diff) diff[i] = v;
@@ -280,7 +329,8 @@ int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p
     // ---- S1: plFindNucleusCandidates ----
     NSC_HIP(hipMemsetAsync(W.counts, 0, 4 * sizeof(int), s));
     hipLaunchKernelGGL(k_s1_pixels, grid1(n), dim3(256), 0, s, bgr, n, p.blue, p.green, p.red, (float)p.T1,
-                       (float)p.T2, (stage && stage[1]) ? W.bgm : nullptr, W.bw1, W.bw2, W.r2b, W.rc, &W.counts[0]);
+                       (float)p.T2, (stage && stage[1]) ? W.bgm : nullptr, W.bw1, W.bw2, W.r2b, W.rc, &W.counts[0],
+                       (reinterpret_cast<uintptr_t>(bgr) & 3) == 0 ? 1 : 0);
     NSC_HIP(hipGetLastError());
     int bgArea = 0;
     NSC_HIP(hipMemcpyAsync(&bgArea, &W.counts[0], sizeof(int), hipMemcpyDeviceToHost, s));

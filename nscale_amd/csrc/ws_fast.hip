@@ -28,6 +28,8 @@
 #include <cstdio>
 #include <cstdlib>
 
+#include <hipcub/hipcub.hpp>
+
 #include "common.hpp"
 
 namespace nsc {
@@ -277,14 +279,18 @@ __global__ void k_phase1(Dev D, const int32_t* __restrict__ box5, int nrun, cons
 }
 
 // ---- phase 1 with the run's neighbourhood staged in LDS ----
-// Same flood as k_phase1, for runs whose bounding box plus a 1-pixel margin has at most LCAP
-// pixels.  The wave stages the box (class, in-run flag, seed labels, push levels, colours) into
+// Same flood as k_phase1, for runs whose bounding box plus a 2-pixel margin has at most LCAP
+// pixels.  The wave stages the frame (class, in-run flag, seed labels, push levels, colours) into
 // LDS and numbers the initial items in raster order with ballots; lane 0 then runs the flood with
-// every read in LDS, writing the per-item records to global memory as it goes.  Leaf slots are
-// reserved per run up front (one per B pixel of the box that could become a leaf), so no atomic
-// sits on the flood's critical path; unused slots keep lf_b = -1.
+// every read in LDS, writing the per-item records to global memory as it goes.  Each queued F
+// pixel carries the key (level, push sequence) and global id of its chain's largest-key item, so a
+// push compares keys without following pointers.  The 2-pixel margin keeps every neighbour of a
+// leaf candidate inside the frame.  Leaf slots are reserved per run up front (one per B pixel of
+// the frame that could become a leaf), so no atomic sits on the flood's critical path; unused
+// slots keep lf_b = -1.
 constexpr uint16_t NIL16 = 0xFFFF;
-enum { K_INRUN = 4, K_ST1 = 8, K_ST2 = 16 };
+enum { K_INRUN = 4, K_ST1 = 8, K_ST2 = 16, K_LV = 32 };
+constexpr int P1_MARGIN = 2;
 
 __device__ __forceinline__ int cdiff_packed(uint32_t a, uint32_t b) {
     const int d0 = abs((int)(a & 255) - (int)(b & 255));
@@ -298,11 +304,10 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                                                    const int* __restrict__ list, const int* __restrict__ count) {
     __shared__ uint8_t code[LCAP];
     __shared__ int32_t lk[LCAP];     // known label: seed label, label of a popped F pixel, else 0
-    __shared__ uint16_t lv0[LCAP];
+    __shared__ uint8_t lv0[LCAP];    // initial push level (valid with K_LV)
     __shared__ uint32_t col[LCAP];
-    __shared__ uint8_t ilv[LCAP];
-    __shared__ uint16_t psq[LCAP];
-    __shared__ uint16_t bnd[LCAP];
+    __shared__ uint32_t bkey[LCAP];  // (level << 16 | push seq) of the chain's largest-key item
+    __shared__ int32_t bglob[LCAP];  // its global item id
     __shared__ uint16_t qn[2 * LCAP];  // FIFO links: F item k, leaf of B pixel k at LCAP + k
     __shared__ uint16_t lfj[LCAP];
     __shared__ uint16_t qhead[NQ], qtail[NQ];
@@ -311,22 +316,40 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     if ((int)blockIdx.x >= *count) return;
     const int r = list[blockIdx.x];
     const int lane = threadIdx.x;
-    const int W = D.W, id = r + 1;
-    const int ox = box5[nrun + r] - 1, oy = box5[3 * nrun + r] - 1;
-    const int lw = box5[2 * nrun + r] - ox + 2, lh = box5[4 * nrun + r] - oy + 2;
-    const int L = lw * lh;  // <= LCAP (bucketed by the host); the box lies inside the frame
-    auto gidx = [&](int k) { return (oy + k / lw) * W + ox + k % lw; };
+    const int W = D.W, H = D.H, id = r + 1;
+    const int ox = box5[nrun + r] - P1_MARGIN, oy = box5[3 * nrun + r] - P1_MARGIN;
+    const int lw = box5[2 * nrun + r] - ox + 1 + P1_MARGIN, lh = box5[4 * nrun + r] - oy + 1 + P1_MARGIN;
+    const int L = lw * lh;  // <= LCAP (bucketed by the host)
+    // k / lw by multiply-high (exact for k < 2^16)
+    const uint32_t magic = (uint32_t)((0x100000000ull + (unsigned)lw - 1) / (unsigned)lw);
+    const int gbase = oy * W + ox;
+    auto gidx = [&](int k) {
+        const int ky = (int)__umulhi((uint32_t)k, magic);
+        return gbase + ky * W + (k - ky * lw);
+    };
 
     for (int k = lane; k < L; k += 64) {
-        const int g = gidx(k);
-        const uint8_t c = D.cls[g];
-        uint8_t cd = c;
-        if (c == C_F && D.run[g] == id) cd |= K_INRUN;
+        const int ky = (int)__umulhi((uint32_t)k, magic), kx = k - ky * lw;
+        const int gx = ox + kx, gy = oy + ky;
+        uint8_t cd = C_FRAME;
+        int32_t l0 = 0;
+        uint8_t lv = 0;
+        uint32_t cc = 0;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {  // the margin may step one pixel off the image
+            const int g = gy * W + gx;
+            const uint8_t c = D.cls[g];
+            cd = c;
+            if (c == C_F && D.run[g] == id) cd |= K_INRUN;
+            if (c == C_SEED) l0 = D.m0[g];
+            const uint16_t v0 = D.lvl0[g];
+            if (v0 != NOPUSH) { cd |= K_LV; lv = (uint8_t)v0; }
+            const uint8_t* q = D.img + 3 * (size_t)g;
+            cc = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16);
+        }
         code[k] = cd;
-        lk[k] = (c == C_SEED) ? D.m0[g] : 0;
-        lv0[k] = D.lvl0[g];
-        const uint8_t* q = D.img + 3 * (size_t)g;
-        col[k] = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16);
+        lk[k] = l0;
+        lv0[k] = lv;
+        col[k] = cc;
     }
     for (int l = lane; l < NQ; l += 64) { qhead[l] = NIL16; qtail[l] = NIL16; }
     __syncthreads();
@@ -337,12 +360,10 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         bool cand = false, init = false;
         if (k < L) {
             const uint8_t cd = code[k];
-            const int kx = k % lw, ky = k / lw;
-            if ((cd & 3) == C_B && lv0[k] == NOPUSH) {
-                cand = (kx > 0 && (code[k - 1] & K_INRUN)) || (kx < lw - 1 && (code[k + 1] & K_INRUN)) ||
-                       (ky > 0 && (code[k - lw] & K_INRUN)) || (ky < lh - 1 && (code[k + lw] & K_INRUN));
-            }
-            init = (cd & K_INRUN) && lv0[k] != NOPUSH;
+            const int ky = (int)__umulhi((uint32_t)k, magic), kx = k - ky * lw;
+            if ((cd & 3) == C_B && !(cd & K_LV) && kx > 0 && kx < lw - 1 && ky > 0 && ky < lh - 1)
+                cand = ((code[k - 1] | code[k + 1] | code[k - lw] | code[k + lw]) & K_INRUN) != 0;
+            init = (cd & K_INRUN) && (cd & K_LV);
         }
         const unsigned long long bc = __ballot(cand), bi = __ballot(init);
         if (init) qn[LCAP + ninit + __popcll(bi & ((1ull << lane) - 1ull))] = (uint16_t)k;
@@ -362,11 +383,10 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     for (int q = lane; q < ninit; q += 64) {
         const int k = qn[LCAP + q];
         const int g = gidx(k);
-        ilv[k] = (uint8_t)lv0[k];
-        psq[k] = (uint16_t)q;
-        bnd[k] = (uint16_t)k;
+        bkey[k] = ((uint32_t)lv0[k] << 16) | (uint32_t)q;
+        bglob[k] = g;
         code[k] |= K_ST1;
-        D.ilvl[g] = (uint8_t)lv0[k];
+        D.ilvl[g] = lv0[k];
         D.pseq[g] = q;
         D.parent[g] = -1;
         D.bnode[g] = g;
@@ -411,6 +431,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         const int gp = gidx(p);
         D.popidx[gp] = popc++;
         int lab = 0;
+#pragma unroll
         for (int d = 0; d < 4; ++d) {
             const int t = lk[p + ld[d]];
             if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
@@ -421,8 +442,9 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         D.st[gp] = 2;
         if (lab == WSHED) continue;
         const uint32_t cp = col[p];
-        const int bp = bnd[p];
-        const int bl = ilv[bp], bs = psq[bp];
+        const uint32_t bk = bkey[p];
+        const int bg = bglob[p];
+#pragma unroll
         for (int d = 0; d < 4; ++d) {
             const int sk = p + ld[d];
             const uint8_t cs = code[sk];
@@ -430,27 +452,24 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 if (cs & (K_ST1 | K_ST2)) continue;
                 const int l = cdiff_packed(cp, col[sk]);
                 const int gs = gp + gd[d];
-                const bool newer = (bl < l) || (bl == l && bs < seq);  // key(bnode[p]) < key(s)
-                const int b = newer ? sk : bp;
-                ilv[sk] = (uint8_t)l;
-                psq[sk] = (uint16_t)seq;
-                bnd[sk] = (uint16_t)b;
+                const uint32_t key = ((uint32_t)l << 16) | (uint32_t)seq;
+                const bool newer = bk < key;  // the new item has the larger key on its chain
+                bkey[sk] = newer ? key : bk;
+                bglob[sk] = newer ? gs : bg;
                 code[sk] = cs | K_ST1;
                 D.ilvl[gs] = (uint8_t)l;
                 D.pseq[gs] = seq;
                 D.parent[gs] = gp;
-                D.bnode[gs] = gidx(b);
+                D.bnode[gs] = newer ? gs : bg;
                 D.st[gs] = 1;
                 ++seq;
                 enq(l, sk);
-            } else if ((cs & 3) == C_B && lv0[sk] == NOPUSH) {
-                const int sx = sk % lw, sy = sk / lw;
+            } else if ((cs & 3) == C_B && !(cs & K_LV)) {
                 bool done = false;  // queued already by an earlier labelled neighbour of this run
+#pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int fx = sx + (e == 0 ? -1 : e == 1 ? 1 : 0), fy = sy + (e == 2 ? -1 : e == 3 ? 1 : 0);
-                    if (fx < 0 || fx >= lw || fy < 0 || fy >= lh) continue;
-                    const int f2 = fy * lw + fx;
-                    if (f2 != p && (code[f2] & K_INRUN) && (code[f2] & K_ST2) && lk[f2] > 0) done = true;
+                    const int f2 = sk + ld[e];
+                    if (f2 != p && (code[f2] & (K_INRUN | K_ST2)) == (K_INRUN | K_ST2) && lk[f2] > 0) done = true;
                 }
                 if (done) continue;
                 if (nlf >= cnt) { atomicOr(D.flags, FB_LEAF_CAP); continue; }
@@ -458,14 +477,14 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 const int slot = base + j;
                 lfj[sk] = (uint16_t)j;
                 const int l = cdiff_packed(cp, col[sk]);
-                const bool newer = (bl < l) || (bl == l && bs < seq);
+                const uint32_t key = ((uint32_t)l << 16) | (uint32_t)seq;
                 D.lf_b[slot] = gp + gd[d];
                 D.lf_run[slot] = id;
                 D.lf_lvl[slot] = (uint8_t)l;
                 D.lf_pseq[slot] = seq;
                 D.lf_parent[slot] = gp;
                 D.lf_popidx[slot] = -1;
-                D.lf_bnode[slot] = newer ? D.N + slot : gidx(bp);
+                D.lf_bnode[slot] = bk < key ? D.N + slot : bg;
                 ++seq;
                 enq(l, LCAP + sk);
             }
@@ -486,11 +505,12 @@ __global__ void k_link_leaves(Dev D) {
 // runs by the size of their LDS frame (box + 1-pixel margin): bucket b holds frames of at most
 // kRunTier[b] pixels; the last bucket floods from global memory
 constexpr int kRunTiers = 5;
-__constant__ int kRunTier[kRunTiers - 1] = {512, 1536, 4096, 7040};
+__constant__ int kRunTier[kRunTiers - 1] = {512, 1536, 4096, 6400};
 __global__ void k_bucket_runs(const int32_t* __restrict__ box5, int nrun, int* __restrict__ lists, int* __restrict__ counts) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrun) return;
-    const long long L = (long long)(box5[2 * nrun + r] - box5[nrun + r] + 3) * (box5[4 * nrun + r] - box5[3 * nrun + r] + 3);
+    const long long L = (long long)(box5[2 * nrun + r] - box5[nrun + r] + 1 + 2 * P1_MARGIN) *
+                        (box5[4 * nrun + r] - box5[3 * nrun + r] + 1 + 2 * P1_MARGIN);
     int b = 0;
     while (b < kRunTiers - 1 && L > kRunTier[b]) ++b;
     lists[(size_t)b * nrun + atomicAdd(&counts[b], 1)] = r;
@@ -665,24 +685,6 @@ __global__ void k_bfs_commit(Dev D, const int* __restrict__ info, const unsigned
         if (kind[p] == 2) { v = IN_QUEUE; D.m[p] = v; }
         umask[p] = (c != C_FRAME && c != C_SEED && (v == 0 || v == IN_QUEUE)) ? 1 : 0;
     }
-}
-
-// exclusive scan of n ints, single block
-__global__ __launch_bounds__(1024) void k_scan_block(const int32_t* __restrict__ in, int32_t* __restrict__ out, int n) {
-    __shared__ int part[1024];
-    const int per = (n + 1023) / 1024;
-    const int start = threadIdx.x * per;
-    int acc = 0;
-    for (int k = 0; k < per && start + k < n; ++k) acc += in[start + k];
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int k = 0; k < 1024; ++k) { const int v = part[k]; part[k] = run; run += v; }
-    }
-    __syncthreads();
-    int run = part[threadIdx.x];
-    for (int k = 0; k < per && start + k < n; ++k) { out[start + k] = run; run += in[start + k]; }
 }
 
 // ---- phase 2: one lane floods one component of unresolved pixels ----
@@ -1251,7 +1253,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         SideStream& ss = side_stream();
         NSC_CHECK(ss.fork(s));
         hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(nrun), dim3(64), 0, ss.side, D, box5, nrun, L(2), bcount + 2);
-        hipLaunchKernelGGL(k_phase1_lds<7040>, dim3(nrun), dim3(64), 0, ss.side, D, box5, nrun, L(3), bcount + 3);
+        hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(nrun), dim3(64), 0, ss.side, D, box5, nrun, L(3), bcount + 3);
         hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, ss.side, D, box5, nrun, L(4), bcount + 4, qh2, qt);
         hipLaunchKernelGGL(k_phase1_lds<1536>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1);
         hipLaunchKernelGGL(k_phase1_lds<512>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(0), bcount);
@@ -1307,7 +1309,13 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         Carver cut3(sc3.p);
         carve3(cut3, b5, area, off, scr, hk);
         NSC_CHECK(bbox_impl(lab2, W, H, ncomp, b5, area, s));
-        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(1024), 0, s, area, off, ncomp);
+        {  // offsets of the components' scratch: exclusive scan of their areas
+            size_t tb = 0;
+            NSC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, area, off, ncomp, s));
+            Scratch st;
+            NSC_CHECK(st.alloc(tb, s));
+            NSC_HIP(hipcub::DeviceScan::ExclusiveSum(st.p, tb, area, off, ncomp, s));
+        }
         P2 Q{lab2, b5, area, off, kind, scr, ncomp};
         tick("p2 setup", &small[16]);
         const int tiecap = 4096;

@@ -964,11 +964,200 @@ __global__ __launch_bounds__(64) void geo_tiles(const uint8_t* __restrict__ pass
     }
 }
 
+// G x G windows per workgroup, shared halos exchanged through LDS after every round (as the
+// reconstruction regions above, in the (min, +1) algebra: halo refreshes take the min, since
+// distances only fall).
+template <int G>
+struct GeoRegion {
+    uint8_t lp[G * G][TH][TW + 4];
+    unsigned tr[G * G][TH][TW + 1];
+    unsigned init_l[G * G][TH], init_r[G * G][TH];
+    unsigned xtop[G * G][TW], xbot[G * G][TW], xleft[G * G][TH], xright[G * G][TH];
+    unsigned chg_bits;
+};
+
+template <int G>
+__global__ __launch_bounds__(64 * G * G) void geo_regions(const uint8_t* __restrict__ pass, unsigned* dist, int w, int h,
+                                                          int tiles_x, int tiles_y, int rgx, int rgy,
+                                                          const int* __restrict__ list, const int* __restrict__ list_count,
+                                                          int* __restrict__ next_list, int* __restrict__ next_count,
+                                                          unsigned* __restrict__ stamp, int pass_no) {
+    __shared__ GeoRegion<G> sh;
+    const int n = pass_no == 0 ? rgx * rgy : *list_count;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int region = pass_no == 0 ? k : list[k];
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int wx = wv % G, wy = wv / G;
+        auto& lp = sh.lp[wv];
+        auto& tr = sh.tr[wv];
+        const int rx = region % rgx, ry = region / rgx;
+        const int tx = rx * G + wx, ty = ry * G + wy;
+        const bool valid = tx < tiles_x && ty < tiles_y;
+        const bool has_l = valid && wx > 0, has_u = valid && wy > 0;
+        const bool has_r = valid && wx < G - 1 && tx + 1 < tiles_x, has_d = valid && wy < G - 1 && ty + 1 < tiles_y;
+        const int gx = tx * OW - 1 + lane;
+        const bool col_in = (gx >= 0 && gx < w);
+        const bool col_frozen = (lane == 0 || lane == TW - 1 || !col_in);
+        if (threadIdx.x == 0) sh.chg_bits = 0;
+        unsigned v[TH];
+        unsigned init_top = 0, init_bot = 0;
+        if (valid) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                const int gy = ty * OH - 1 + y;
+                const bool inb = col_in && gy >= 0 && gy < h;
+                unsigned val = GINF;
+                uint8_t ps = 0;
+                if (inb) {
+                    const size_t i = (size_t)gy * w + gx;
+                    val = dist[i];
+                    ps = pass[i] ? 1 : 0;
+                }
+                const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb || !ps;
+                v[y] = val;
+                lp[y][lane] = frozen ? 0 : 1;
+            }
+            if (lane == 1) {
+#pragma unroll
+                for (int y = 0; y < TH; ++y) { sh.init_l[wv][y] = v[y]; sh.xleft[wv][y] = v[y]; }
+            }
+            if (lane == TW - 2) {
+#pragma unroll
+                for (int y = 0; y < TH; ++y) { sh.init_r[wv][y] = v[y]; sh.xright[wv][y] = v[y]; }
+            }
+            init_top = v[1];
+            init_bot = v[TH - 2];
+            sh.xtop[wv][lane] = v[1];
+            sh.xbot[wv][lane] = v[TH - 2];
+        }
+        __syncthreads();
+        bool owned_change = false;
+        for (int round = 0; round < (1 << 20); ++round) {
+            bool changed = false, swept = false;
+            if (valid) {
+                if (has_u) {
+                    const unsigned nb = sh.xbot[wv - G][lane];
+                    if (nb < v[0]) { v[0] = nb; changed = true; }
+                }
+                if (has_d) {
+                    const unsigned nb = sh.xtop[wv + G][lane];
+                    if (nb < v[TH - 1]) { v[TH - 1] = nb; changed = true; }
+                }
+                unsigned pc = v[0];
+#pragma unroll
+                for (int y = 1; y < TH - 1; ++y) {
+                    const unsigned nv = lp[y][lane] ? min(v[y], inc1(pc)) : v[y];
+                    swept |= (nv != v[y]);
+                    v[y] = nv;
+                    pc = nv;
+                }
+                pc = v[TH - 1];
+#pragma unroll
+                for (int y = TH - 2; y >= 1; --y) {
+                    const unsigned nv = lp[y][lane] ? min(v[y], inc1(pc)) : v[y];
+                    swept |= (nv != v[y]);
+                    v[y] = nv;
+                    pc = nv;
+                }
+#pragma unroll
+                for (int y = 0; y < TH; ++y) tr[y][lane] = v[y];
+            }
+            __syncthreads();
+            if (valid) {
+#pragma unroll
+                for (int x = 0; x < TW; ++x) v[x] = tr[lane][x];
+                if (has_l) {
+                    const unsigned nb = sh.xright[wv - 1][lane];
+                    if (nb < v[0]) { v[0] = nb; changed = true; }
+                }
+                if (has_r) {
+                    const unsigned nb = sh.xleft[wv + 1][lane];
+                    if (nb < v[TW - 1]) { v[TW - 1] = nb; changed = true; }
+                }
+                unsigned prev = v[0];
+#pragma unroll
+                for (int x = 1; x < TW - 1; ++x) {
+                    const unsigned nv = lp[lane][x] ? min(v[x], inc1(prev)) : v[x];
+                    swept |= (nv != v[x]);
+                    v[x] = nv;
+                    prev = nv;
+                }
+                prev = v[TW - 1];
+#pragma unroll
+                for (int x = TW - 2; x >= 1; --x) {
+                    const unsigned nv = lp[lane][x] ? min(v[x], inc1(prev)) : v[x];
+                    swept |= (nv != v[x]);
+                    v[x] = nv;
+                    prev = nv;
+                }
+                sh.xleft[wv][lane] = v[1];
+                sh.xright[wv][lane] = v[TW - 2];
+#pragma unroll
+                for (int x = 0; x < TW; ++x) tr[lane][x] = v[x];
+            }
+            __syncthreads();
+            if (valid) {
+#pragma unroll
+                for (int y = 0; y < TH; ++y) v[y] = tr[y][lane];
+                sh.xtop[wv][lane] = v[1];
+                sh.xbot[wv][lane] = v[TH - 2];
+            }
+            owned_change |= swept;
+            if (!__syncthreads_or((changed || swept) ? 1 : 0)) break;
+        }
+        if (valid && __any(owned_change) && !col_frozen) {
+#pragma unroll
+            for (int y = 1; y < TH - 1; ++y) {
+                const int gy = ty * OH - 1 + y;
+                if (gy < h) dist[(size_t)gy * w + gx] = v[y];
+            }
+        }
+        unsigned bits = 0;
+        if (valid) {
+            const bool edge_l = wx == 0, edge_r = wx == G - 1 || tx == tiles_x - 1;
+            const bool edge_u = wy == 0, edge_d = wy == G - 1 || ty == tiles_y - 1;
+            if (lane >= 1 && lane <= TW - 2) {
+                if (edge_u && v[1] != init_top) bits |= CH_TOP;
+                if (edge_d && v[TH - 2] != init_bot) bits |= CH_BOT;
+            }
+            if (lane == 1 && edge_l) {
+                for (int y = 1; y < TH - 1; ++y)
+                    if (v[y] != sh.init_l[wv][y]) bits |= CH_LEFT;
+            }
+            if (lane == TW - 2 && edge_r) {
+                for (int y = 1; y < TH - 1; ++y)
+                    if (v[y] != sh.init_r[wv][y]) bits |= CH_RIGHT;
+            }
+        }
+        if (bits) atomicOr(&sh.chg_bits, bits);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned b = sh.chg_bits;
+            const unsigned np = (unsigned)pass_no + 1;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if ((dx == 0) == (dy == 0)) continue;  // 4-connected: edge neighbours only
+                    bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) ||
+                                (dx == -1 && (b & CH_LEFT)) || (dx == 1 && (b & CH_RIGHT));
+                    const int nx = rx + dx, ny = ry + dy;
+                    if (need && nx >= 0 && nx < rgx && ny >= 0 && ny < rgy)
+                        mark_tile(ny * rgx + nx, np, stamp, next_list, next_count);
+                }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t s) {
+    static int G = [] {
+        const char* e = getenv("NSC_GEO_REGION");
+        return e ? std::max(1, std::min(atoi(e), 2)) : 2;
+    }();
     const int tiles_x = (int)cdiv(w, OW), tiles_y = (int)cdiv(h, OH);
-    const int nt = tiles_x * tiles_y;
+    const int rgx = (int)cdiv(tiles_x, G), rgy = (int)cdiv(tiles_y, G);
+    const int nt = rgx * rgy;
     const int max_pass = 4 * nt + 8;
     Scratch sc;
     NSC_CHECK(sc.alloc(sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2), s));
@@ -978,15 +1167,22 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
     int* counts = listB + nt;
     NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt, s));
     NSC_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (max_pass + 2), s));
-    hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, pass, dist, w, h, tiles_x, tiles_y, nullptr, nullptr, listA,
-                       &counts[0], stamp, 0);
+    const unsigned grid = (unsigned)std::min(nt, 1 << 16);
+    auto launch = [&](int p, const int* lst, const int* cnt, int* nl, int* nc) {
+        if (G == 2)
+            hipLaunchKernelGGL(geo_regions<2>, dim3(grid), dim3(256), 0, s, pass, dist, w, h, tiles_x, tiles_y, rgx, rgy, lst,
+                               cnt, nl, nc, stamp, p);
+        else
+            hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, pass, dist, w, h, tiles_x, tiles_y, lst, cnt, nl, nc, stamp,
+                               p);
+    };
+    launch(0, nullptr, nullptr, listA, &counts[0]);
     int p = 1;
     for (;;) {
         for (int b = 0; b < 4 && p <= max_pass; ++b, ++p) {
             int* cur = (p & 1) ? listA : listB;
             int* nxt = (p & 1) ? listB : listA;
-            hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, pass, dist, w, h, tiles_x, tiles_y, cur,
-                               &counts[p - 1], nxt, &counts[p], stamp, p);
+            launch(p, cur, &counts[p - 1], nxt, &counts[p]);
         }
         NSC_HIP(hipGetLastError());
         int c = 0;

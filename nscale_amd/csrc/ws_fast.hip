@@ -494,11 +494,46 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
 }
 
 // every used leaf slot joins the list of its B pixel (list order is irrelevant to its readers)
-__global__ void k_link_leaves(Dev D) {
+// wave-aggregated append of the lanes whose `take` is set
+__device__ __forceinline__ void wave_append(bool take, int v, int* __restrict__ list, int* __restrict__ count) {
+    const unsigned long long m = __ballot(take);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader);
+    if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+// every used leaf slot joins the list of its B pixel (list order is irrelevant to its readers);
+// the first leaf of a pixel also puts the pixel on the list of B pixels with queue entries
+__global__ void k_link_leaves(Dev D, int* __restrict__ blist, int* __restrict__ bcount) {
     const int n = min(*D.nleaf, D.leafcap);
-    for (int sl = blockIdx.x * blockDim.x + threadIdx.x; sl < n; sl += gridDim.x * blockDim.x) {
-        const int b = D.lf_b[sl];
-        if (b >= 0 && b < D.N) D.lf_next[sl] = atomicExch(&D.head[b], sl);
+    const int stride = gridDim.x * blockDim.x;
+    const int n_up = ((n + 63) / 64) * 64;  // whole waves take part in the append
+    for (int sl = blockIdx.x * blockDim.x + threadIdx.x; sl < n_up; sl += stride) {
+        bool first = false;
+        int b = -1;
+        if (sl < n) {
+            b = D.lf_b[sl];
+            if (b >= 0 && b < D.N) {
+                const int old = atomicExch(&D.head[b], sl);
+                D.lf_next[sl] = old;
+                first = old < 0;
+            }
+        }
+        wave_append(first, b, blist, bcount);
+    }
+}
+
+// B pixels queued in the initial scan (next to a seed)
+__global__ void k_collect_b0(Dev D, int* __restrict__ blist, int* __restrict__ bcount) {
+    const int stride = gridDim.x * blockDim.x;
+    const int n_up = ((D.N + 63) / 64) * 64;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_up; i += stride) {
+        const bool take = i < D.N && D.cls[i] == C_B && D.lvl0[i] != NOPUSH;
+        wave_append(take, i, blist, bcount);
     }
 }
 
@@ -517,11 +552,14 @@ __global__ void k_bucket_runs(const int32_t* __restrict__ box5, int nrun, int* _
 }
 
 // ---- B items: the real queue entry of each B pixel and its label decision at pop time ----
-__global__ void k_bresolve(Dev D, int32_t* __restrict__ real, int* __restrict__ cand, int* __restrict__ ncand) {
+// over the B pixels that have a queue entry (listed by k_collect_b0 / k_link_leaves); real[] of
+// every other pixel stays -1
+__global__ void k_bresolve(Dev D, int32_t* __restrict__ real, int* __restrict__ cand, int* __restrict__ ncand,
+                           const int* __restrict__ blist, const int* __restrict__ bcount) {
     const int W = D.W;
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < D.N; b += gridDim.x * blockDim.x) {
-        real[b] = -1;
-        if (D.cls[b] != C_B) continue;
+    const int nb = *bcount;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x) {
+        const int b = blist[k];
         int r = -1;
         if (D.lvl0[b] != NOPUSH) r = b;
         else {
@@ -1199,7 +1237,8 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     Carver cut(sc.p);
     carve(cut, cls, lvl0, fmask, run, st, ilvl, pseq, parent, bnode, popidx, lab, head, real, lab2, kind, small);
     // small[]: 0 nF, 1 nrun, 2 nleaf, 3 flags, 4 ncand, 8..10 info, 16 ncomp, 20 deferred tie components,
-    // 24..25 tie pool bump pointer (long long), 26..30 run bucket sizes
+    // 24..25 tie pool bump pointer (long long), 26..30 run bucket sizes, 32..34 LDS flood queue counts,
+    // 35 B pixels with queue entries
     NSC_HIP(hipMemsetAsync(small, 0, 64 * sizeof(int), s));
 
     Dev D{};
@@ -1220,7 +1259,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     Carver m2(nullptr);
     auto carve2 = [&](Carver& c, int32_t*& box5, int*& qh, int*& qt, int32_t*& qnext, int32_t*& lf) {
         box5 = c.take<int32_t>(5 * (size_t)std::max(nrun, 1));
-        qh = c.take<int>((size_t)std::max(nrun, 1) * (NQ + kRunTiers));  // + run buckets
+        qh = c.take<int>((size_t)std::max(nrun, 1) * (NQ + kRunTiers));  // run buckets, then FIFO heads
         qt = c.take<int>((size_t)std::max(nrun, 1) * NQ);
         qnext = c.take<int32_t>(N + (size_t)leafcap);
         lf = c.take<int32_t>((size_t)leafcap * 8);
@@ -1258,12 +1297,17 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         hipLaunchKernelGGL(k_phase1_lds<1536>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1);
         hipLaunchKernelGGL(k_phase1_lds<512>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(0), bcount);
         NSC_CHECK(ss.join(s));
-        hipLaunchKernelGGL(k_link_leaves, grid1(N), dim3(256), 0, s, D);
     }
+    // B pixels with a queue entry: initially queued ones, then those with leaves (the phase-1
+    // queue links are dead from here on and hold the list)
+    int* blist = qnext;  // N + leafcap ints
+    hipLaunchKernelGGL(k_collect_b0, grid1(N), dim3(256), 0, s, D, blist, &small[35]);
+    if (nrun > 0) hipLaunchKernelGGL(k_link_leaves, grid1(N), dim3(256), 0, s, D, blist, &small[35]);
     tick("phase1", &small[2]);
     // B resolution and b1
     int* cand = lab2;  // reuse (N ints)
-    hipLaunchKernelGGL(k_bresolve, grid1(N), dim3(256), 0, s, D, real, cand, &small[4]);
+    NSC_HIP(hipMemsetAsync(real, 0xFF, sizeof(int32_t) * N, s));  // -1 everywhere
+    hipLaunchKernelGGL(k_bresolve, grid1(N), dim3(256), 0, s, D, real, cand, &small[4], blist, &small[35]);
     tick("bresolve", &small[4]);
     {
         int* partial = small + 64;

@@ -146,23 +146,28 @@ inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 // ---- fork/join onto a side stream ---------------------------------------------------------
 // Independent launches (e.g. long-latency work on a few runs beside many short ones) can overlap
-// on a per-thread side stream (default priority); join() makes the caller's stream wait for it.
+// on per-thread side streams (default priority); join() makes the caller's stream wait for them.
 struct SideStream {
-    hipStream_t side = nullptr;
-    hipEvent_t forked = nullptr, joined = nullptr;
+    static constexpr int K = 2;
+    hipStream_t side[K] = {nullptr, nullptr};
+    hipEvent_t forked = nullptr, joined[K] = {nullptr, nullptr};
     int fork(hipStream_t s) {
-        if (!side) {
-            NSC_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        if (!forked) {
             NSC_HIP(hipEventCreateWithFlags(&forked, hipEventDisableTiming));
-            NSC_HIP(hipEventCreateWithFlags(&joined, hipEventDisableTiming));
+            for (int k = 0; k < K; ++k) {
+                NSC_HIP(hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking));
+                NSC_HIP(hipEventCreateWithFlags(&joined[k], hipEventDisableTiming));
+            }
         }
         NSC_HIP(hipEventRecord(forked, s));
-        NSC_HIP(hipStreamWaitEvent(side, forked, 0));
+        for (int k = 0; k < K; ++k) NSC_HIP(hipStreamWaitEvent(side[k], forked, 0));
         return NSC_OK;
     }
     int join(hipStream_t s) {
-        NSC_HIP(hipEventRecord(joined, side));
-        NSC_HIP(hipStreamWaitEvent(s, joined, 0));
+        for (int k = 0; k < K; ++k) {
+            NSC_HIP(hipEventRecord(joined[k], side[k]));
+            NSC_HIP(hipStreamWaitEvent(s, joined[k], 0));
+        }
         return NSC_OK;
     }
 };

@@ -413,13 +413,14 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     const int gd[4] = {-1, 1, -W, W};
     for (;;) {
         if (active >= NQ) break;
-        if (qhead[active] == NIL16) {
+        int x = qhead[active];
+        if (x == NIL16) {
             int l = active + 1;
             while (l < NQ && qhead[l] == NIL16) ++l;
             if (l == NQ) break;
             active = l;
+            x = qhead[l];
         }
-        const int x = qhead[active];
         const uint16_t nx = qn[x];
         qhead[active] = nx;
         if (nx == NIL16) qtail[active] = NIL16;
@@ -428,29 +429,42 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             continue;
         }
         const int p = x;
+        // every LDS value this pop reads up front, so the reads overlap
+        int nl[4];
+        uint8_t nc[4];
+        uint32_t ncol[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            nl[d] = lk[p + ld[d]];
+            nc[d] = code[p + ld[d]];
+            ncol[d] = col[p + ld[d]];
+        }
+        const uint8_t cpc = code[p];
+        const uint32_t cp = col[p];
+        const uint32_t bk = bkey[p];
+        const int bg = bglob[p];
         const int gp = gidx(p);
         D.popidx[gp] = popc++;
         int lab = 0;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-            const int t = lk[p + ld[d]];
+            const int t = nl[d];
             if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
         }
         lk[p] = lab;
-        code[p] = (uint8_t)((code[p] & ~K_ST1) | K_ST2);
+        code[p] = (uint8_t)((cpc & ~K_ST1) | K_ST2);
         D.lab[gp] = lab;
         D.st[gp] = 2;
         if (lab == WSHED) continue;
-        const uint32_t cp = col[p];
-        const uint32_t bk = bkey[p];
-        const int bg = bglob[p];
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             const int sk = p + ld[d];
-            const uint8_t cs = code[sk];
+            // a neighbour's state may have changed since the batch read only through this pop's own
+            // earlier pushes, which go to other pixels
+            const uint8_t cs = nc[d];
             if ((cs & 3) == C_F) {
                 if (cs & (K_ST1 | K_ST2)) continue;
-                const int l = cdiff_packed(cp, col[sk]);
+                const int l = cdiff_packed(cp, ncol[d]);
                 const int gs = gp + gd[d];
                 const uint32_t key = ((uint32_t)l << 16) | (uint32_t)seq;
                 const bool newer = bk < key;  // the new item has the larger key on its chain
@@ -465,18 +479,23 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 ++seq;
                 enq(l, sk);
             } else if ((cs & 3) == C_B && !(cs & K_LV)) {
-                bool done = false;  // queued already by an earlier labelled neighbour of this run
+                uint8_t fc[4];
+                int fl[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int f2 = sk + ld[e];
-                    if (f2 != p && (code[f2] & (K_INRUN | K_ST2)) == (K_INRUN | K_ST2) && lk[f2] > 0) done = true;
+                    fc[e] = code[sk + ld[e]];
+                    fl[e] = lk[sk + ld[e]];
                 }
+                bool done = false;  // queued already by an earlier labelled neighbour of this run
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (sk + ld[e] != p && (fc[e] & (K_INRUN | K_ST2)) == (K_INRUN | K_ST2) && fl[e] > 0) done = true;
                 if (done) continue;
                 if (nlf >= cnt) { atomicOr(D.flags, FB_LEAF_CAP); continue; }
                 const int j = nlf++;
                 const int slot = base + j;
                 lfj[sk] = (uint16_t)j;
-                const int l = cdiff_packed(cp, col[sk]);
+                const int l = cdiff_packed(cp, ncol[d]);
                 const uint32_t key = ((uint32_t)l << 16) | (uint32_t)seq;
                 D.lf_b[slot] = gp + gd[d];
                 D.lf_run[slot] = id;
@@ -493,7 +512,6 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     }
 }
 
-// every used leaf slot joins the list of its B pixel (list order is irrelevant to its readers)
 // wave-aggregated append of the lanes whose `take` is set
 __device__ __forceinline__ void wave_append(bool take, int v, int* __restrict__ list, int* __restrict__ count) {
     const unsigned long long m = __ballot(take);
@@ -539,8 +557,8 @@ __global__ void k_collect_b0(Dev D, int* __restrict__ blist, int* __restrict__ b
 
 // runs by the size of their LDS frame (box + 1-pixel margin): bucket b holds frames of at most
 // kRunTier[b] pixels; the last bucket floods from global memory
-constexpr int kRunTiers = 5;
-__constant__ int kRunTier[kRunTiers - 1] = {512, 1536, 4096, 6400};
+constexpr int kRunTiers = 6;
+__constant__ int kRunTier[kRunTiers - 1] = {512, 1024, 2048, 4096, 6400};
 __global__ void k_bucket_runs(const int32_t* __restrict__ box5, int nrun, int* __restrict__ lists, int* __restrict__ counts) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrun) return;
@@ -1237,7 +1255,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     Carver cut(sc.p);
     carve(cut, cls, lvl0, fmask, run, st, ilvl, pseq, parent, bnode, popidx, lab, head, real, lab2, kind, small);
     // small[]: 0 nF, 1 nrun, 2 nleaf, 3 flags, 4 ncand, 8..10 info, 16 ncomp, 20 deferred tie components,
-    // 24..25 tie pool bump pointer (long long), 26..30 run bucket sizes, 32..34 LDS flood queue counts,
+    // 24..25 tie pool bump pointer (long long), 26..31 run bucket sizes, 32..34 LDS flood queue counts,
     // 35 B pixels with queue entries
     NSC_HIP(hipMemsetAsync(small, 0, 64 * sizeof(int), s));
 
@@ -1287,14 +1305,21 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         int* bcount = &small[26];                  // kRunTiers counters
         int* qh2 = qh + (size_t)kRunTiers * nrun;  // FIFO heads for the global-flood bucket
         hipLaunchKernelGGL(k_bucket_runs, dim3(cdiv(nrun, 256)), dim3(256), 0, s, box5, nrun, lists, bcount);
+        if (dbg) {
+            int bc[kRunTiers];
+            (void)hipMemcpy(bc, bcount, sizeof(bc), hipMemcpyDeviceToHost);
+            fprintf(stderr, "[ws] run tiers %d %d %d %d %d %d\n", bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
+        }
         auto L = [&](int b) { return lists + (size_t)b * nrun; };
-        // the few large runs set the latency: they run on the side stream beside the many small ones
+        // the few large runs set the latency: the largest tiers start first, each on a side stream,
+        // beside the many small runs on the caller's stream
         SideStream& ss = side_stream();
         NSC_CHECK(ss.fork(s));
-        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(nrun), dim3(64), 0, ss.side, D, box5, nrun, L(2), bcount + 2);
-        hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(nrun), dim3(64), 0, ss.side, D, box5, nrun, L(3), bcount + 3);
-        hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, ss.side, D, box5, nrun, L(4), bcount + 4, qh2, qt);
-        hipLaunchKernelGGL(k_phase1_lds<1536>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1);
+        hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(nrun), dim3(64), 0, ss.side[0], D, box5, nrun, L(4), bcount + 4);
+        hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(5), bcount + 5, qh2, qt);
+        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(nrun), dim3(64), 0, ss.side[1], D, box5, nrun, L(3), bcount + 3);
+        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(nrun), dim3(64), 0, ss.side[1], D, box5, nrun, L(2), bcount + 2);
+        hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1);
         hipLaunchKernelGGL(k_phase1_lds<512>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(0), bcount);
         NSC_CHECK(ss.join(s));
     }

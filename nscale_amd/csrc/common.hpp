@@ -187,6 +187,26 @@ template <typename R>
 __device__ __forceinline__ R smin(R a, R b) { return (b < a) ? b : a; }
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+// The same pointer in the global address space: loads and stores through it compile to global_*
+// instructions instead of flat ones (flat accesses also count against the LDS wait counter).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// x where keep is all ones, 0 (all bits clear) where keep is 0.  Written as a bit operation on
+// purpose: a select between a loaded value and a constant may be compiled into a branch around
+// the load, which serialises the loads of an unrolled window.
+template <typename R>
+__device__ __forceinline__ R keep_bits(R x, uint32_t keep) {
+    static_assert(sizeof(R) == 4, "four-byte values");
+    return __builtin_bit_cast(R, __builtin_bit_cast(uint32_t, x) & keep);
+}
+
 
 // value from lane-1 (lane 0 receives `old`) and from lane+1 (lane 63 receives `old`):
 // full-wave DPP shifts (wave_shr:1 / wave_shl:1, GFX9 DPP controls 0x138 / 0x130)

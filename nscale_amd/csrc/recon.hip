@@ -26,12 +26,18 @@
 #include <atomic>
 #include <cstdlib>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
 
 namespace nsc {
 namespace {
+
+// minimum waves per SIMD the window kernels are compiled for (bounds their registers)
+#ifndef RECON_WAVES
+#define RECON_WAVES 2
+#endif
 
 constexpr int TW = 64;  // window width (= wave size)
 constexpr int TH = 64;  // window height
@@ -70,13 +76,178 @@ struct RShared {
     unsigned chg_bits;
 };
 
+// ---- per-step arithmetic of the window sweeps ------------------------------------------------
+// Values only rise (out starts at min(marker, mask) <= mask), so min(max(v, n), mk) = med3(v, n, mk)
+// for integer types.  Frozen pixels have mk = v, hence med3(v, n, v) = v whatever n holds; that is
+// why the neighbour maxima may read zero-filled DPP lanes (lanes 0 and 63 are always frozen).
+template <typename R>
+__device__ __forceinline__ R clamp_up(R v, R n, R mk) { return smin(smax(v, n), mk); }
+template <>
+__device__ __forceinline__ uint32_t clamp_up(uint32_t v, uint32_t n, uint32_t mk) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(n), "v"(mk));
+    return r;
+}
+template <>
+__device__ __forceinline__ int32_t clamp_up(int32_t v, int32_t n, int32_t mk) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(n), "v"(mk));
+    return r;
+}
+// max of a value and its left and right lane neighbours (8-connectivity); lanes whose DPP source
+// is outside the wave read 0 (only frozen lanes, see above)
+template <typename R>
+__device__ __forceinline__ R nbr3(R pc) {
+    const R l = __builtin_bit_cast(R, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, pc), 0x138, 0xF, 0xF, true));
+    const R r = __builtin_bit_cast(R, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, pc), 0x130, 0xF, 0xF, true));
+    return smax(pc, smax(l, r));
+}
+template <>
+__device__ __forceinline__ uint32_t nbr3(uint32_t pc) {
+    const uint32_t l = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pc, 0x138, 0xF, 0xF, true);
+    const uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pc, 0x130, 0xF, 0xF, true);
+    return max(max(l, pc), r);
+}
+template <>
+__device__ __forceinline__ int32_t nbr3(int32_t pc) {
+    const int32_t l = __builtin_amdgcn_update_dpp(0, pc, 0x138, 0xF, 0xF, true);
+    const int32_t r = __builtin_amdgcn_update_dpp(0, pc, 0x130, 0xF, 0xF, true);
+    return max(max(l, pc), r);
+}
+
+// Mask of a window in both orientations.  uint8: packed four to a register (column state: rows
+// 4j..4j+3 of the lane's column; row state: columns 4j..4j+3 of the lane's row), so the sweeps
+// never touch LDS for it.  Four-byte types read it from the LDS copy, prefetching at most eight
+// rows ahead (a scheduling fence every eight steps keeps the register count at four waves/SIMD).
+template <typename T>
+struct WinMask {
+    using R = typename RegOf<T>::type;
+    const T (*lm)[TW + 4];
+    int lane;
+    __device__ void build() {}
+    __device__ void opaque() {}
+    __device__ __forceinline__ R clamp_col(R v, R n, int y) const { return clamp_up(v, n, (R)lm[y][lane]); }
+    __device__ __forceinline__ R clamp_row(R v, R n, int x) const { return clamp_up(v, n, (R)lm[lane][x]); }
+    static __device__ __forceinline__ void fence(int k) {
+        if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+};
+template <>
+struct WinMask<uint8_t> {
+    const uint8_t (*lm)[TW + 4];
+    int lane;
+    uint32_t mc[TH / 4], mr[TW / 4];
+    // mc is filled while the window loads; mr comes from the LDS copy (row `lane`, 4-byte words)
+    __device__ __forceinline__ void build() {
+#pragma unroll
+        for (int j = 0; j < TW / 4; ++j) mr[j] = *reinterpret_cast<const uint32_t*>(&lm[lane][4 * j]);
+    }
+    // called at the top of every round: hides the packed words from loop-invariant code motion,
+    // which would otherwise unpack all 124 masks once before the loop and spill them
+    __device__ __forceinline__ void opaque() {
+#pragma unroll
+        for (int j = 0; j < TH / 4; ++j) asm volatile("" : "+v"(mc[j]), "+v"(mr[j]));
+    }
+    // min(max(v, n), mask byte): the byte extraction folds into the min as an SDWA operand select
+    __device__ __forceinline__ uint32_t clamp_col(uint32_t v, uint32_t n, int y) const {
+        return min(max(v, n), (mc[y >> 2] >> (8 * (y & 3))) & 0xFFu);
+    }
+    __device__ __forceinline__ uint32_t clamp_row(uint32_t v, uint32_t n, int x) const {
+        return min(max(v, n), (mr[x >> 2] >> (8 * (x & 3))) & 0xFFu);
+    }
+    static __device__ __forceinline__ void fence(int) {}
+};
+
+// one round of the four direction sweeps over a window held in v (column state on entry and exit);
+// returns whether any value changed (uint8: decided by the caller from column sums)
+template <typename T, int CONN>
+__device__ __forceinline__ void sweep_round(typename RegOf<T>::type (&v)[TH], WinMask<T>& M, T (*tr)[TW + 1],
+                                            int lane) {
+    using R = typename RegOf<T>::type;
+    M.opaque();
+    {
+        R pc = v[0];
+#pragma unroll
+        for (int y = 1; y < TH - 1; ++y) {
+            const R up = (CONN == 8) ? nbr3(pc) : pc;
+            const R nv = M.clamp_col(v[y], up, y);
+            v[y] = nv;
+            pc = nv;
+            M.fence(y);
+        }
+    }
+    {
+        M.opaque();  // no sharing of unpacked masks between the two directions (register pressure)
+        R pc = v[TH - 1];
+#pragma unroll
+        for (int y = TH - 2; y >= 1; --y) {
+            const R dn = (CONN == 8) ? nbr3(pc) : pc;
+            const R nv = M.clamp_col(v[y], dn, y);
+            v[y] = nv;
+            pc = nv;
+            M.fence(y);
+        }
+    }
+    // ---- transpose to row state (lane = row) ----
+#pragma unroll
+    for (int y = 0; y < TH; ++y) tr[y][lane] = (T)v[y];
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < TW; ++x) v[x] = (R)tr[lane][x];
+    M.opaque();
+    {
+        R prev = v[0];
+#pragma unroll
+        for (int x = 1; x < TW - 1; ++x) {
+            const R nv = M.clamp_row(v[x], prev, x);
+            v[x] = nv;
+            prev = nv;
+            M.fence(x);
+        }
+        M.opaque();
+        prev = v[TW - 1];
+#pragma unroll
+        for (int x = TW - 2; x >= 1; --x) {
+            const R nv = M.clamp_row(v[x], prev, x);
+            v[x] = nv;
+            prev = nv;
+            M.fence(x);
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < TW; ++x) tr[lane][x] = (T)v[x];
+    __syncthreads();
+#pragma unroll
+    for (int y = 0; y < TH; ++y) v[y] = (R)tr[y][lane];
+    __syncthreads();
+}
+
+// Change detection: values only rise, so a column changed in a round iff the sum of an
+// order-preserving unsigned key of its values rose.  The sweeps carry no per-step compares (the
+// compiler would keep all 248 compare results in scalar registers and OR them as a tree).
+__device__ __forceinline__ uint32_t order_key(uint32_t v) { return v; }
+__device__ __forceinline__ uint32_t order_key(int32_t v) { return (uint32_t)v ^ 0x80000000u; }
+__device__ __forceinline__ uint32_t order_key(float v) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, v);
+    return b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
+}
+template <typename R>
+__device__ __forceinline__ auto col_sum(const R (&v)[TH]) {
+    // uint8 values: 64 x 255 fits 32 bits; four-byte keys need 64
+    using S = typename std::conditional<std::is_same<R, uint32_t>::value, uint32_t, uint64_t>::type;
+    S s = 0;
+#pragma unroll
+    for (int y = 0; y < TH; ++y) s += (S)order_key(v[y]);
+    return s;
+}
+
 // one tile of one image: load window, sweep to stability, write owned pixels, mark neighbours
 template <typename T, int CONN>
 __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass, RShared<T>& sh, int* __restrict__ next_list,
                                            int* __restrict__ next_count, unsigned* __restrict__ stamp) {
     using R = typename RegOf<T>::type;
+    constexpr bool U8 = std::is_same<T, uint8_t>::value;
     auto& lm = sh.lm;
-    auto& tr = sh.tr;
     auto& init_l = sh.init_l;
     auto& init_r = sh.init_r;
     auto& chg_bits = sh.chg_bits;
@@ -91,21 +262,33 @@ __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass
     const bool col_frozen = (lane == 0 || lane == TW - 1 || !col_in);
     if (lane == 0) chg_bits = 0;
 
+    WinMask<T> M;
+    M.lm = lm;
+    M.lane = lane;
     R v[TH];
+    // every lane loads from a clamped (always valid) address so that the 128 loads of the window
+    // issue back to back; pixels outside the image are replaced by the frame value afterwards
+    const auto gmask = as_global(mask);
+    const auto gsrc = as_global(pass == 0 ? marker : (const T*)out);
+    const size_t gxc = (size_t)clampi(gx, 0, w - 1);
 #pragma unroll
     for (int y = 0; y < TH; ++y) {
         const int gy = ty * OH - 1 + y;
         const bool inb = col_in && gy >= 0 && gy < h;
-        R val = 0, mk = 0;
-        if (inb) {
-            const size_t i = (size_t)gy * w + gx;
-            mk = (R)mask[i];
-            const R a = (pass == 0) ? (R)marker[i] : (R)out[i];
-            val = smin(a, mk);
-        }
+        const size_t i = (size_t)clampi(gy, 0, h - 1) * w + gxc;
+        const R m_ld = (R)gmask[i], s_ld = (R)gsrc[i];  // unconditional: the address is valid
+        const uint32_t keep = inb ? ~0u : 0u;
+        const R mk = keep_bits(m_ld, keep);
+        const R val = keep_bits(smin(s_ld, m_ld), keep);
         const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb;
         v[y] = val;
-        lm[y][lane] = (T)(frozen ? val : mk);
+        const R mw = frozen ? val : mk;
+        lm[y][lane] = (T)mw;
+        if constexpr (U8) {
+            if ((y & 3) == 0) M.mc[y >> 2] = 0;
+            M.mc[y >> 2] |= (uint32_t)mw << (8 * (y & 3));
+        }
+        if ((y & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // 32 loads in flight at most
     }
     if (lane == 1) {
 #pragma unroll
@@ -117,77 +300,18 @@ __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass
     }
     const R init_top = v[1], init_bot = v[TH - 2];
     __syncthreads();
+    M.build();
 
-    bool any_change = false;
+    const auto s0 = col_sum(v);
+    auto sv = s0;
     for (int round = 0; round < 4096; ++round) {
-        bool changed = false;
-        // ---- column state: vertical sweeps ----
-        {
-            R pc = v[0];
-            R pl = from_left(pc, pc), pr = from_right(pc, pc);
-#pragma unroll
-            for (int y = 1; y < TH - 1; ++y) {
-                R up = pc;
-                if (CONN == 8) up = smax(up, smax(pl, pr));
-                const R mk = (R)lm[y][lane];
-                const R nv = smin(smax(v[y], up), mk);
-                changed |= (nv != v[y]);
-                v[y] = nv;
-                pc = nv;
-                if (CONN == 8) { pl = from_left(pc, pc); pr = from_right(pc, pc); }
-            }
-        }
-        {
-            R pc = v[TH - 1];
-            R pl = from_left(pc, pc), pr = from_right(pc, pc);
-#pragma unroll
-            for (int y = TH - 2; y >= 1; --y) {
-                R dn = pc;
-                if (CONN == 8) dn = smax(dn, smax(pl, pr));
-                const R mk = (R)lm[y][lane];
-                const R nv = smin(smax(v[y], dn), mk);
-                changed |= (nv != v[y]);
-                v[y] = nv;
-                pc = nv;
-                if (CONN == 8) { pl = from_left(pc, pc); pr = from_right(pc, pc); }
-            }
-        }
-        // ---- transpose to row state (lane = row) ----
-#pragma unroll
-        for (int y = 0; y < TH; ++y) tr[y][lane] = (T)v[y];
-        __syncthreads();
-#pragma unroll
-        for (int x = 0; x < TW; ++x) v[x] = (R)tr[lane][x];
-        {
-            R prev = v[0];
-#pragma unroll
-            for (int x = 1; x < TW - 1; ++x) {
-                const R mk = (R)lm[lane][x];
-                const R nv = smin(smax(v[x], prev), mk);
-                changed |= (nv != v[x]);
-                v[x] = nv;
-                prev = nv;
-            }
-            prev = v[TW - 1];
-#pragma unroll
-            for (int x = TW - 2; x >= 1; --x) {
-                const R mk = (R)lm[lane][x];
-                const R nv = smin(smax(v[x], prev), mk);
-                changed |= (nv != v[x]);
-                v[x] = nv;
-                prev = nv;
-            }
-        }
-#pragma unroll
-        for (int x = 0; x < TW; ++x) tr[lane][x] = (T)v[x];
-        __syncthreads();
-#pragma unroll
-        for (int y = 0; y < TH; ++y) v[y] = (R)tr[y][lane];
-        __syncthreads();
-        const bool wave_changed = __any(changed);
-        any_change |= wave_changed;
+        sweep_round<T, CONN>(v, M, sh.tr, lane);
+        const auto s1 = col_sum(v);
+        const bool wave_changed = __any(s1 != sv);
+        sv = s1;
         if (!wave_changed) break;
     }
+    const bool any_change = __any(sv != s0);
 
     // pass 0 writes every owned pixel (the output starts as min(marker, mask)); later passes
     // only rewrite tiles that changed
@@ -195,10 +319,14 @@ __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass
 
     // ---- write the owned pixels ----
     if (!col_frozen) {
+        // the column pointer is made opaque so that its per-row addresses are formed here and not
+        // hoisted above the sweep loop, where they would hold 128 registers
+        auto gcol = as_global(out) + gx;
+        asm volatile("" : "+v"(gcol));
 #pragma unroll
         for (int y = 1; y < TH - 1; ++y) {
             const int gy = ty * OH - 1 + y;
-            if (gy < h) out[(size_t)gy * w + gx] = (T)v[y];
+            if (gy < h) gcol[(size_t)gy * w] = (T)v[y];
         }
     }
     // ---- owned-border changes -> mark neighbours ----
@@ -238,7 +366,7 @@ __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass
 // stride over their work; a batch's images are found through tile_img (global tile -> image).
 // CONN 0 = per-image connectivity.
 template <typename T, int CONN>
-__global__ __launch_bounds__(64) void recon_tiles(const RImg<T>* __restrict__ imgs, RImg<T> one, const int* __restrict__ tile_img,
+__global__ __launch_bounds__(64, RECON_WAVES) void recon_tiles(const RImg<T>* __restrict__ imgs, RImg<T> one, const int* __restrict__ tile_img,
                                                   int ntiles, const int* __restrict__ list, const int* __restrict__ list_count,
                                                   int* __restrict__ next_list, int* __restrict__ next_count,
                                                   unsigned* __restrict__ stamp, int pass) {
@@ -822,14 +950,124 @@ namespace {
 
 constexpr unsigned GINF = 0xFFFFFFFFu;
 
-__device__ __forceinline__ unsigned inc1(unsigned v) { return v == GINF ? GINF : v + 1u; }
+// Frozen pixels of a window (window halo, outside the image, not passable) as bits, in both
+// orientations: column state fc (bit y & 31 of fc[y >> 5] = pixel (y, lane)), row state fr (bit
+// x & 31 of fr[x >> 5] = pixel (lane, x)).  A step is min(v, sat(prev + 1) | blk) with blk = all
+// ones on frozen pixels (value kept) and 0 on free ones.
+struct GeoFlags {
+    uint32_t fc[2], fr[2];
+    // called at the top of every round: keeps the 124 unpacked flags out of loop-invariant code
+    // motion (they would be computed once before the loop and spilled)
+    __device__ __forceinline__ void opaque() { asm volatile("" : "+v"(fc[0]), "+v"(fc[1]), "+v"(fr[0]), "+v"(fr[1])); }
+    __device__ __forceinline__ unsigned blk_col(int y) const { return 0u - ((fc[y >> 5] >> (y & 31)) & 1u); }
+    __device__ __forceinline__ unsigned blk_row(int x) const { return 0u - ((fr[x >> 5] >> (x & 31)) & 1u); }
+};
 
-__global__ __launch_bounds__(64) void geo_tiles(const uint8_t* __restrict__ pass, unsigned* dist, int w, int h, int tiles_x,
-                                                int tiles_y, const int* __restrict__ list, const int* __restrict__ list_count,
-                                                int* __restrict__ next_list, int* __restrict__ next_count,
-                                                unsigned* __restrict__ stamp, int pass_no) {
-    __shared__ uint8_t lp[TH][TW + 4];       // 1 = free pixel of the window, 0 = frozen
+__device__ __forceinline__ unsigned geo_step(unsigned v, unsigned prev, unsigned blk) {
+    return min(v, __builtin_elementwise_add_sat(prev, 1u) | blk);
+}
+
+// Load a window's distances into v (column state) and its frozen flags; lanes read clamped,
+// always valid addresses so the loads issue back to back.
+__device__ __forceinline__ void geo_load(const uint8_t* pass, const unsigned* dist, int w, int h, int tx, int ty, int lane,
+                                         unsigned (&v)[TH], GeoFlags& F, uint8_t (*lp)[TW + 4]) {
+    const int gx = tx * OW - 1 + lane;
+    const bool col_in = (gx >= 0 && gx < w);
+    const bool col_frozen = (lane == 0 || lane == TW - 1 || !col_in);
+    const auto gdist = as_global(dist);
+    const auto gpass = as_global(pass);
+    const size_t gxc = (size_t)clampi(gx, 0, w - 1);
+    F.fc[0] = F.fc[1] = F.fr[0] = F.fr[1] = 0;
+#pragma unroll
+    for (int y = 0; y < TH; ++y) {
+        const int gy = ty * OH - 1 + y;
+        const bool inb = col_in && gy >= 0 && gy < h;
+        const size_t i = (size_t)clampi(gy, 0, h - 1) * w + gxc;
+        const unsigned d_ld = gdist[i];
+        const unsigned p_ld = gpass[i];
+        v[y] = d_ld | (inb ? 0u : GINF);
+        const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb || p_ld == 0;
+        F.fc[y >> 5] |= (frozen ? 1u : 0u) << (y & 31);
+        lp[y][lane] = frozen ? 1 : 0;
+        if ((y & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // at most 32 loads in flight
+    }
+    __syncthreads();
+    // row state: the lane's row of flag bytes, four at a time; (w * 0x10204080) >> 28 gathers the
+    // low bits of the four bytes (partial products never overlap, so nothing carries into them)
+#pragma unroll
+    for (int j = 0; j < TW / 4; ++j) {
+        const uint32_t wd = *reinterpret_cast<const uint32_t*>(&lp[lane][4 * j]);
+        F.fr[j >> 3] |= ((wd * 0x10204080u) >> 28) << (4 * (j & 7));
+    }
+}
+
+// The sweeps do not track changes step by step (a compare per step, and the compiler keeps the
+// compare results in scalar registers until it ORs them as a tree); distances only fall, so a
+// column changed iff the 64-bit sum of ~v over it rose (geo_key_sum).
+__device__ __forceinline__ void geo_col_sweeps(unsigned (&v)[TH], GeoFlags& F) {
+    F.opaque();
+    unsigned pc = v[0];
+#pragma unroll
+    for (int y = 1; y < TH - 1; ++y) {
+        v[y] = geo_step(v[y], pc, F.blk_col(y));
+        pc = v[y];
+    }
+    F.opaque();  // no sharing of unpacked flags between the two directions (register pressure)
+    pc = v[TH - 1];
+#pragma unroll
+    for (int y = TH - 2; y >= 1; --y) {
+        v[y] = geo_step(v[y], pc, F.blk_col(y));
+        pc = v[y];
+    }
+}
+
+__device__ __forceinline__ void geo_row_sweeps(unsigned (&v)[TW], GeoFlags& F) {
+    F.opaque();
+    unsigned prev = v[0];
+#pragma unroll
+    for (int x = 1; x < TW - 1; ++x) {
+        v[x] = geo_step(v[x], prev, F.blk_row(x));
+        prev = v[x];
+    }
+    F.opaque();
+    prev = v[TW - 1];
+#pragma unroll
+    for (int x = TW - 2; x >= 1; --x) {
+        v[x] = geo_step(v[x], prev, F.blk_row(x));
+        prev = v[x];
+    }
+}
+
+// sum of ~v over rows [y0, y1) of the lane's column (monotone in every value as distances fall)
+__device__ __forceinline__ uint64_t geo_key_sum(const unsigned (&v)[TH], int y0, int y1) {
+    uint64_t s = 0;
+#pragma unroll
+    for (int y = 0; y < TH; ++y)
+        if (y >= y0 && y < y1) s += (uint64_t)(~v[y]);
+    return s;
+}
+
+// owned pixels of a window back to dist (the column pointer is opaque so that the per-row
+// addresses are formed here, not hoisted above the sweep loop)
+__device__ __forceinline__ void geo_store(unsigned* dist, int w, int h, int tx, int ty, int lane, const unsigned (&v)[TH]) {
+    const int gx = tx * OW - 1 + lane;
+    if (lane == 0 || lane == TW - 1 || gx >= w) return;
+    auto gcol = as_global(dist) + gx;
+    asm volatile("" : "+v"(gcol));
+#pragma unroll
+    for (int y = 1; y < TH - 1; ++y) {
+        const int gy = ty * OH - 1 + y;
+        if (gy < h) gcol[(size_t)gy * w] = v[y];
+    }
+}
+
+__global__ __launch_bounds__(64, 2) void geo_tiles(const uint8_t* __restrict__ pass, unsigned* dist, int w, int h,
+                                                             int tiles_x, int tiles_y, const int* __restrict__ list,
+                                                             const int* __restrict__ list_count, int* __restrict__ next_list,
+                                                             int* __restrict__ next_count, unsigned* __restrict__ stamp,
+                                                             int pass_no) {
     __shared__ unsigned tr[TH][TW + 1];
+    __shared__ uint8_t lp[TH][TW + 4];  // frozen flags, row-major (staging for the row-state bits)
     __shared__ unsigned init_l[TH], init_r[TH];
     __shared__ unsigned chg_bits;
     int tile;
@@ -840,26 +1078,10 @@ __global__ __launch_bounds__(64) void geo_tiles(const uint8_t* __restrict__ pass
     }
     const int tx = tile % tiles_x, ty = tile / tiles_x;
     const int lane = threadIdx.x;
-    const int gx = tx * OW - 1 + lane;
-    const bool col_in = (gx >= 0 && gx < w);
-    const bool col_frozen = (lane == 0 || lane == TW - 1 || !col_in);
     if (lane == 0) chg_bits = 0;
     unsigned v[TH];
-#pragma unroll
-    for (int y = 0; y < TH; ++y) {
-        const int gy = ty * OH - 1 + y;
-        const bool inb = col_in && gy >= 0 && gy < h;
-        unsigned val = GINF;
-        uint8_t ps = 0;
-        if (inb) {
-            const size_t i = (size_t)gy * w + gx;
-            val = dist[i];
-            ps = pass[i] ? 1 : 0;
-        }
-        const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb || !ps;
-        v[y] = val;
-        lp[y][lane] = frozen ? 0 : 1;
-    }
+    GeoFlags F;
+    geo_load(pass, dist, w, h, tx, ty, lane, v, F, lp);
     if (lane == 1) {
 #pragma unroll
         for (int y = 0; y < TH; ++y) init_l[y] = v[y];
@@ -870,68 +1092,29 @@ __global__ __launch_bounds__(64) void geo_tiles(const uint8_t* __restrict__ pass
     }
     const unsigned init_top = v[1], init_bot = v[TH - 2];
     __syncthreads();
-    bool any_change = false;
+    const uint64_t s0 = geo_key_sum(v, 0, TH);
+    uint64_t s = s0;
     for (int round = 0; round < 1 << 20; ++round) {
-        bool changed = false;
-        {
-            unsigned pc = v[0];
-#pragma unroll
-            for (int y = 1; y < TH - 1; ++y) {
-                const unsigned nv = lp[y][lane] ? min(v[y], inc1(pc)) : v[y];
-                changed |= (nv != v[y]);
-                v[y] = nv;
-                pc = nv;
-            }
-            pc = v[TH - 1];
-#pragma unroll
-            for (int y = TH - 2; y >= 1; --y) {
-                const unsigned nv = lp[y][lane] ? min(v[y], inc1(pc)) : v[y];
-                changed |= (nv != v[y]);
-                v[y] = nv;
-                pc = nv;
-            }
-        }
+        geo_col_sweeps(v, F);
 #pragma unroll
         for (int y = 0; y < TH; ++y) tr[y][lane] = v[y];
         __syncthreads();
 #pragma unroll
         for (int x = 0; x < TW; ++x) v[x] = tr[lane][x];
-        {
-            unsigned prev = v[0];
-#pragma unroll
-            for (int x = 1; x < TW - 1; ++x) {
-                const unsigned nv = lp[lane][x] ? min(v[x], inc1(prev)) : v[x];
-                changed |= (nv != v[x]);
-                v[x] = nv;
-                prev = nv;
-            }
-            prev = v[TW - 1];
-#pragma unroll
-            for (int x = TW - 2; x >= 1; --x) {
-                const unsigned nv = lp[lane][x] ? min(v[x], inc1(prev)) : v[x];
-                changed |= (nv != v[x]);
-                v[x] = nv;
-                prev = nv;
-            }
-        }
+        geo_row_sweeps(v, F);
 #pragma unroll
         for (int x = 0; x < TW; ++x) tr[lane][x] = v[x];
         __syncthreads();
 #pragma unroll
         for (int y = 0; y < TH; ++y) v[y] = tr[y][lane];
         __syncthreads();
-        const bool wave_changed = __any(changed);
-        any_change |= wave_changed;
+        const uint64_t s1 = geo_key_sum(v, 0, TH);
+        const bool wave_changed = __any(s1 != s);
+        s = s1;
         if (!wave_changed) break;
     }
-    if (!any_change) return;
-    if (!col_frozen) {
-#pragma unroll
-        for (int y = 1; y < TH - 1; ++y) {
-            const int gy = ty * OH - 1 + y;
-            if (gy < h) dist[(size_t)gy * w + gx] = v[y];
-        }
-    }
+    if (!__any(s != s0)) return;
+    geo_store(dist, w, h, tx, ty, lane, v);
     unsigned bits = 0;
     if (lane >= 1 && lane <= TW - 2) {
         if (v[1] != init_top) bits |= CH_TOP;
@@ -969,8 +1152,8 @@ __global__ __launch_bounds__(64) void geo_tiles(const uint8_t* __restrict__ pass
 // distances only fall).
 template <int G>
 struct GeoRegion {
-    uint8_t lp[G * G][TH][TW + 4];
     unsigned tr[G * G][TH][TW + 1];
+    uint8_t lp[G * G][TH][TW + 4];
     unsigned init_l[G * G][TH], init_r[G * G][TH];
     unsigned xtop[G * G][TW], xbot[G * G][TW], xleft[G * G][TH], xright[G * G][TH];
     unsigned chg_bits;
@@ -988,35 +1171,18 @@ __global__ __launch_bounds__(64 * G * G) void geo_regions(const uint8_t* __restr
         const int region = pass_no == 0 ? k : list[k];
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int wx = wv % G, wy = wv / G;
-        auto& lp = sh.lp[wv];
         auto& tr = sh.tr[wv];
         const int rx = region % rgx, ry = region / rgx;
         const int tx = rx * G + wx, ty = ry * G + wy;
         const bool valid = tx < tiles_x && ty < tiles_y;
         const bool has_l = valid && wx > 0, has_u = valid && wy > 0;
         const bool has_r = valid && wx < G - 1 && tx + 1 < tiles_x, has_d = valid && wy < G - 1 && ty + 1 < tiles_y;
-        const int gx = tx * OW - 1 + lane;
-        const bool col_in = (gx >= 0 && gx < w);
-        const bool col_frozen = (lane == 0 || lane == TW - 1 || !col_in);
         if (threadIdx.x == 0) sh.chg_bits = 0;
         unsigned v[TH];
+        GeoFlags F;
         unsigned init_top = 0, init_bot = 0;
         if (valid) {
-#pragma unroll
-            for (int y = 0; y < TH; ++y) {
-                const int gy = ty * OH - 1 + y;
-                const bool inb = col_in && gy >= 0 && gy < h;
-                unsigned val = GINF;
-                uint8_t ps = 0;
-                if (inb) {
-                    const size_t i = (size_t)gy * w + gx;
-                    val = dist[i];
-                    ps = pass[i] ? 1 : 0;
-                }
-                const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb || !ps;
-                v[y] = val;
-                lp[y][lane] = frozen ? 0 : 1;
-            }
+            geo_load(pass, dist, w, h, tx, ty, lane, v, F, sh.lp[wv]);
             if (lane == 1) {
 #pragma unroll
                 for (int y = 0; y < TH; ++y) { sh.init_l[wv][y] = v[y]; sh.xleft[wv][y] = v[y]; }
@@ -1031,34 +1197,22 @@ __global__ __launch_bounds__(64 * G * G) void geo_regions(const uint8_t* __restr
             sh.xbot[wv][lane] = v[TH - 2];
         }
         __syncthreads();
-        bool owned_change = false;
+        // owned pixels: rows 1..62 of lanes 1..62; the halo rows and lanes change only through the
+        // exchanges below, which the full-column sums also see
+        const uint64_t own0 = geo_key_sum(v, 1, TH - 1);
+        uint64_t full = own0 + (uint64_t)(~v[0]) + (uint64_t)(~v[TH - 1]), own = own0;
         for (int round = 0; round < (1 << 20); ++round) {
-            bool changed = false, swept = false;
+            bool changed = false;
             if (valid) {
                 if (has_u) {
                     const unsigned nb = sh.xbot[wv - G][lane];
-                    if (nb < v[0]) { v[0] = nb; changed = true; }
+                    if (nb < v[0]) v[0] = nb;
                 }
                 if (has_d) {
                     const unsigned nb = sh.xtop[wv + G][lane];
-                    if (nb < v[TH - 1]) { v[TH - 1] = nb; changed = true; }
+                    if (nb < v[TH - 1]) v[TH - 1] = nb;
                 }
-                unsigned pc = v[0];
-#pragma unroll
-                for (int y = 1; y < TH - 1; ++y) {
-                    const unsigned nv = lp[y][lane] ? min(v[y], inc1(pc)) : v[y];
-                    swept |= (nv != v[y]);
-                    v[y] = nv;
-                    pc = nv;
-                }
-                pc = v[TH - 1];
-#pragma unroll
-                for (int y = TH - 2; y >= 1; --y) {
-                    const unsigned nv = lp[y][lane] ? min(v[y], inc1(pc)) : v[y];
-                    swept |= (nv != v[y]);
-                    v[y] = nv;
-                    pc = nv;
-                }
+                geo_col_sweeps(v, F);
 #pragma unroll
                 for (int y = 0; y < TH; ++y) tr[y][lane] = v[y];
             }
@@ -1068,28 +1222,13 @@ __global__ __launch_bounds__(64 * G * G) void geo_regions(const uint8_t* __restr
                 for (int x = 0; x < TW; ++x) v[x] = tr[lane][x];
                 if (has_l) {
                     const unsigned nb = sh.xright[wv - 1][lane];
-                    if (nb < v[0]) { v[0] = nb; changed = true; }
+                    if (nb < v[0]) v[0] = nb;
                 }
                 if (has_r) {
                     const unsigned nb = sh.xleft[wv + 1][lane];
-                    if (nb < v[TW - 1]) { v[TW - 1] = nb; changed = true; }
+                    if (nb < v[TW - 1]) v[TW - 1] = nb;
                 }
-                unsigned prev = v[0];
-#pragma unroll
-                for (int x = 1; x < TW - 1; ++x) {
-                    const unsigned nv = lp[lane][x] ? min(v[x], inc1(prev)) : v[x];
-                    swept |= (nv != v[x]);
-                    v[x] = nv;
-                    prev = nv;
-                }
-                prev = v[TW - 1];
-#pragma unroll
-                for (int x = TW - 2; x >= 1; --x) {
-                    const unsigned nv = lp[lane][x] ? min(v[x], inc1(prev)) : v[x];
-                    swept |= (nv != v[x]);
-                    v[x] = nv;
-                    prev = nv;
-                }
+                geo_row_sweeps(v, F);
                 sh.xleft[wv][lane] = v[1];
                 sh.xright[wv][lane] = v[TW - 2];
 #pragma unroll
@@ -1101,17 +1240,15 @@ __global__ __launch_bounds__(64 * G * G) void geo_regions(const uint8_t* __restr
                 for (int y = 0; y < TH; ++y) v[y] = tr[y][lane];
                 sh.xtop[wv][lane] = v[1];
                 sh.xbot[wv][lane] = v[TH - 2];
+                own = geo_key_sum(v, 1, TH - 1);
+                const uint64_t f1 = own + (uint64_t)(~v[0]) + (uint64_t)(~v[TH - 1]);
+                changed = f1 != full;
+                full = f1;
             }
-            owned_change |= swept;
-            if (!__syncthreads_or((changed || swept) ? 1 : 0)) break;
+            if (!__syncthreads_or(changed ? 1 : 0)) break;
         }
-        if (valid && __any(owned_change) && !col_frozen) {
-#pragma unroll
-            for (int y = 1; y < TH - 1; ++y) {
-                const int gy = ty * OH - 1 + y;
-                if (gy < h) dist[(size_t)gy * w + gx] = v[y];
-            }
-        }
+        const bool lane_owned = lane >= 1 && lane <= TW - 2;
+        if (valid && __any(lane_owned && own != own0)) geo_store(dist, w, h, tx, ty, lane, v);
         unsigned bits = 0;
         if (valid) {
             const bool edge_l = wx == 0, edge_r = wx == G - 1 || tx == tiles_x - 1;

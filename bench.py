@@ -27,6 +27,12 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The served step runs 8 server slots, each with its own stream and two side streams; HIP maps a
+# process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default), where kernels of
+# unrelated streams wait behind each other.  8 queues measured 7% more served throughput than 4.
+# Must be set before the HIP runtime starts (before torch touches the GPU).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 

@@ -57,6 +57,23 @@ __device__ __forceinline__ void mark_tile(int t, unsigned next_pass, unsigned* s
     }
 }
 
+// Threads 0..7 of the calling workgroup mark one neighbour each (all 8, or the 4 edge neighbours
+// when edges_only) whose halo the changed borders in b feed; the returning atomics of the
+// different neighbours overlap instead of running one after another.
+__device__ __forceinline__ void mark_neighbours(int t, unsigned b, int cx, int cy, int nx_max, int ny_max, int base,
+                                                bool edges_only, unsigned next_pass, unsigned* stamp, int* next_list,
+                                                int* next_count) {
+    if (t >= 8) return;
+    const int k = t < 4 ? t : t + 1;  // 3 x 3 neighbourhood without its centre
+    const int dx = k % 3 - 1, dy = k / 3 - 1;
+    if (edges_only && dx != 0 && dy != 0) return;
+    const bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) || (dx == -1 && (b & CH_LEFT)) ||
+                      (dx == 1 && (b & CH_RIGHT));
+    const int nx = cx + dx, ny = cy + dy;
+    if (need && nx >= 0 && nx < nx_max && ny >= 0 && ny < ny_max)
+        mark_tile(base + ny * nx_max + nx, next_pass, stamp, next_list, next_count);
+}
+
 // one image of a reconstruction launch: its planes, size, connectivity and first global tile id
 template <typename T>
 struct RImg {
@@ -271,24 +288,35 @@ __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass
     const auto gmask = as_global(mask);
     const auto gsrc = as_global(pass == 0 ? marker : (const T*)out);
     const size_t gxc = (size_t)clampi(gx, 0, w - 1);
+    // 16 rows at a time: all 32 loads of a chunk are issued before any of them is used (the
+    // scheduling fences stop the compiler from sinking loads to their uses)
 #pragma unroll
-    for (int y = 0; y < TH; ++y) {
-        const int gy = ty * OH - 1 + y;
-        const bool inb = col_in && gy >= 0 && gy < h;
-        const size_t i = (size_t)clampi(gy, 0, h - 1) * w + gxc;
-        const R m_ld = (R)gmask[i], s_ld = (R)gsrc[i];  // unconditional: the address is valid
-        const uint32_t keep = inb ? ~0u : 0u;
-        const R mk = keep_bits(m_ld, keep);
-        const R val = keep_bits(smin(s_ld, m_ld), keep);
-        const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb;
-        v[y] = val;
-        const R mw = frozen ? val : mk;
-        lm[y][lane] = (T)mw;
-        if constexpr (U8) {
-            if ((y & 3) == 0) M.mc[y >> 2] = 0;
-            M.mc[y >> 2] |= (uint32_t)mw << (8 * (y & 3));
+    for (int c = 0; c < TH; c += 16) {
+        R m_ld[16], s_ld[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const size_t i = (size_t)clampi(ty * OH - 1 + c + j, 0, h - 1) * w + gxc;
+            m_ld[j] = (R)gmask[i];  // unconditional: the address is valid
+            s_ld[j] = (R)gsrc[i];
         }
-        if ((y & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // 32 loads in flight at most
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int y = c + j, gy = ty * OH - 1 + y;
+            const bool inb = col_in && gy >= 0 && gy < h;
+            const uint32_t keep = inb ? ~0u : 0u;
+            const R mk = keep_bits(m_ld[j], keep);
+            const R val = keep_bits(smin(s_ld[j], m_ld[j]), keep);
+            const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb;
+            v[y] = val;
+            const R mw = frozen ? val : mk;
+            lm[y][lane] = (T)mw;
+            if constexpr (U8) {
+                if ((y & 3) == 0) M.mc[y >> 2] = 0;
+                M.mc[y >> 2] |= (uint32_t)mw << (8 * (y & 3));
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
     if (lane == 1) {
 #pragma unroll
@@ -347,19 +375,8 @@ __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass
     }
     if (bits) atomicOr(&chg_bits, bits);
     __syncthreads();
-    if (lane == 0) {
-        const unsigned b = chg_bits;
-        const unsigned np = (unsigned)pass + 1;
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                if (dx == 0 && dy == 0) continue;
-                bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) ||
-                            (dx == -1 && (b & CH_LEFT)) || (dx == 1 && (b & CH_RIGHT));
-                const int nx = tx + dx, ny = ty + dy;
-                if (need && nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y)
-                    mark_tile(im.base + ny * tiles_x + nx, np, stamp, next_list, next_count);
-            }
-    }
+    mark_neighbours(lane, chg_bits, tx, ty, tiles_x, tiles_y, im.base, false, (unsigned)pass + 1, stamp, next_list,
+                    next_count);
 }
 
 // Pass 0 runs every tile of every image; pass k > 0 runs the tiles listed by pass k - 1.  Blocks
@@ -579,19 +596,8 @@ __device__ __forceinline__ void recon_region(const RImg<T>& im, int region, int 
     }
     if (bits) atomicOr(&sh.chg_bits, bits);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned b = sh.chg_bits;
-        const unsigned np = (unsigned)pass + 1;
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                if (dx == 0 && dy == 0) continue;
-                bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) ||
-                            (dx == -1 && (b & CH_LEFT)) || (dx == 1 && (b & CH_RIGHT));
-                const int nx = rx + dx, ny = ry + dy;
-                if (need && nx >= 0 && nx < im.rgx && ny >= 0 && ny < im.rgy)
-                    mark_tile(im.base + ny * im.rgx + nx, np, stamp, next_list, next_count);
-            }
-    }
+    mark_neighbours(threadIdx.x, sh.chg_bits, rx, ry, im.rgx, im.rgy, im.base, false, (unsigned)pass + 1, stamp,
+                    next_list, next_count);
 }
 
 template <typename T, int CONN, int G>
@@ -949,6 +955,9 @@ namespace nsc {
 namespace {
 
 constexpr unsigned GINF = 0xFFFFFFFFu;
+#ifndef GEO_CHUNK
+#define GEO_CHUNK 16
+#endif
 
 // Frozen pixels of a window (window halo, outside the image, not passable) as bits, in both
 // orientations: column state fc (bit y & 31 of fc[y >> 5] = pixel (y, lane)), row state fr (bit
@@ -978,18 +987,29 @@ __device__ __forceinline__ void geo_load(const uint8_t* pass, const unsigned* di
     const auto gpass = as_global(pass);
     const size_t gxc = (size_t)clampi(gx, 0, w - 1);
     F.fc[0] = F.fc[1] = F.fr[0] = F.fr[1] = 0;
+    // 16 rows at a time: all 32 loads of a chunk are issued before any of them is used (the
+    // scheduling fences stop the compiler from sinking loads to their uses, which would wait on
+    // every row in turn)
 #pragma unroll
-    for (int y = 0; y < TH; ++y) {
-        const int gy = ty * OH - 1 + y;
-        const bool inb = col_in && gy >= 0 && gy < h;
-        const size_t i = (size_t)clampi(gy, 0, h - 1) * w + gxc;
-        const unsigned d_ld = gdist[i];
-        const unsigned p_ld = gpass[i];
-        v[y] = d_ld | (inb ? 0u : GINF);
-        const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb || p_ld == 0;
-        F.fc[y >> 5] |= (frozen ? 1u : 0u) << (y & 31);
-        lp[y][lane] = frozen ? 1 : 0;
-        if ((y & 15) == 15) __builtin_amdgcn_sched_barrier(0);  // at most 32 loads in flight
+    for (int c = 0; c < TH; c += GEO_CHUNK) {
+        unsigned d_ld[GEO_CHUNK], p_ld[GEO_CHUNK];
+#pragma unroll
+        for (int j = 0; j < GEO_CHUNK; ++j) {
+            const size_t i = (size_t)clampi(ty * OH - 1 + c + j, 0, h - 1) * w + gxc;
+            d_ld[j] = gdist[i];
+            p_ld[j] = gpass[i];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < GEO_CHUNK; ++j) {
+            const int y = c + j, gy = ty * OH - 1 + y;
+            const bool inb = col_in && gy >= 0 && gy < h;
+            v[y] = d_ld[j] | (inb ? 0u : GINF);
+            const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb || p_ld[j] == 0;
+            F.fc[y >> 5] |= (frozen ? 1u : 0u) << (y & 31);
+            lp[y][lane] = frozen ? 1 : 0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
     // row state: the lane's row of flag bytes, four at a time; (w * 0x10204080) >> 28 gathers the
@@ -1132,19 +1152,9 @@ __global__ __launch_bounds__(64, 2) void geo_tiles(const uint8_t* __restrict__ p
     }
     if (bits) atomicOr(&chg_bits, bits);
     __syncthreads();
-    if (lane == 0) {
-        const unsigned b = chg_bits;
-        const unsigned np = (unsigned)pass_no + 1;
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                if ((dx == 0) == (dy == 0)) continue;  // 4-connected: edge neighbours only
-                bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) || (dx == -1 && (b & CH_LEFT)) ||
-                            (dx == 1 && (b & CH_RIGHT));
-                const int nx = tx + dx, ny = ty + dy;
-                if (need && nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y)
-                    mark_tile(ny * tiles_x + nx, np, stamp, next_list, next_count);
-            }
-    }
+    // 4-connected: edge neighbours only
+    mark_neighbours(lane, chg_bits, tx, ty, tiles_x, tiles_y, 0, true, (unsigned)pass_no + 1, stamp, next_list,
+                    next_count);
 }
 
 // G x G windows per workgroup, shared halos exchanged through LDS after every round (as the
@@ -1268,19 +1278,8 @@ __global__ __launch_bounds__(64 * G * G) void geo_regions(const uint8_t* __restr
         }
         if (bits) atomicOr(&sh.chg_bits, bits);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const unsigned b = sh.chg_bits;
-            const unsigned np = (unsigned)pass_no + 1;
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    if ((dx == 0) == (dy == 0)) continue;  // 4-connected: edge neighbours only
-                    bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) ||
-                                (dx == -1 && (b & CH_LEFT)) || (dx == 1 && (b & CH_RIGHT));
-                    const int nx = rx + dx, ny = ry + dy;
-                    if (need && nx >= 0 && nx < rgx && ny >= 0 && ny < rgy)
-                        mark_tile(ny * rgx + nx, np, stamp, next_list, next_count);
-                }
-        }
+        mark_neighbours(threadIdx.x, sh.chg_bits, rx, ry, rgx, rgy, 0, true, (unsigned)pass_no + 1, stamp, next_list,
+                        next_count);  // 4-connected: edge neighbours only
         __syncthreads();
     }
 }
@@ -1316,7 +1315,9 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
     launch(0, nullptr, nullptr, listA, &counts[0]);
     int p = 1;
     for (;;) {
-        for (int b = 0; b < 4 && p <= max_pass; ++b, ++p) {
+        // eight passes per host check: the front needs tens of passes to cross a tile, and an
+        // empty pass costs a few microseconds against a host round trip of tens
+        for (int b = 0; b < 8 && p <= max_pass; ++b, ++p) {
             int* cur = (p & 1) ? listA : listB;
             int* nxt = (p & 1) ? listB : listA;
             launch(p, cur, &counts[p - 1], nxt, &counts[p]);

@@ -313,8 +313,13 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     __shared__ uint16_t qhead[NQ], qtail[NQ];
     __shared__ int sh_base;
 
-    if ((int)blockIdx.x >= *count) return;
-    const int r = list[blockIdx.x];
+    // a fixed grid strides over the tier's runs: launching one workgroup per run of the whole
+    // image (most of them idle for this tier) would hold the large LDS allocation of every idle
+    // workgroup across the device
+    const int nr = *count;
+    for (int it = blockIdx.x; it < nr; it += gridDim.x) {
+    [&]() {  // one run; `return` ends the run
+    const int r = list[it];
     const int lane = threadIdx.x;
     const int W = D.W, H = D.H, id = r + 1;
     const int ox = box5[nrun + r] - P1_MARGIN, oy = box5[3 * nrun + r] - P1_MARGIN;
@@ -509,6 +514,9 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             }
         }
         if (seq >= 65535) { atomicOr(D.flags, FB_LEAF_CAP); return; }
+    }
+    }();
+    __syncthreads();  // the LDS arrays are reused by the next run
     }
 }
 
@@ -1315,12 +1323,12 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         // beside the many small runs on the caller's stream
         SideStream& ss = side_stream();
         NSC_CHECK(ss.fork(s));
-        hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(nrun), dim3(64), 0, ss.side[0], D, box5, nrun, L(4), bcount + 4);
+        hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(std::min(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(4), bcount + 4);
         hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(5), bcount + 5, qh2, qt);
-        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(nrun), dim3(64), 0, ss.side[1], D, box5, nrun, L(3), bcount + 3);
-        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(nrun), dim3(64), 0, ss.side[1], D, box5, nrun, L(2), bcount + 2);
-        hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1);
-        hipLaunchKernelGGL(k_phase1_lds<512>, dim3(nrun), dim3(64), 0, s, D, box5, nrun, L(0), bcount);
+        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(std::min(nrun, 256)), dim3(64), 0, ss.side[1], D, box5, nrun, L(3), bcount + 3);
+        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(std::min(nrun, 768)), dim3(64), 0, ss.side[1], D, box5, nrun, L(2), bcount + 2);
+        hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(std::min(nrun, 1536)), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1);
+        hipLaunchKernelGGL(k_phase1_lds<512>, dim3(std::min(nrun, 3072)), dim3(64), 0, s, D, box5, nrun, L(0), bcount);
         NSC_CHECK(ss.join(s));
     }
     // B pixels with a queue entry: initially queued ones, then those with leaves (the phase-1
@@ -1410,8 +1418,8 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         hipLaunchKernelGGL(k_phase2, dim3(cdiv(ncomp, 64)), dim3(64), 0, s, D, Q, real, hk, ties, &small[20], tiecap,
                            pool_used, poolcap, F);
         tick("phase2", &small[20]);
-        hipLaunchKernelGGL(k_flood_lds<FL_BIG>, dim3(1024), dim3(64), 0, s, D, Q, real, F, 1);
-        hipLaunchKernelGGL(k_flood_lds<FL_SMALL>, dim3(4096), dim3(64), 0, s, D, Q, real, F, 0);
+        hipLaunchKernelGGL(k_flood_lds<FL_BIG>, dim3(256), dim3(64), 0, s, D, Q, real, F, 1);
+        hipLaunchKernelGGL(k_flood_lds<FL_SMALL>, dim3(1024), dim3(64), 0, s, D, Q, real, F, 0);
         hipLaunchKernelGGL(k_tie_commit, dim3(1024), dim3(256), 0, s, D, Q, F);
         tick("lds flood", &small[32]);
         hipLaunchKernelGGL(k_phase2_ties, dim3(tiecap), dim3(64), 0, s, D, Q, real, ties, &small[20], pool);

@@ -219,29 +219,35 @@ int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipS
 // ------------------------------------------------------------------------------------------
 // operators on top of the labels
 // ------------------------------------------------------------------------------------------
-// one atomic per run of equal labels within a thread's 16 consecutive pixels (the background
-// component would otherwise take one atomic per pixel on a single address)
+// Component areas with one atomic per run of equal labels in each wave's 64 consecutive pixels
+// (coalesced loads; the background component of a fill would otherwise take one atomic per pixel
+// on a single address).  A lane starts a run when its label differs from the previous lane's;
+// the run ends where the next one starts.
 __global__ void k_area(const int32_t* __restrict__ L, int32_t* __restrict__ area, size_t n) {
-    const size_t chunks = (n + 15) / 16;
-    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks; c += (size_t)gridDim.x * blockDim.x) {
-        const size_t i0 = c * 16, i1 = i0 + 16 < n ? i0 + 16 : n;
-        int cur = -1, cnt = 0;
-        for (size_t i = i0; i < i1; ++i) {
-            const int r = L[i];
-            if (r != cur) {
-                if (cur >= 0) atomicAdd(&area[cur], cnt);
-                cur = r;
-                cnt = 0;
-            }
-            ++cnt;
+    const int lane = threadIdx.x & 63;
+    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const size_t nwave = ((size_t)gridDim.x * blockDim.x) >> 6;
+    for (size_t base = wave * 64; base < n; base += nwave * 64) {
+        const size_t i = base + lane;
+        const int r = i < n ? L[i] : -2;
+        const int prev = __shfl_up(r, 1);
+        const bool head = (lane == 0 || prev != r) && r != -2;
+        const unsigned long long heads = __ballot(head);
+        if (head && r >= 0) {
+            // next run start after this lane (or the end of the window / of the image)
+            const unsigned long long later = lane == 63 ? 0ull : (heads >> (lane + 1));
+            int end = later ? lane + 1 + __ffsll((long long)later) - 1 : 64;
+            const size_t lim = n - base;  // valid lanes in this window
+            if ((size_t)end > lim) end = (int)lim;
+            atomicAdd(&area[r], end - lane);
         }
-        if (cur >= 0) atomicAdd(&area[cur], cnt);
     }
 }
 
 // bwareaopen2 flatten output: 255 where lo <= area(root) < hi; count kept roots
 __global__ void k_area_select(const int32_t* __restrict__ L, const int32_t* __restrict__ area, uint8_t* __restrict__ out,
                               size_t n, int lo, int hi, int* count) {
+    int local = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         int r = L[i];
         uint8_t o = 0;
@@ -249,10 +255,11 @@ __global__ void k_area_select(const int32_t* __restrict__ L, const int32_t* __re
             int a = area[r];
             bool keep = a >= lo && a < hi;
             o = keep ? 255 : 0;
-            if (keep && (size_t)r == i && count) atomicAdd(count, 1);
+            local += (keep && (size_t)r == i) ? 1 : 0;
         }
         out[i] = o;
     }
+    if (count) block_count_add(local, count);  // `count` is the same for every thread
 }
 
 // flag[root] = 1 for every component that holds a pixel with src[i] != 0 (src may be null:
@@ -456,7 +463,7 @@ int bwareaopen_impl(const uint8_t* img, uint8_t* out, int w, int h, int lo, int 
     NSC_HIP(hipMemsetAsync(area, 0, n * sizeof(int32_t), s));
     hipLaunchKernelGGL(k_area, grid1(n), dim3(256), 0, s, labels, area, n);
     if (dcount) NSC_HIP(hipMemsetAsync(dcount, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_area_select, grid1(n), dim3(256), 0, s, labels, area, out, n, lo, hi, dcount);
+    hipLaunchKernelGGL(k_area_select, grid_counted(n), dim3(256), 0, s, labels, area, out, n, lo, hi, dcount);
     NSC_HIP(hipGetLastError());
     return NSC_OK;
 }

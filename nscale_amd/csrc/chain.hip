@@ -99,9 +99,7 @@ __global__ void k_s1_pixels(const uint8_t* __restrict__ bgr, size_t n, uint8_t b
         r2b[i] = (uint8_t)rb;
         rc[i] = (uint8_t)rcv;
     }
-    // wave reduction then one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
-    if ((threadIdx.x & 63) == 0 && local) atomicAdd(bg_count, local);
+    block_count_add(local, bg_count);
 }
 
 // diffIm = sat(rc - rc_recon) (HE.cpp:211); thresholds at G1 / G2 (:227, :279)
@@ -328,7 +326,7 @@ int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p
 
     // ---- S1: plFindNucleusCandidates ----
     NSC_HIP(hipMemsetAsync(W.counts, 0, 4 * sizeof(int), s));
-    hipLaunchKernelGGL(k_s1_pixels, grid1(n), dim3(256), 0, s, bgr, n, p.blue, p.green, p.red, (float)p.T1,
+    hipLaunchKernelGGL(k_s1_pixels, grid_counted(cdiv(n, 4)), dim3(256), 0, s, bgr, n, p.blue, p.green, p.red, (float)p.T1,
                        (float)p.T2, (stage && stage[1]) ? W.bgm : nullptr, W.bw1, W.bw2, W.r2b, W.rc, &W.counts[0],
                        (reinterpret_cast<uintptr_t>(bgr) & 3) == 0 ? 1 : 0);
     NSC_HIP(hipGetLastError());

@@ -198,6 +198,44 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p) 
     return (__attribute__((address_space(1))) T*)p;
 }
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Counters shared by a whole launch: atomics on one address from every wave serialise (one
+// address lives in one L2 channel), so counts are summed per workgroup first and added with one
+// atomic per workgroup.  Every thread of the block calls these, with the same control flow.
+__device__ __forceinline__ int block_sum(int v) {
+    __shared__ int part[16];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    const int wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();  // part[] may still be read by a previous call
+    if ((threadIdx.x & 63) == 0) part[wv] = v;
+    __syncthreads();
+    int s = 0;
+    for (int k = 0; k < nw; ++k) s += part[k];
+    return s;
+}
+__device__ __forceinline__ void block_count_add(int v, int* counter) {
+    const int s = block_sum(v);
+    if (threadIdx.x == 0 && s) atomicAdd(counter, s);
+}
+// appends v for the threads whose `take` is set: one atomic per workgroup, list order arbitrary
+__device__ __forceinline__ void block_append(bool take, int v, int* __restrict__ list, int* __restrict__ count) {
+    __shared__ int wbase[16];
+    __shared__ int bbase;
+    const unsigned long long m = __ballot(take);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();  // wbase / bbase may still be read by a previous call
+    if (lane == 0) wbase[wv] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int k = 0; k < nw; ++k) { const int c = wbase[k]; wbase[k] = tot; tot += c; }
+        bbase = tot ? atomicAdd(count, tot) : 0;
+    }
+    __syncthreads();
+    if (take) list[bbase + wbase[wv] + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+// grid for a grid-stride pass over n items with one counter update per workgroup
+inline dim3 grid_counted(size_t n) { return dim3((unsigned)std::min<size_t>(cdiv(n, 256), 1024)); }
 // x where keep is all ones, 0 (all bits clear) where keep is 0.  Written as a bit operation on
 // purpose: a select between a loaded value and a constant may be compiled into a branch around
 // the load, which serialises the loads of an unrolled window.

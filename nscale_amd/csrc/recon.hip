@@ -100,12 +100,6 @@ struct RShared {
 template <typename R>
 __device__ __forceinline__ R clamp_up(R v, R n, R mk) { return smin(smax(v, n), mk); }
 template <>
-__device__ __forceinline__ uint32_t clamp_up(uint32_t v, uint32_t n, uint32_t mk) {
-    uint32_t r;
-    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(n), "v"(mk));
-    return r;
-}
-template <>
 __device__ __forceinline__ int32_t clamp_up(int32_t v, int32_t n, int32_t mk) {
     int32_t r;
     asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(n), "v"(mk));

@@ -166,8 +166,7 @@ __global__ void k_classify(Dev D, uint8_t* cls, uint16_t* lvl0, uint8_t* fmask, 
         D.m[i] = frame ? WSHED : (v < 0 ? 0 : v);
         local += (c == C_F);
     }
-    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
-    if ((threadIdx.x & 63) == 0 && local) atomicAdd(nF, local);
+    block_count_add(local, nF);
 }
 
 // ---- phase 1: one lane floods one run ----
@@ -584,29 +583,34 @@ __global__ void k_bresolve(Dev D, int32_t* __restrict__ real, int* __restrict__ 
                            const int* __restrict__ blist, const int* __restrict__ bcount) {
     const int W = D.W;
     const int nb = *bcount;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += gridDim.x * blockDim.x) {
-        const int b = blist[k];
-        int r = -1;
-        if (D.lvl0[b] != NOPUSH) r = b;
-        else {
-            for (int sl = D.head[b]; sl >= 0; sl = D.lf_next[sl]) {
-                const int lid = D.N + sl;
-                if (r < 0 || before(D, D.lf_parent[sl], it_parent(D, r))) r = lid;
+    // block-uniform trip count (block_append synchronises the block)
+    for (int k0 = blockIdx.x * blockDim.x; k0 < nb; k0 += gridDim.x * blockDim.x) {
+        const int k = k0 + threadIdx.x;
+        int b = -1, lab = 0;
+        if (k < nb) {
+            b = blist[k];
+            int r = -1;
+            if (D.lvl0[b] != NOPUSH) r = b;
+            else {
+                for (int sl = D.head[b]; sl >= 0; sl = D.lf_next[sl]) {
+                    const int lid = D.N + sl;
+                    if (r < 0 || before(D, D.lf_parent[sl], it_parent(D, r))) r = lid;
+                }
+            }
+            real[b] = r;
+            if (r >= 0) {
+                const int dirs[4] = {-1, 1, -W, W};
+                for (int d = 0; d < 4; ++d) {
+                    const int s = b + dirs[d];
+                    int t = 0;
+                    const uint8_t cs = D.cls[s];
+                    if (cs == C_SEED) t = D.m0[s];
+                    else if (cs == C_F && D.st[s] == 2 && D.lab[s] > 0 && before(D, s, r)) t = D.lab[s];
+                    if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+                }
             }
         }
-        real[b] = r;
-        if (r < 0) continue;
-        const int dirs[4] = {-1, 1, -W, W};
-        int lab = 0;
-        for (int d = 0; d < 4; ++d) {
-            const int s = b + dirs[d];
-            int t = 0;
-            const uint8_t cs = D.cls[s];
-            if (cs == C_SEED) t = D.m0[s];
-            else if (cs == C_F && D.st[s] == 2 && D.lab[s] > 0 && before(D, s, r)) t = D.lab[s];
-            if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
-        }
-        if (lab > 0) cand[atomicAdd(ncand, 1)] = b;
+        block_append(lab > 0, b, cand, ncand);
     }
 }
 
@@ -1271,7 +1275,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     D.img = img; D.W = W; D.H = H; D.N = (int)N; D.m0 = m0; D.m = m; D.cls = cls; D.lvl0 = lvl0; D.run = run;
     D.st = st; D.ilvl = ilvl; D.pseq = pseq; D.parent = parent; D.bnode = bnode; D.popidx = popidx; D.lab = lab;
     D.head = head; D.nleaf = &small[2]; D.flags = &small[3];
-    hipLaunchKernelGGL(k_classify, grid1(N), dim3(256), 0, s, D, cls, lvl0, fmask, &small[0]);
+    hipLaunchKernelGGL(k_classify, grid_counted(N), dim3(256), 0, s, D, cls, lvl0, fmask, &small[0]);
     tick("classify", &small[0]);
     NSC_CHECK(ccl_neq(fmask, 0, run, W, H, 4, s));
     NSC_CHECK(relabel_impl(run, W, H, -1, &small[1], lab2, s));
@@ -1340,7 +1344,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     // B resolution and b1
     int* cand = lab2;  // reuse (N ints)
     NSC_HIP(hipMemsetAsync(real, 0xFF, sizeof(int32_t) * N, s));  // -1 everywhere
-    hipLaunchKernelGGL(k_bresolve, grid1(N), dim3(256), 0, s, D, real, cand, &small[4], blist, &small[35]);
+    hipLaunchKernelGGL(k_bresolve, grid_counted(N), dim3(256), 0, s, D, real, cand, &small[4], blist, &small[35]);
     tick("bresolve", &small[4]);
     {
         int* partial = small + 64;

@@ -52,9 +52,7 @@ __device__ __forceinline__ bool joins(const Px& P, size_t i, size_t j) {
 // the same pointer continues its own loop with the value it replaced (see unite), so a lost
 // update only shortens a path.
 // relaxed atomic accesses: other threads update the forest concurrently, so no access may be
-// cached or reordered by the compiler
-__device__ __forceinline__ int ld_rel(const int* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ void st_rel(int* a, int v) { __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// kept in a register by the compiler
 __device__ __forceinline__ int ld_wg(const int* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ __forceinline__ void st_wg(int* a, int v) { __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
@@ -82,6 +80,10 @@ __device__ void lunite(int* L, int a, int b) {
         b = old;
     }
 }
+
+// relaxed device-scope accesses: unions from every workgroup of the launch update the forest
+__device__ __forceinline__ int ld_rel(const int* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void st_rel(int* a, int v) { __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 __device__ __forceinline__ int gfind(int32_t* L, int x) {
     int p = ld_rel(&L[x]);
@@ -148,7 +150,11 @@ __global__ __launch_bounds__(256) void ccl_local(Px P, int32_t* __restrict__ L, 
 }
 
 // unions across tile seams: every pixel in a tile's first row, first column or last column
-// joins its "backward" neighbours (left, up, up-left, up-right) that lie in another tile
+// joins its "backward" neighbours (left, up, up-left, up-right) that lie in another tile.  Along a
+// seam, consecutive pixels usually link the same two tile-local components: a pixel skips a union
+// when the previous pixel of the seam joins, in the same direction, two pixels whose labels are the
+// ones it sees (the labels read are ancestors of those pixels, so the previous pixel's union joins
+// the same two sets).
 template <int MODE, int CONN>
 __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h, int tiles_x, int ntiles) {
     const int per_tile = CT_W + 2 * CT_H;
@@ -156,26 +162,39 @@ __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h,
     if (gid >= ntiles * per_tile) return;
     const int t = gid / per_tile, e = gid % per_tile;
     const int tx = t % tiles_x, ty = t / tiles_x;
-    int lx, ly;
-    if (e < CT_W) { lx = e; ly = 0; }
-    else if (e < CT_W + CT_H) { lx = 0; ly = e - CT_W; }
-    else { lx = CT_W - 1; ly = e - CT_W - CT_H; }
+    int lx, ly, sx, sy;  // position on the seam and the step to the previous seam pixel
+    if (e < CT_W) { lx = e; ly = 0; sx = -1; sy = 0; }
+    else if (e < CT_W + CT_H) { lx = 0; ly = e - CT_W; sx = 0; sy = -1; }
+    else { lx = CT_W - 1; ly = e - CT_W - CT_H; sx = 0; sy = -1; }
+    const bool has_prev = (sx ? lx : ly) > 0;
     const int x = tx * CT_W + lx, y = ty * CT_H + ly;
     if (x >= w || y >= h) return;
     const size_t i = (size_t)y * w + x;
-    if (L[i] < 0) return;
+    const int li = L[i];
+    if (li < 0) return;
     auto cross = [&](int qx, int qy) { return (qx / CT_W != tx) || (qy / CT_H != ty); };
-    auto try_join = [&](int qx, int qy) {
-        if (qx < 0 || qy < 0 || qx >= w || !cross(qx, qy)) return;
-        const size_t j = (size_t)qy * w + qx;
-        if (L[j] < 0 || !joins<MODE>(P, i, j)) return;
-        gunite(L, (int)i, (int)j);
+    // label of the cross-tile partner of pixel (px, py) in direction (dx, dy), -1 if it does not join
+    auto partner = [&](int px, int py, int dx, int dy, int lp) {
+        const int qx = px + dx, qy = py + dy;
+        if (lp < 0 || qx < 0 || qy < 0 || qx >= w || !cross(qx, qy)) return -1;
+        const size_t pi = (size_t)py * w + px, j = (size_t)qy * w + qx;
+        const int lj = L[j];
+        if (lj < 0 || !joins<MODE>(P, pi, j)) return -1;
+        return lj;
     };
-    try_join(x - 1, y);
-    try_join(x, y - 1);
+    const int px = x + sx, py = y + sy;
+    const int lprev = has_prev ? L[(size_t)py * w + px] : -1;
+    auto try_join = [&](int dx, int dy) {
+        const int lj = partner(x, y, dx, dy, li);
+        if (lj < 0) return;
+        if (has_prev && lprev == li && partner(px, py, dx, dy, lprev) == lj) return;  // same union
+        gunite(L, (int)i, (int)((size_t)(y + dy) * w + (x + dx)));
+    };
+    try_join(-1, 0);
+    try_join(0, -1);
     if (CONN == 8) {
-        try_join(x - 1, y - 1);
-        try_join(x + 1, y - 1);
+        try_join(-1, -1);
+        try_join(1, -1);
     }
 }
 
@@ -187,10 +206,11 @@ __global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval) {
             continue;
         }
         // read-only walk: a path-jumping find here could re-point a pixel whose final root another
-        // thread has already written back to an intermediate ancestor
+        // thread has already written back to an intermediate ancestor.  Roots are final in this
+        // kernel and every value an entry takes is an ancestor, so cached loads are safe.
         int r = p;
         while (true) {
-            const int q = ld_rel(&L[r]);
+            const int q = ld_wg(&L[r]);
             if (q == r) break;
             r = q;
         }

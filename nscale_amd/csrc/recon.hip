@@ -1018,38 +1018,40 @@ __device__ __forceinline__ void geo_load(const uint8_t* pass, const unsigned* di
 // The sweeps do not track changes step by step (a compare per step, and the compiler keeps the
 // compare results in scalar registers until it ORs them as a tree); distances only fall, so a
 // column changed iff the 64-bit sum of ~v over it rose (geo_key_sum).
+// Along one line the shortest path to a pixel comes straight from above or straight from below
+// (a path that turns back covers pixels twice), so the downward and the upward sweep can run on
+// the same input as two independent chains, interleaved for instruction-level parallelism, and
+// their minimum is exactly the sequential down-then-up result.
 __device__ __forceinline__ void geo_col_sweeps(unsigned (&v)[TH], GeoFlags& F) {
     F.opaque();
-    unsigned pc = v[0];
+    unsigned dn[TH], up[TH];
+    unsigned pd = v[0], pu = v[TH - 1];
 #pragma unroll
-    for (int y = 1; y < TH - 1; ++y) {
-        v[y] = geo_step(v[y], pc, F.blk_col(y));
-        pc = v[y];
+    for (int k = 1; k < TH - 1; ++k) {
+        const int yd = k, yu = TH - 1 - k;
+        dn[yd] = geo_step(v[yd], pd, F.blk_col(yd));
+        pd = dn[yd];
+        up[yu] = geo_step(v[yu], pu, F.blk_col(yu));
+        pu = up[yu];
     }
-    F.opaque();  // no sharing of unpacked flags between the two directions (register pressure)
-    pc = v[TH - 1];
 #pragma unroll
-    for (int y = TH - 2; y >= 1; --y) {
-        v[y] = geo_step(v[y], pc, F.blk_col(y));
-        pc = v[y];
-    }
+    for (int y = 1; y < TH - 1; ++y) v[y] = min(dn[y], up[y]);
 }
 
 __device__ __forceinline__ void geo_row_sweeps(unsigned (&v)[TW], GeoFlags& F) {
     F.opaque();
-    unsigned prev = v[0];
+    unsigned lt[TW], rt[TW];
+    unsigned pl = v[0], pr = v[TW - 1];
 #pragma unroll
-    for (int x = 1; x < TW - 1; ++x) {
-        v[x] = geo_step(v[x], prev, F.blk_row(x));
-        prev = v[x];
+    for (int k = 1; k < TW - 1; ++k) {
+        const int xl = k, xr = TW - 1 - k;
+        lt[xl] = geo_step(v[xl], pl, F.blk_row(xl));
+        pl = lt[xl];
+        rt[xr] = geo_step(v[xr], pr, F.blk_row(xr));
+        pr = rt[xr];
     }
-    F.opaque();
-    prev = v[TW - 1];
 #pragma unroll
-    for (int x = TW - 2; x >= 1; --x) {
-        v[x] = geo_step(v[x], prev, F.blk_row(x));
-        prev = v[x];
-    }
+    for (int x = 1; x < TW - 1; ++x) v[x] = min(lt[x], rt[x]);
 }
 
 // sum of ~v over rows [y0, y1) of the lane's column (monotone in every value as distances fall)

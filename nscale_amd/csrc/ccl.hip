@@ -19,6 +19,7 @@
 //                FG_FEQ  -> foreground = cls != 0; neighbours join when float values are equal
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -112,26 +113,43 @@ __device__ void gunite(int32_t* L, int a, int b) {
 
 template <int MODE, int CONN>
 __global__ __launch_bounds__(256) void ccl_local(Px P, int32_t* __restrict__ L, int w, int h, int tiles_x) {
+    constexpr int PER = CT_N / 256;  // pixels per thread
     __shared__ int ll[CT_N];
+    // pixel values the joins compare (equal-value modes), staged once
+    __shared__ typename std::conditional<MODE == FG_FEQ, float, uint8_t>::type lv[MODE == FG_NEQ ? 1 : CT_N];
     const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
     const int x0 = tx * CT_W, y0 = ty * CT_H;
-    for (int k = threadIdx.x; k < CT_N; k += blockDim.x) {
-        const int lx = k % CT_W, ly = k / CT_W;
-        const int gx = x0 + lx, gy = y0 + ly;
-        bool f = false;
-        if (gx < w && gy < h) f = fg_of<MODE>(P, (size_t)gy * w + gx);
-        ll[k] = f ? k : -1;
+    // every load first, from clamped (always valid) addresses: a load per iteration with its use in
+    // the same iteration compiles to one memory round trip after another
+    uint8_t c[PER];
+    float f[PER];
+    const auto gimg = as_global(P.img);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int k = threadIdx.x + 256 * j, lx = k % CT_W, ly = k / CT_W;
+        const size_t i = (size_t)clampi(y0 + ly, 0, h - 1) * w + clampi(x0 + lx, 0, w - 1);
+        c[j] = gimg[i];
+        if (MODE == FG_FEQ) f[j] = as_global(P.fv)[i];
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int k = threadIdx.x + 256 * j, lx = k % CT_W, ly = k / CT_W;
+        const bool in = x0 + lx < w && y0 + ly < h;
+        const bool fg = in && (MODE == FG_NEQ ? c[j] != P.key : c[j] != 0);
+        ll[k] = fg ? k : -1;
+        if (MODE == FG_EQ) lv[k] = c[j];
+        if (MODE == FG_FEQ) lv[k] = f[j];
     }
     __syncthreads();
+    auto join = [&](int a, int b) { return MODE == FG_NEQ || lv[a] == lv[b]; };
     for (int k = threadIdx.x; k < CT_N; k += blockDim.x) {
         if (ll[k] < 0) continue;
         const int lx = k % CT_W, ly = k / CT_W;
-        const size_t gi = (size_t)(y0 + ly) * w + (x0 + lx);
-        if (lx > 0 && ll[k - 1] >= 0 && joins<MODE>(P, gi, gi - 1)) lunite(ll, k, k - 1);
-        if (ly > 0 && ll[k - CT_W] >= 0 && joins<MODE>(P, gi, gi - w)) lunite(ll, k, k - CT_W);
+        if (lx > 0 && ll[k - 1] >= 0 && join(k, k - 1)) lunite(ll, k, k - 1);
+        if (ly > 0 && ll[k - CT_W] >= 0 && join(k, k - CT_W)) lunite(ll, k, k - CT_W);
         if (CONN == 8 && ly > 0) {
-            if (lx > 0 && ll[k - CT_W - 1] >= 0 && joins<MODE>(P, gi, gi - w - 1)) lunite(ll, k, k - CT_W - 1);
-            if (lx < CT_W - 1 && ll[k - CT_W + 1] >= 0 && joins<MODE>(P, gi, gi - w + 1)) lunite(ll, k, k - CT_W + 1);
+            if (lx > 0 && ll[k - CT_W - 1] >= 0 && join(k, k - CT_W - 1)) lunite(ll, k, k - CT_W - 1);
+            if (lx < CT_W - 1 && ll[k - CT_W + 1] >= 0 && join(k, k - CT_W + 1)) lunite(ll, k, k - CT_W + 1);
         }
     }
     __syncthreads();
@@ -157,6 +175,7 @@ __global__ __launch_bounds__(256) void ccl_local(Px P, int32_t* __restrict__ L, 
 // the same two sets).
 template <int MODE, int CONN>
 __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h, int tiles_x, int ntiles) {
+    constexpr int ND = CONN == 8 ? 4 : 2;
     const int per_tile = CT_W + 2 * CT_H;
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= ntiles * per_tile) return;
@@ -169,32 +188,42 @@ __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h,
     const bool has_prev = (sx ? lx : ly) > 0;
     const int x = tx * CT_W + lx, y = ty * CT_H + ly;
     if (x >= w || y >= h) return;
-    const size_t i = (size_t)y * w + x;
-    const int li = L[i];
+    const int px = x + sx, py = y + sy;
+    const int dxs[4] = {-1, 0, -1, 1}, dys[4] = {0, -1, -1, -1};
+    // every label and value this pixel needs, loaded together from clamped (always valid)
+    // addresses; the checks below discard what lies outside the image or inside the tile
+    auto at = [&](int qx, int qy) { return (size_t)clampi(qy, 0, h - 1) * w + clampi(qx, 0, w - 1); };
+    const size_t i = (size_t)y * w + x, ip = at(px, py);
+    int lj[ND], ljp[ND];
+    size_t j[ND], jp[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        j[d] = at(x + dxs[d], y + dys[d]);
+        jp[d] = at(px + dxs[d], py + dys[d]);
+    }
+    const int li = ld_wg(&L[i]), lp = ld_wg(&L[ip]);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        lj[d] = ld_wg(&L[j[d]]);
+        ljp[d] = ld_wg(&L[jp[d]]);
+    }
     if (li < 0) return;
     auto cross = [&](int qx, int qy) { return (qx / CT_W != tx) || (qy / CT_H != ty); };
-    // label of the cross-tile partner of pixel (px, py) in direction (dx, dy), -1 if it does not join
-    auto partner = [&](int px, int py, int dx, int dy, int lp) {
-        const int qx = px + dx, qy = py + dy;
-        if (lp < 0 || qx < 0 || qy < 0 || qx >= w || !cross(qx, qy)) return -1;
-        const size_t pi = (size_t)py * w + px, j = (size_t)qy * w + qx;
-        const int lj = L[j];
-        if (lj < 0 || !joins<MODE>(P, pi, j)) return -1;
-        return lj;
-    };
-    const int px = x + sx, py = y + sy;
-    const int lprev = has_prev ? L[(size_t)py * w + px] : -1;
-    auto try_join = [&](int dx, int dy) {
-        const int lj = partner(x, y, dx, dy, li);
-        if (lj < 0) return;
-        if (has_prev && lprev == li && partner(px, py, dx, dy, lprev) == lj) return;  // same union
-        gunite(L, (int)i, (int)((size_t)(y + dy) * w + (x + dx)));
-    };
-    try_join(-1, 0);
-    try_join(0, -1);
-    if (CONN == 8) {
-        try_join(-1, -1);
-        try_join(1, -1);
+    auto valid = [&](int qx, int qy) { return qx >= 0 && qy >= 0 && qx < w && qy < h && cross(qx, qy); };
+    const int lprev = has_prev ? lp : -1;
+    // Along a seam, consecutive pixels usually link the same two tile-local components: a pixel
+    // skips its union when the previous seam pixel joins, in the same direction, two pixels whose
+    // labels are the ones it sees (labels read are ancestors of those pixels, so the previous
+    // pixel's union joins the same two sets).
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const int qx = x + dxs[d], qy = y + dys[d];
+        if (!valid(qx, qy) || lj[d] < 0 || !joins<MODE>(P, i, j[d])) continue;
+        if (has_prev && lprev == li) {
+            const int rx = px + dxs[d], ry = py + dys[d];
+            if (valid(rx, ry) && ljp[d] == lj[d] && joins<MODE>(P, ip, jp[d])) continue;  // same union
+        }
+        gunite(L, (int)i, (int)j[d]);
     }
 }
 

@@ -50,23 +50,23 @@ template <int CONN>
 __global__ void k_lmax_cand(const float* __restrict__ c, const float* __restrict__ marker, const float* __restrict__ rec,
                             uint8_t* __restrict__ cand, uint8_t* __restrict__ greater, int w, int h) {
     const size_t n = (size_t)w * h;
+    const auto gc = as_global(c);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % w), y = (int)(i / w);
-        const bool cd = !(marker[i] < rec[i]);
-        cand[i] = cd ? 1 : 0;
-        bool g = false;
-        if (cd) {
-            const float v = c[i];
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    if ((dx | dy) == 0) continue;
-                    if (CONN == 4 && dx != 0 && dy != 0) continue;
-                    const int xx = x + dx, yy = y + dy;
-                    if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
-                    if (v < c[(size_t)yy * w + xx]) g = true;
-                }
+        // all nine values from clamped (always valid) addresses, issued together; a neighbour
+        // outside the image reads the pixel itself, which is never greater
+        const int xl = x > 0 ? x - 1 : x, xr = x + 1 < w ? x + 1 : x;
+        const size_t ru = (size_t)(y > 0 ? y - 1 : y) * w, r0 = (size_t)y * w, rd = (size_t)(y + 1 < h ? y + 1 : y) * w;
+        const float mk = marker[i], rc = rec[i], v = gc[i];
+        const float n0 = gc[r0 + xl], n1 = gc[r0 + xr], n2 = gc[ru + x], n3 = gc[rd + x];
+        bool g = (v < n0) | (v < n1) | (v < n2) | (v < n3);
+        if (CONN == 8) {
+            const float n4 = gc[ru + xl], n5 = gc[ru + xr], n6 = gc[rd + xl], n7 = gc[rd + xr];
+            g = g | (v < n4) | (v < n5) | (v < n6) | (v < n7);
         }
-        greater[i] = g ? 1 : 0;
+        const bool cd = !(mk < rc);
+        cand[i] = cd ? 1 : 0;
+        greater[i] = (cd && g) ? 1 : 0;
     }
 }
 

@@ -437,20 +437,33 @@ __device__ __forceinline__ void recon_region(const RImg<T>& im, int region, int 
     R v[TH];
     R init_top = 0, init_bot = 0;
     if (valid) {
+        // 16 rows at a time from clamped (always valid) addresses, all loads of a chunk before any
+        // use (see recon_tile)
+        const auto gmask = as_global(mask);
+        const auto gsrc = as_global(pass == 0 ? marker : (const T*)out);
+        const size_t gxc = (size_t)clampi(gx, 0, w - 1);
 #pragma unroll
-        for (int y = 0; y < TH; ++y) {
-            const int gy = ty * OH - 1 + y;
-            const bool inb = col_in && gy >= 0 && gy < h;
-            R val = 0, mk = 0;
-            if (inb) {
-                const size_t i = (size_t)gy * w + gx;
-                mk = (R)mask[i];
-                const R a = (pass == 0) ? (R)marker[i] : (R)out[i];
-                val = smin(a, mk);
+        for (int c = 0; c < TH; c += 16) {
+            R m_ld[16], s_ld[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const size_t i = (size_t)clampi(ty * OH - 1 + c + j, 0, h - 1) * w + gxc;
+                m_ld[j] = (R)gmask[i];
+                s_ld[j] = (R)gsrc[i];
             }
-            const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb;
-            v[y] = val;
-            lm[y][lane] = (T)(frozen ? val : mk);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int y = c + j, gy = ty * OH - 1 + y;
+                const bool inb = col_in && gy >= 0 && gy < h;
+                const uint32_t keep = inb ? ~0u : 0u;
+                const R mk = keep_bits(m_ld[j], keep);
+                const R val = keep_bits(smin(s_ld[j], m_ld[j]), keep);
+                const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb;
+                v[y] = val;
+                lm[y][lane] = (T)(frozen ? val : mk);
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
         if (lane == 1) {
 #pragma unroll

@@ -1323,10 +1323,10 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
     };
     launch(0, nullptr, nullptr, listA, &counts[0]);
     int p = 1;
-    for (;;) {
-        // eight passes per host check: the front needs tens of passes to cross a tile, and an
-        // empty pass costs a few microseconds against a host round trip of tens
-        for (int b = 0; b < 8 && p <= max_pass; ++b, ++p) {
+    // eight passes before the first host check, doubling up to 32: the front needs tens of passes
+    // to cross a tile, and an empty pass costs a few microseconds against a host round trip of tens
+    for (int batch = 8;; batch = std::min(2 * batch, 32)) {
+        for (int b = 0; b < batch && p <= max_pass; ++b, ++p) {
             int* cur = (p & 1) ? listA : listB;
             int* nxt = (p & 1) ? listB : listA;
             launch(p, cur, &counts[p - 1], nxt, &counts[p]);

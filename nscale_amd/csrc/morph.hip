@@ -45,12 +45,29 @@ __global__ __launch_bounds__(256) void morph_kernel(const uint8_t* __restrict__ 
     uint8_t* H = sm + WW * WS;           // nrun x WW x MT
     const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
     const int x0 = tx * MT - r, y0 = ty * MT - r;
-    for (int k = threadIdx.x; k < WW * WW; k += blockDim.x) {
-        const int lx = k % WW, ly = k / WW;
-        const int gx = x0 + lx, gy = y0 + ly;
-        uint8_t v = outside;
-        if (gx >= 0 && gx < w && gy >= 0 && gy < h) v = in[(size_t)gy * w + gx];
-        win[ly * WS + lx] = v;
+    // window load, eight loads per thread in flight: every lane reads a clamped (always valid)
+    // address and pixels beyond the image take `outside` afterwards (a load per iteration with its
+    // use in the same iteration compiles to one memory round trip after another)
+    const auto gin = as_global(in);
+    const int WN = WW * WW;
+    for (int k0 = 0; k0 < WN; k0 += 8 * 256) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = min(k0 + (int)threadIdx.x + 256 * j, WN - 1);
+            const int lx = k % WW, ly = k / WW;
+            v[j] = gin[(size_t)clampi(y0 + ly, 0, h - 1) * w + clampi(x0 + lx, 0, w - 1)];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = k0 + (int)threadIdx.x + 256 * j;
+            if (k >= WN) continue;
+            const int lx = k % WW, ly = k / WW;
+            const int gx = x0 + lx, gy = y0 + ly;
+            const uint32_t keep = (gx >= 0 && gx < w && gy >= 0 && gy < h) ? 0xFFu : 0u;
+            win[ly * WS + lx] = (uint8_t)((v[j] & keep) | ((uint32_t)outside & ~keep));
+        }
     }
     __syncthreads();
     for (int d = 0; d < P.nrun; ++d) {

@@ -899,12 +899,23 @@ __global__ void k_phase2(Dev D, P2 Q, const int32_t* __restrict__ real, long lon
     long long* hk = heapk + (size_t)Q.off[c];
     const int minx = Q.box5[Q.ncomp + c], maxx = Q.box5[2 * Q.ncomp + c];
     const int miny = Q.box5[3 * Q.ncomp + c], maxy = Q.box5[4 * Q.ncomp + c];
-    // collect queued items
+    // collect queued items (raster order), eight pixels' loads in flight at a time: one thread
+    // scans the whole box, and a load per pixel with its test in the same iteration waits on every
+    // pixel in turn
     int ni = 0;
     for (int y = miny; y <= maxy; ++y)
-        for (int x = minx; x <= maxx; ++x) {
-            const int i = y * W + x;
-            if (Q.comp[i] == id && D.m[i] == IN_QUEUE) items[ni++] = i;
+        for (int x0 = minx; x0 <= maxx; x0 += 8) {
+            int cv[8], mv[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int i = y * W + min(x0 + j, maxx);
+                cv[j] = Q.comp[i];
+                mv[j] = D.m[i];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (x0 + j <= maxx && cv[j] == id && mv[j] == IN_QUEUE) items[ni++] = y * W + x0 + j;
         }
     // insertion sort by (level, push time)
     for (int a = 1; a < ni; ++a) {

@@ -8,9 +8,10 @@
 // Labelling is union-find with the smallest raster index as root (the reference's merge links
 // the larger root under the smaller, ConnComponents.cpp:34-43, so its labels are also the
 // smallest index of each component):
-//   1. ccl_local: 64x64 tiles in LDS, unions with atomicMin on local indices (local raster
-//      order = global raster order inside a tile), then each pixel stores the global index of
-//      its tile-local root;
+//   1. ccl_local: 64x64 tiles in LDS, pixels start at the first pixel of their row run, runs
+//      of adjacent rows unite with atomicMin on local indices (local raster order = global
+//      raster order inside a tile), then each pixel stores the global index of its tile-local
+//      root;
 //   2. ccl_seams: pixels on tile seams unite with their cross-tile neighbours in global memory
 //      (atomicMin union, retry while the target stopped being a root);
 //   3. ccl_flatten: every pixel follows its chain to the root (with path compression).
@@ -111,59 +112,117 @@ __device__ void gunite(int32_t* L, int a, int b) {
     }
 }
 
+// One 64 x 64 tile per workgroup; a row of the tile is one wave (lane = column), and thread t
+// handles rows t / 64, t / 64 + 4, ... .  Horizontal joins need no union: each pixel starts out
+// pointing at the first pixel of its run of left-joined pixels (from a ballot of the row).  A
+// vertical or diagonal join is united only where no join further left already connects the
+// same two runs:
+//   up (k, k-64):        skipped when k joins k-1, k-1 joins up, and k-64 joins k-65;
+//   up-left (k, k-65):   skipped when k joins up and k-64 joins k-65, or k joins k-1 and k-1
+//                        joins up;
+//   up-right (k, k-63):  skipped when k joins up and k-63 joins k-64, or k+1 joins k and k+1
+//                        joins up.
+// Every skipped join is implied by a path through joins that are united or skipped only by a rule
+// that looks further left, so the components are those of all joins.
 template <int MODE, int CONN>
 __global__ __launch_bounds__(256) void ccl_local(Px P, int32_t* __restrict__ L, int w, int h, int tiles_x) {
-    constexpr int PER = CT_N / 256;  // pixels per thread
+    constexpr int PER = CT_N / 256;  // rows per thread
+    using V = typename std::conditional<MODE == FG_FEQ, float, uint32_t>::type;
     __shared__ int ll[CT_N];
-    // pixel values the joins compare (equal-value modes), staged once
+    // pixel values the joins compare (equal-value modes)
     __shared__ typename std::conditional<MODE == FG_FEQ, float, uint8_t>::type lv[MODE == FG_NEQ ? 1 : CT_N];
+    __shared__ unsigned long long jrow[CT_H];  // bit x of row y: pixel x joins pixel x - 1
     const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
     const int x0 = tx * CT_W, y0 = ty * CT_H;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // every load first, from clamped (always valid) addresses: a load per iteration with its use in
     // the same iteration compiles to one memory round trip after another
     uint8_t c[PER];
     float f[PER];
     const auto gimg = as_global(P.img);
+    const size_t gxc = (size_t)clampi(x0 + lane, 0, w - 1);
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        const int k = threadIdx.x + 256 * j, lx = k % CT_W, ly = k / CT_W;
-        const size_t i = (size_t)clampi(y0 + ly, 0, h - 1) * w + clampi(x0 + lx, 0, w - 1);
+        const size_t i = (size_t)clampi(y0 + wv + 4 * j, 0, h - 1) * w + gxc;
         c[j] = gimg[i];
         if (MODE == FG_FEQ) f[j] = as_global(P.fv)[i];
     }
+    auto val = [&](int j) -> V {
+        if constexpr (MODE == FG_FEQ) return f[j];
+        else return (V)c[j];
+    };
+    auto left_of = [&](V v) -> V { return from_left((V)0, v); };
+    unsigned fgbits = 0;  // bit j: row wv + 4 j of this column is foreground
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        const int k = threadIdx.x + 256 * j, lx = k % CT_W, ly = k / CT_W;
-        const bool in = x0 + lx < w && y0 + ly < h;
+        const int k = threadIdx.x + 256 * j, ly = wv + 4 * j;
+        const bool in = x0 + lane < w && y0 + ly < h;
         const bool fg = in && (MODE == FG_NEQ ? c[j] != P.key : c[j] != 0);
-        ll[k] = fg ? k : -1;
+        fgbits |= (fg ? 1u : 0u) << j;
         if (MODE == FG_EQ) lv[k] = c[j];
         if (MODE == FG_FEQ) lv[k] = f[j];
+        // the left neighbour is lane - 1 of the same wave (lane 0 receives 0: no left join).  Every
+        // cross-lane read runs on all lanes (none of them under a condition): a lane that is
+        // switched off supplies nothing to its neighbour.
+        const bool fg_left = from_left(0, fg ? 1 : 0) != 0;
+        const V v_left = left_of(val(j));
+        bool jl = fg && fg_left;
+        if (MODE != FG_NEQ) jl = jl && val(j) == v_left;
+        const unsigned long long m = __ballot(jl);
+        // run start: the last position <= lane without a left join (bit 0 is never a join)
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const int start = 63 - __clzll((long long)(~m & upto));
+        ll[k] = fg ? ly * CT_W + start : -1;
+        if (lane == 0) jrow[ly] = m;
     }
     __syncthreads();
-    auto join = [&](int a, int b) { return MODE == FG_NEQ || lv[a] == lv[b]; };
-    for (int k = threadIdx.x; k < CT_N; k += blockDim.x) {
-        if (ll[k] < 0) continue;
-        const int lx = k % CT_W, ly = k / CT_W;
-        if (lx > 0 && ll[k - 1] >= 0 && join(k, k - 1)) lunite(ll, k, k - 1);
-        if (ly > 0 && ll[k - CT_W] >= 0 && join(k, k - CT_W)) lunite(ll, k, k - CT_W);
-        if (CONN == 8 && ly > 0) {
-            if (lx > 0 && ll[k - CT_W - 1] >= 0 && join(k, k - CT_W - 1)) lunite(ll, k, k - CT_W - 1);
-            if (lx < CT_W - 1 && ll[k - CT_W + 1] >= 0 && join(k, k - CT_W + 1)) lunite(ll, k, k - CT_W + 1);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int k = threadIdx.x + 256 * j, ly = wv + 4 * j;
+        if (ly == 0) continue;  // uniform over the wave
+        const bool fg = (fgbits >> j) & 1u;
+        const bool up = ll[k - CT_W] >= 0;
+        V vu = 0;
+        if (MODE != FG_NEQ) vu = (V)lv[k - CT_W];
+        const unsigned long long jr = jrow[ly], ju_row = jrow[ly - 1];
+        const bool jl = (jr >> lane) & 1ull;
+        const bool jl_up = (ju_row >> lane) & 1ull;  // k - 64 joins k - 65
+        bool ju = fg && up;
+        if (MODE != FG_NEQ) ju = ju && val(j) == vu;
+        const int ju_i = ju ? 1 : 0, up_i = up ? 1 : 0;
+        const bool ju_left = from_left(0, ju_i) != 0;
+        const bool do_up = ju && !(jl && ju_left && jl_up);
+        bool do_ul = false, do_ur = false;
+        if (CONN == 8) {
+            const bool up_left = from_left(0, up_i) != 0, up_right = from_right(0, up_i) != 0;
+            const bool ju_right = from_right(0, ju_i) != 0;
+            const V vul = left_of(vu), vur = from_right((V)0, vu);
+            bool jul = fg && up_left, jur = fg && up_right;
+            if (MODE != FG_NEQ) {
+                jul = jul && val(j) == vul;
+                jur = jur && val(j) == vur;
+            }
+            const bool jl_upright = lane < 63 && ((ju_row >> (lane + 1)) & 1ull);  // k - 63 joins k - 64
+            const bool jl_right = lane < 63 && ((jr >> (lane + 1)) & 1ull);        // k + 1 joins k
+            do_ul = jul && !((ju && jl_up) || (jl && ju_left));
+            do_ur = jur && !((ju && jl_upright) || (jl_right && ju_right));
         }
+        if (do_up) lunite(ll, k, k - CT_W);
+        if (do_ul) lunite(ll, k, k - CT_W - 1);
+        if (do_ur) lunite(ll, k, k - CT_W + 1);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < CT_N; k += blockDim.x) {
-        const int lx = k % CT_W, ly = k / CT_W;
-        const int gx = x0 + lx, gy = y0 + ly;
-        if (gx >= w || gy >= h) continue;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int k = threadIdx.x + 256 * j, ly = wv + 4 * j;
+        const int gx = x0 + lane, gy = y0 + ly;
         int r = ll[k];
         int32_t out = -1;
         if (r >= 0) {
             r = lfind(ll, k);
             out = (int32_t)((size_t)(y0 + r / CT_W) * w + (x0 + r % CT_W));
         }
-        L[(size_t)gy * w + gx] = out;
+        if (gx < w && gy < h) L[(size_t)gy * w + gx] = out;
     }
 }
 

@@ -962,10 +962,6 @@ namespace nsc {
 namespace {
 
 constexpr unsigned GINF = 0xFFFFFFFFu;
-#ifndef GEO_CHUNK
-#define GEO_CHUNK 16
-#endif
-
 // Frozen pixels of a window (window halo, outside the image, not passable) as bits, in both
 // orientations: column state fc (bit y & 31 of fc[y >> 5] = pixel (y, lane)), row state fr (bit
 // x & 31 of fr[x >> 5] = pixel (lane, x)).  A step is min(v, sat(prev + 1) | blk) with blk = all
@@ -983,40 +979,33 @@ __device__ __forceinline__ unsigned geo_step(unsigned v, unsigned prev, unsigned
     return min(v, __builtin_elementwise_add_sat(prev, 1u) | blk);
 }
 
-// Load a window's distances into v (column state) and its frozen flags; lanes read clamped,
-// always valid addresses so the loads issue back to back.
-__device__ __forceinline__ void geo_load(const uint8_t* pass, const unsigned* dist, int w, int h, int tx, int ty, int lane,
-                                         unsigned (&v)[TH], GeoFlags& F, uint8_t (*lp)[TW + 4]) {
+// Frozen-pixel bits of every window (the passable set is fixed for a whole geodesic call, so
+// they are computed once): one uint4 per lane, {fc[0], fc[1], fr[0], fr[1]}.
+__global__ __launch_bounds__(64) void geo_flags(const uint8_t* __restrict__ pass, uint4* __restrict__ flags, int w, int h,
+                                                int tiles_x) {
+    __shared__ uint8_t lp[TH][TW + 4];  // frozen flags, row-major (staging for the row-state bits)
+    const int tile = blockIdx.x, tx = tile % tiles_x, ty = tile / tiles_x;
+    const int lane = threadIdx.x;
     const int gx = tx * OW - 1 + lane;
     const bool col_in = (gx >= 0 && gx < w);
     const bool col_frozen = (lane == 0 || lane == TW - 1 || !col_in);
-    const auto gdist = as_global(dist);
     const auto gpass = as_global(pass);
     const size_t gxc = (size_t)clampi(gx, 0, w - 1);
-    F.fc[0] = F.fc[1] = F.fr[0] = F.fr[1] = 0;
-    // 16 rows at a time: all 32 loads of a chunk are issued before any of them is used (the
-    // scheduling fences stop the compiler from sinking loads to their uses, which would wait on
-    // every row in turn)
+    uint32_t fc[2] = {0, 0}, fr[2] = {0, 0};
 #pragma unroll
-    for (int c = 0; c < TH; c += GEO_CHUNK) {
-        unsigned d_ld[GEO_CHUNK], p_ld[GEO_CHUNK];
+    for (int c = 0; c < TH; c += 16) {
+        uint32_t p_ld[16];
 #pragma unroll
-        for (int j = 0; j < GEO_CHUNK; ++j) {
-            const size_t i = (size_t)clampi(ty * OH - 1 + c + j, 0, h - 1) * w + gxc;
-            d_ld[j] = gdist[i];
-            p_ld[j] = gpass[i];
-        }
+        for (int j = 0; j < 16; ++j) p_ld[j] = gpass[(size_t)clampi(ty * OH - 1 + c + j, 0, h - 1) * w + gxc];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int j = 0; j < GEO_CHUNK; ++j) {
+        for (int j = 0; j < 16; ++j) {
             const int y = c + j, gy = ty * OH - 1 + y;
             const bool inb = col_in && gy >= 0 && gy < h;
-            v[y] = d_ld[j] | (inb ? 0u : GINF);
             const bool frozen = col_frozen || y == 0 || y == TH - 1 || !inb || p_ld[j] == 0;
-            F.fc[y >> 5] |= (frozen ? 1u : 0u) << (y & 31);
+            fc[y >> 5] |= (frozen ? 1u : 0u) << (y & 31);
             lp[y][lane] = frozen ? 1 : 0;
         }
-        __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
     // row state: the lane's row of flag bytes, four at a time; (w * 0x10204080) >> 28 gathers the
@@ -1024,8 +1013,34 @@ __device__ __forceinline__ void geo_load(const uint8_t* pass, const unsigned* di
 #pragma unroll
     for (int j = 0; j < TW / 4; ++j) {
         const uint32_t wd = *reinterpret_cast<const uint32_t*>(&lp[lane][4 * j]);
-        F.fr[j >> 3] |= ((wd * 0x10204080u) >> 28) << (4 * (j & 7));
+        fr[j >> 3] |= ((wd * 0x10204080u) >> 28) << (4 * (j & 7));
     }
+    flags[(size_t)tile * TW + lane] = make_uint4(fc[0], fc[1], fr[0], fr[1]);
+}
+
+// Load a window's distances into v (column state) and its frozen bits.  Every lane reads clamped,
+// always valid addresses, and all 64 rows are issued before any is used (the scheduling fence
+// stops the compiler from sinking loads to their uses, which would wait on every row in turn).
+__device__ __forceinline__ void geo_load(const uint4* __restrict__ flags, const unsigned* dist, int w, int h, int tx, int ty,
+                                         int tiles_x, int lane, unsigned (&v)[TH], GeoFlags& F) {
+    const int gx = tx * OW - 1 + lane;
+    const bool col_in = (gx >= 0 && gx < w);
+    const auto gdist = as_global(dist);
+    const size_t gxc = (size_t)clampi(gx, 0, w - 1);
+    const uint4 fw = flags[((size_t)ty * tiles_x + tx) * TW + lane];
+#pragma unroll
+    for (int y = 0; y < TH; ++y) v[y] = gdist[(size_t)clampi(ty * OH - 1 + y, 0, h - 1) * w + gxc];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int y = 0; y < TH; ++y) {
+        const int gy = ty * OH - 1 + y;
+        const bool inb = col_in && gy >= 0 && gy < h;
+        v[y] |= inb ? 0u : GINF;
+    }
+    F.fc[0] = fw.x;
+    F.fc[1] = fw.y;
+    F.fr[0] = fw.z;
+    F.fr[1] = fw.w;
 }
 
 // The sweeps do not track changes step by step (a compare per step, and the compiler keeps the
@@ -1090,13 +1105,12 @@ __device__ __forceinline__ void geo_store(unsigned* dist, int w, int h, int tx, 
     }
 }
 
-__global__ __launch_bounds__(64, 2) void geo_tiles(const uint8_t* __restrict__ pass, unsigned* dist, int w, int h,
+__global__ __launch_bounds__(64, 2) void geo_tiles(const uint4* __restrict__ flags, unsigned* dist, int w, int h,
                                                              int tiles_x, int tiles_y, const int* __restrict__ list,
                                                              const int* __restrict__ list_count, int* __restrict__ next_list,
                                                              int* __restrict__ next_count, unsigned* __restrict__ stamp,
                                                              int pass_no) {
     __shared__ unsigned tr[TH][TW + 1];
-    __shared__ uint8_t lp[TH][TW + 4];  // frozen flags, row-major (staging for the row-state bits)
     __shared__ unsigned init_l[TH], init_r[TH];
     __shared__ unsigned chg_bits;
     int tile;
@@ -1110,7 +1124,7 @@ __global__ __launch_bounds__(64, 2) void geo_tiles(const uint8_t* __restrict__ p
     if (lane == 0) chg_bits = 0;
     unsigned v[TH];
     GeoFlags F;
-    geo_load(pass, dist, w, h, tx, ty, lane, v, F, lp);
+    geo_load(flags, dist, w, h, tx, ty, tiles_x, lane, v, F);
     if (lane == 1) {
 #pragma unroll
         for (int y = 0; y < TH; ++y) init_l[y] = v[y];
@@ -1172,14 +1186,13 @@ __global__ __launch_bounds__(64, 2) void geo_tiles(const uint8_t* __restrict__ p
 template <int G>
 struct GeoRegion {
     unsigned tr[G * G][TH][TW + 1];
-    uint8_t lp[G * G][TH][TW + 4];
     unsigned init_l[G * G][TH], init_r[G * G][TH];
     unsigned xtop[G * G][TW], xbot[G * G][TW], xleft[G * G][TH], xright[G * G][TH];
     unsigned chg_bits;
 };
 
 template <int G>
-__global__ __launch_bounds__(64 * G * G) void geo_regions(const uint8_t* __restrict__ pass, unsigned* dist, int w, int h,
+__global__ __launch_bounds__(64 * G * G) void geo_regions(const uint4* __restrict__ flags, unsigned* dist, int w, int h,
                                                           int tiles_x, int tiles_y, int rgx, int rgy,
                                                           const int* __restrict__ list, const int* __restrict__ list_count,
                                                           int* __restrict__ next_list, int* __restrict__ next_count,
@@ -1201,7 +1214,7 @@ __global__ __launch_bounds__(64 * G * G) void geo_regions(const uint8_t* __restr
         GeoFlags F;
         unsigned init_top = 0, init_bot = 0;
         if (valid) {
-            geo_load(pass, dist, w, h, tx, ty, lane, v, F, sh.lp[wv]);
+            geo_load(flags, dist, w, h, tx, ty, tiles_x, lane, v, F);
             if (lane == 1) {
 #pragma unroll
                 for (int y = 0; y < TH; ++y) { sh.init_l[wv][y] = v[y]; sh.xleft[wv][y] = v[y]; }
@@ -1304,22 +1317,27 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
     const int rgx = (int)cdiv(tiles_x, G), rgy = (int)cdiv(tiles_y, G);
     const int nt = rgx * rgy;
     const int max_pass = 4 * nt + 8;
+    const int nwin = tiles_x * tiles_y;
     Scratch sc;
-    NSC_CHECK(sc.alloc(sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2), s));
-    unsigned* stamp = sc.as<unsigned>();
+    NSC_CHECK(sc.alloc(sizeof(uint4) * TW * (size_t)nwin + sizeof(unsigned) * nt + 2 * sizeof(int) * nt +
+                           sizeof(int) * (max_pass + 2),
+                       s));
+    uint4* flags = sc.as<uint4>();
+    unsigned* stamp = reinterpret_cast<unsigned*>(flags + TW * (size_t)nwin);
     int* listA = reinterpret_cast<int*>(stamp + nt);
     int* listB = listA + nt;
     int* counts = listB + nt;
     NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt, s));
     NSC_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (max_pass + 2), s));
+    hipLaunchKernelGGL(geo_flags, dim3(nwin), dim3(TW), 0, s, pass, flags, w, h, tiles_x);
     const unsigned grid = (unsigned)std::min(nt, 1 << 16);
     auto launch = [&](int p, const int* lst, const int* cnt, int* nl, int* nc) {
         if (G == 2)
-            hipLaunchKernelGGL(geo_regions<2>, dim3(grid), dim3(256), 0, s, pass, dist, w, h, tiles_x, tiles_y, rgx, rgy, lst,
+            hipLaunchKernelGGL(geo_regions<2>, dim3(grid), dim3(256), 0, s, flags, dist, w, h, tiles_x, tiles_y, rgx, rgy, lst,
                                cnt, nl, nc, stamp, p);
         else
-            hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, pass, dist, w, h, tiles_x, tiles_y, lst, cnt, nl, nc, stamp,
-                               p);
+            hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, flags, dist, w, h, tiles_x, tiles_y, lst, cnt, nl, nc,
+                               stamp, p);
     };
     launch(0, nullptr, nullptr, listA, &counts[0]);
     int p = 1;

@@ -14,7 +14,7 @@
 //      root;
 //   2. ccl_seams: pixels on tile seams unite with their cross-tile neighbours in global memory
 //      (atomicMin union, retry while the target stopped being a root);
-//   3. ccl_flatten: every pixel follows its chain to the root (with path compression).
+//   3. ccl_flatten: every pixel follows its chain to the root (a read-only walk).
 // Pixel classes: FG_NEQ  -> foreground = value != key; neighbours join when both foreground
 //                FG_EQ   -> foreground = value != 0; neighbours join when values are equal
 //                FG_FEQ  -> foreground = cls != 0; neighbours join when float values are equal
@@ -286,23 +286,38 @@ __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h,
     }
 }
 
+// Four pixels per thread, i, i + S, i + 2S, i + 3S (S = threads in the grid): each of the four
+// loads is coalesced over the wave, and the four labels and the four first hops are each issued
+// together instead of one dependent round trip after another.
 __global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        int p = L[i];
-        if (p < 0) {
-            if (bgval != -1) L[i] = bgval;
-            continue;
+    const size_t S = (size_t)gridDim.x * blockDim.x;
+    for (size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * S) {
+        int p[4], q[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = i0 + j * S;
+            p[j] = i < n ? ld_wg(&L[i]) : -1;
         }
-        // read-only walk: a path-jumping find here could re-point a pixel whose final root another
-        // thread has already written back to an intermediate ancestor.  Roots are final in this
-        // kernel and every value an entry takes is an ancestor, so cached loads are safe.
-        int r = p;
-        while (true) {
-            const int q = ld_wg(&L[r]);
-            if (q == r) break;
-            r = q;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = p[j] >= 0 ? ld_wg(&L[p[j]]) : p[j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = i0 + j * S;
+            if (i >= n) continue;
+            if (p[j] < 0) {
+                if (bgval != -1) L[i] = bgval;
+                continue;
+            }
+            // read-only walk: a path-jumping find here could re-point a pixel whose final root
+            // another thread has already written back to an intermediate ancestor.  Roots are final
+            // in this kernel and every value an entry takes is an ancestor, so cached loads are safe.
+            int r = p[j], nx = q[j];
+            while (nx != r) {
+                r = nx;
+                nx = ld_wg(&L[r]);
+            }
+            if (r != p[j]) L[i] = r;
         }
-        if (r != p) L[i] = r;
     }
 }
 
@@ -319,7 +334,7 @@ int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipS
         hipLaunchKernelGGL((ccl_local<MODE, 4>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x);
         hipLaunchKernelGGL((ccl_seams<MODE, 4>), dim3(cdiv(nseam, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
     }
-    hipLaunchKernelGGL(ccl_flatten, dim3(std::min<unsigned>(cdiv(n, 256), 65535u)), dim3(256), 0, s, L, n, bgval);
+    hipLaunchKernelGGL(ccl_flatten, dim3(std::min<unsigned>(cdiv(n, 1024), 65535u)), dim3(256), 0, s, L, n, bgval);
     NSC_HIP(hipGetLastError());
     return NSC_OK;
 }

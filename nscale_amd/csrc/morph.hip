@@ -10,7 +10,9 @@
 //      its result by the extra taps only (disk19: 19 taps per pixel for all 5 distinct runs);
 //   3. the output is the min/max over kernel rows of the run result of the matching window row.
 // Traffic: 1 byte in + 1 byte out per pixel (plus the halo re-read, r/32 of the tile).
+// 3 x 3 elements take a direct kernel instead (morph3_kernel).
 #include <algorithm>
+#include <cstdint>
 #include <vector>
 
 #include "common.hpp"
@@ -103,6 +105,62 @@ __global__ __launch_bounds__(256) void morph_kernel(const uint8_t* __restrict__ 
     }
 }
 
+// 3 x 3 elements (the cross of HistologicalEntities.cpp:330 and :498): a thread makes four
+// consecutive outputs of one row from the 3 x 6 input bytes around them (read as one aligned word
+// per row plus its two edge bytes when `words`: both planes 4-byte aligned and the row width a
+// multiple of four; else byte by byte).
+// `mask` bit 3 * (dy + 1) + (dx + 1) selects the element's taps.
+template <bool ERODE>
+__global__ __launch_bounds__(256) void morph3_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, int w, int h,
+                                                     unsigned mask, uint8_t outside, bool words) {
+    const int qw = (w + 3) >> 2;  // four-pixel groups per row
+    const size_t ngroups = (size_t)qw * h;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ngroups; t += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(t / qw), x0 = (int)(t - (size_t)y * qw) * 4;
+        uint32_t b[3][6];  // rows y-1..y+1, columns x0-1..x0+4
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int yy = y + r - 1;
+            const bool row_in = yy >= 0 && yy < h;
+            const uint8_t* row = in + (size_t)clampi(yy, 0, h - 1) * w;
+            if (words && x0 + 3 < w) {
+                const uint32_t wd = *reinterpret_cast<const uint32_t*>(row + x0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[r][1 + j] = (wd >> (8 * j)) & 255u;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[r][1 + j] = row[clampi(x0 + j, 0, w - 1)];
+            }
+            b[r][0] = row[clampi(x0 - 1, 0, w - 1)];
+            b[r][5] = row[clampi(x0 + 4, 0, w - 1)];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                const int xx = x0 + j - 1;
+                if (!row_in || xx < 0 || xx >= w) b[r][j] = outside;
+            }
+        }
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t v = ERODE ? 255u : 0u;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if ((mask >> (3 * r + c)) & 1u) v = pick<ERODE>(v, b[r][j + c]);
+            o[j] = v;
+        }
+        uint8_t* orow = out + (size_t)y * w;
+        if (words && x0 + 3 < w) {
+            *reinterpret_cast<uint32_t*>(orow + x0) = o[0] | (o[1] << 8) | (o[2] << 16) | (o[3] << 24);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (x0 + j < w) orow[x0 + j] = (uint8_t)o[j];
+        }
+    }
+}
+
 int make_plan(const uint8_t* kernel, int k, MorphPlan& P) {
     if (k < 1 || k > MAXK || (k % 2) == 0) {
         set_error("structuring element must be odd-sized and at most %d", MAXK);
@@ -165,6 +223,19 @@ int morph_impl(const uint8_t* in, uint8_t* out, int w, int h, const uint8_t* ker
     const int tiles_x = (int)cdiv(w, MT), tiles_y = (int)cdiv(h, MT);
     if (P.nrow == 0) {  // empty element: erosion -> 255, dilation -> 0 everywhere
         NSC_HIP(hipMemsetAsync(out, erode ? 255 : 0, (size_t)w * h, s));
+        return NSC_OK;
+    }
+    if (k == 3) {
+        unsigned mask = 0;
+        for (int i = 0; i < 9; ++i) mask |= (kernel[i] ? 1u : 0u) << i;
+        const size_t groups = cdiv((size_t)w, 4) * (size_t)h;
+        const dim3 grid((unsigned)std::min<size_t>(cdiv(groups, 256), 16384));
+        const bool words = (w & 3) == 0 && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 3) == 0;
+        if (erode)
+            hipLaunchKernelGGL((morph3_kernel<true>), grid, dim3(256), 0, s, in, out, w, h, mask, (uint8_t)outside, words);
+        else
+            hipLaunchKernelGGL((morph3_kernel<false>), grid, dim3(256), 0, s, in, out, w, h, mask, (uint8_t)outside, words);
+        NSC_HIP(hipGetLastError());
         return NSC_OK;
     }
     if (erode)

@@ -103,15 +103,17 @@ def test_imfill_matlab_doc_example():
     assert np.array_equal(host(G().imfill(dev(d["bw1"]), dev(seeds), 8)), d["bw2"])
 
 
-@pytest.mark.parametrize("kernel", ["disk19", "cross3", "rect5", "holes7"])
-@pytest.mark.parametrize("shape", [(1, 1), (5, 300), (130, 129), (300, 257)])
+@pytest.mark.parametrize("kernel", ["disk19", "cross3", "rect5", "holes7", "rand3"])
+@pytest.mark.parametrize("shape", [(1, 1), (5, 300), (130, 129), (300, 257), (67, 256)])
 def test_morphology(kernel, shape):
     rng = np.random.default_rng(sum(shape))
     img = rng.integers(0, 256, shape).astype(np.uint8)
     k = {"disk19": DISK19, "cross3": CROSS3, "rect5": np.ones((5, 5), np.uint8)}.get(kernel)
-    if k is None:
+    if kernel == "holes7":
         k = (np.random.default_rng(3).random((7, 7)) < 0.5).astype(np.uint8)
         k[3, 3] = 1
+    if kernel == "rand3":  # 3 x 3 without its centre: the direct 3 x 3 kernel, any taps
+        k = np.array([[1, 0, 1], [0, 0, 1], [1, 1, 0]], np.uint8)
     print("morph", kernel, shape, flush=True)
     for outside in (0, 255, 40):
         assert np.array_equal(host(G().erode(dev(img), k, outside)), o_morph("erode", img, k, outside))

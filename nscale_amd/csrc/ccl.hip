@@ -367,20 +367,31 @@ __global__ void k_area(const int32_t* __restrict__ L, int32_t* __restrict__ area
     }
 }
 
+// The per-pixel label consumers below take four pixels per thread and iteration, i, i + S, i + 2S,
+// i + 3S (S = threads in the grid): the four labels, then the four per-root gathers, are issued
+// together instead of one dependent label -> gather round trip per pixel.
+#define NSC_FOR4(n)                                                                                  \
+    for (size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, S = (size_t)gridDim.x * blockDim.x; \
+         i0 < (n); i0 += 4 * S)
+
 // bwareaopen2 flatten output: 255 where lo <= area(root) < hi; count kept roots
 __global__ void k_area_select(const int32_t* __restrict__ L, const int32_t* __restrict__ area, uint8_t* __restrict__ out,
                               size_t n, int lo, int hi, int* count) {
     int local = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        int r = L[i];
-        uint8_t o = 0;
-        if (r >= 0) {
-            int a = area[r];
-            bool keep = a >= lo && a < hi;
-            o = keep ? 255 : 0;
-            local += (keep && (size_t)r == i) ? 1 : 0;
+    NSC_FOR4(n) {
+        int r[4], a[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[j] = i0 + j * S < n ? L[i0 + j * S] : -1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] = r[j] >= 0 ? area[r[j]] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = i0 + j * S;
+            if (i >= n) continue;
+            const bool keep = r[j] >= 0 && a[j] >= lo && a[j] < hi;
+            out[i] = keep ? 255 : 0;
+            local += (keep && (size_t)r[j] == i) ? 1 : 0;
         }
-        out[i] = o;
     }
     if (count) block_count_add(local, count);  // `count` is the same for every thread
 }
@@ -389,17 +400,23 @@ __global__ void k_area_select(const int32_t* __restrict__ L, const int32_t* __re
 // then the image border acts as the source)
 __global__ void k_flag_roots(const int32_t* __restrict__ L, const uint8_t* __restrict__ src, uint8_t* flag, int w, int h) {
     const size_t n = (size_t)w * h;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        int r = L[i];
-        if (r < 0) continue;
-        bool s;
-        if (src) {
-            s = src[i] != 0;
-        } else {
-            int x = (int)(i % w), y = (int)(i / w);
-            s = (x == 0 || y == 0 || x == w - 1 || y == h - 1);
+    NSC_FOR4(n) {
+        int r[4];
+        bool sv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = i0 + j * S;
+            r[j] = i < n ? L[i] : -1;
+            if (src) {
+                sv[j] = i < n && src[i] != 0;
+            } else {
+                const int x = (int)(i % w), y = (int)(i / w);
+                sv[j] = x == 0 || y == 0 || x == w - 1 || y == h - 1;
+            }
         }
-        if (s) flag[r] = 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (r[j] >= 0 && sv[j]) flag[r[j]] = 1;
     }
 }
 
@@ -407,10 +424,20 @@ __global__ void k_flag_roots(const int32_t* __restrict__ L, const uint8_t* __res
 // source, else 0.  imfill additionally ORs with the image (mode 1) using mask = 255 - img.
 __global__ void k_binrecon_out(const int32_t* __restrict__ L, const uint8_t* __restrict__ flag,
                                const uint8_t* __restrict__ marker, uint8_t* __restrict__ out, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        uint8_t m = marker[i];
-        int r = L[i];
-        out[i] = m ? m : ((r >= 0 && flag[r]) ? 255 : 0);
+    NSC_FOR4(n) {
+        int r[4];
+        uint8_t m[4], f[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = i0 + j * S;
+            r[j] = i < n ? L[i] : -1;
+            m[j] = i < n ? marker[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f[j] = r[j] >= 0 ? flag[r[j]] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (i0 + j * S < n) out[i0 + j * S] = m[j] ? m[j] : (f[j] ? 255 : 0);
     }
 }
 
@@ -418,33 +445,59 @@ __global__ void k_binrecon_out(const int32_t* __restrict__ L, const uint8_t* __r
 // reached pixels, 0 elsewhere
 __global__ void k_imfill_out(const int32_t* __restrict__ L, const uint8_t* __restrict__ flag, const uint8_t* __restrict__ img,
                              const uint8_t* __restrict__ seeds, uint8_t* __restrict__ out, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        uint8_t v = img[i];
-        uint8_t rec;
-        if (seeds[i]) rec = (uint8_t)(255 - v);
-        else {
-            int r = L[i];
-            rec = (r >= 0 && flag[r]) ? 255 : 0;
+    NSC_FOR4(n) {
+        int r[4];
+        uint8_t v[4], sd[4], f[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = i0 + j * S;
+            r[j] = i < n ? L[i] : -1;
+            v[j] = i < n ? img[i] : 0;
+            sd[j] = i < n ? seeds[i] : 0;
         }
-        out[i] = v | rec;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f[j] = r[j] >= 0 ? flag[r[j]] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint8_t rec = sd[j] ? (uint8_t)(255 - v[j]) : (f[j] ? 255 : 0);
+            if (i0 + j * S < n) out[i0 + j * S] = v[j] | rec;
+        }
     }
 }
 
 // imfillHoles(binary): 255 unless the pixel is passable (!= 255) and its component touches the border
 __global__ void k_fillholes_out(const int32_t* __restrict__ L, const uint8_t* __restrict__ flag, uint8_t* __restrict__ out,
                                 size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        int r = L[i];
-        out[i] = (r >= 0 && flag[r]) ? 0 : 255;
+    NSC_FOR4(n) {
+        int r[4];
+        uint8_t f[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[j] = i0 + j * S < n ? L[i0 + j * S] : -1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f[j] = r[j] >= 0 ? flag[r[j]] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (i0 + j * S < n) out[i0 + j * S] = f[j] ? 0 : 255;
     }
 }
 
 // bwselect: bw where the component holds a seed pixel
 __global__ void k_bwselect_out(const int32_t* __restrict__ L, const uint8_t* __restrict__ flag, const uint8_t* __restrict__ bw,
                                uint8_t* __restrict__ out, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        int r = L[i];
-        out[i] = (r >= 0 && flag[r]) ? bw[i] : 0;
+    NSC_FOR4(n) {
+        int r[4];
+        uint8_t b[4], f[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const size_t i = i0 + j * S;
+            r[j] = i < n ? L[i] : -1;
+            b[j] = i < n ? bw[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f[j] = r[j] >= 0 ? flag[r[j]] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (i0 + j * S < n) out[i0 + j * S] = f[j] ? b[j] : 0;
     }
 }
 

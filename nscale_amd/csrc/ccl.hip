@@ -100,15 +100,21 @@ __device__ __forceinline__ int gfind(int32_t* L, int x) {
     return p;
 }
 
+// Union in global memory of two ancestors a, b of the pixels to join (the labels a seam pixel has
+// already read, usually tile-local roots that are still roots): both root checks are issued before
+// either is used, and a find runs only for an ancestor that stopped being a root.
 __device__ void gunite(int32_t* L, int a, int b) {
+    const int pa = ld_rel(&L[a]), pb = ld_rel(&L[b]);
+    if (pa != a) a = gfind(L, a);
+    if (pb != b) b = gfind(L, b);
     while (true) {
-        a = gfind(L, a);
-        b = gfind(L, b);
         if (a == b) return;
         if (a > b) { int t = a; a = b; b = t; }
-        int old = atomicMin(&L[b], a);
+        const int old = atomicMin(&L[b], a);
         if (old == b) return;
         b = old;
+        a = gfind(L, a);
+        b = gfind(L, b);
     }
 }
 
@@ -282,7 +288,7 @@ __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h,
             const int rx = px + dxs[d], ry = py + dys[d];
             if (valid(rx, ry) && ljp[d] == lj[d] && joins<MODE>(P, ip, jp[d])) continue;  // same union
         }
-        gunite(L, (int)i, (int)j[d]);
+        gunite(L, li, lj[d]);  // labels are ancestors of their pixels
     }
 }
 

@@ -766,23 +766,36 @@ struct P2 {
     int ncomp;
 };
 
-// order of two queued items of the same level by push time: phase-1 items by pusher order,
-// items queued by the flood from b1 after them (their mutual order is not known: returns
-// false both ways, the caller checks that it does not matter)
-__device__ bool push_before(const Dev& D, const int32_t* real, const uint8_t* kind, int a, int b) {
-    const bool ba = kind[a] == 2, bb = kind[b] == 2;
+// Order of two queued items of the same level by push time: phase-1 items by pusher order, items
+// queued by the flood from b1 after them (their mutual order is not known: they are ordered by the
+// b1 flood's layer, then in raster order, and the caller checks that the order does not matter).
+// The data this reads of one item is packed once, so that a sort compares cached values:
+// bit 63 = queued by the b1 flood (kind 2); then, for kind 2, its label (the layer) in the low 32
+// bits; else (pusher + 1) in bits 32..62 and the push sequence in the low 32 bits.
+__device__ long long push_key(const Dev& D, const int32_t* real, const uint8_t* kind, int a) {
+    if (kind[a] == 2) return (long long)(1ull << 63) | (long long)(uint32_t)D.lab[a];
+    const int ia = D.cls[a] == C_B ? real[a] : a;
+    const int pa = it_parent(D, ia);
+    return ((long long)(uint32_t)(pa + 1) << 32) | (long long)(uint32_t)it_pseq(D, ia);
+}
+// a pushed before b, from their packed data
+__device__ bool push_before_key(const Dev& D, const int32_t* real, int a, long long ka, int b, long long kb) {
+    const bool ba = ka < 0, bb = kb < 0;
     if (ba != bb) return bb;
-    if (ba) {  // the b1 flood's layer first; equal layers: canonical raster order (order unknown)
-        if (D.lab[a] != D.lab[b]) return D.lab[a] < D.lab[b];
+    if (ba) {
+        const int la = (int)(uint32_t)ka, lb = (int)(uint32_t)kb;
+        if (la != lb) return la < lb;
         return a < b;
     }
-    const int ia = D.cls[a] == C_B ? real[a] : a;
-    const int ib = D.cls[b] == C_B ? real[b] : b;
-    const int pa = it_parent(D, ia), pb = it_parent(D, ib);
-    if (pa < 0 && pb < 0) return ia < ib;
+    const int pa = (int)((ka >> 32) & 0x7FFFFFFF) - 1, pb = (int)((kb >> 32) & 0x7FFFFFFF) - 1;
+    if (pa < 0 && pb < 0) {
+        const int ia = D.cls[a] == C_B ? real[a] : a;
+        const int ib = D.cls[b] == C_B ? real[b] : b;
+        return ia < ib;
+    }
     if (pa < 0) return true;
     if (pb < 0) return false;
-    if (pa == pb) return it_pseq(D, ia) < it_pseq(D, ib);
+    if (pa == pb) return (int)(uint32_t)ka < (int)(uint32_t)kb;
     return before(D, pa, pb);
 }
 
@@ -917,20 +930,31 @@ __global__ void k_phase2(Dev D, P2 Q, const int32_t* __restrict__ real, long lon
             for (int j = 0; j < 8; ++j)
                 if (x0 + j <= maxx && cv[j] == id && mv[j] == IN_QUEUE) items[ni++] = y * W + x0 + j;
         }
-    // insertion sort by (level, push time)
-    for (int a = 1; a < ni; ++a) {
+    // insertion sort by (level, push time).  Each item's level and push data are computed once
+    // (into hv[] and hk[], free until the flood below, and ni <= A) and move with the item, so a
+    // comparison reads three cached values instead of re-deriving both items' data.
+    for (int a = 0; a < ni; ++a) {
         const int v = items[a];
-        const int lv = item_level(D, real, Q.kind, v);
+        hv[a] = item_level(D, real, Q.kind, v);
+        hk[a] = push_key(D, real, Q.kind, v);
+    }
+    for (int a = 1; a < ni; ++a) {
+        const int v = items[a], lv = hv[a];
+        const long long kv = hk[a];
         int b = a - 1;
         while (b >= 0) {
-            const int u = items[b];
-            const int lu = item_level(D, real, Q.kind, u);
-            const bool gt = (lu > lv) || (lu == lv && push_before(D, real, Q.kind, v, u));
+            const int u = items[b], lu = hv[b];
+            const long long ku = hk[b];
+            const bool gt = (lu > lv) || (lu == lv && push_before_key(D, real, v, kv, u, ku));
             if (!gt) break;
             items[b + 1] = u;
+            hv[b + 1] = lu;
+            hk[b + 1] = ku;
             --b;
         }
         items[b + 1] = v;
+        hv[b + 1] = lv;
+        hk[b + 1] = kv;
     }
     // groups of items queued by the b1 flood at the same level and layer: order unknown
     TieRec R;
@@ -939,11 +963,9 @@ __global__ void k_phase2(Dev D, P2 Q, const int32_t* __restrict__ real, long lon
     for (int a = 0; a < ni;) {
         const int v = items[a];
         if (Q.kind[v] != 2) { ++a; continue; }
-        const int lv = item_level(D, real, Q.kind, v);
+        const int lv = hv[a];  // cached levels, aligned with the sorted items
         int e = a + 1;
-        while (e < ni && Q.kind[items[e]] == 2 && item_level(D, real, Q.kind, items[e]) == lv &&
-               D.lab[items[e]] == D.lab[v])
-            ++e;
+        while (e < ni && Q.kind[items[e]] == 2 && hv[e] == lv && D.lab[items[e]] == D.lab[v]) ++e;
         if (e - a >= 2) {
             if (R.ntie == MAXG || e - a > MAXGL) { atomicOr(D.flags, FB_TIES); return; }
             R.gfirst[R.ntie] = a;

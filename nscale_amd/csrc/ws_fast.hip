@@ -78,6 +78,9 @@ struct Dev {
     int* nleaf;
     int leafcap;
     int* flags;
+    // per item, packed after phase 1 for the pop-order walks: {run, bnode, popidx, parent} and level
+    int4* wr;
+    uint8_t* wl;
 };
 
 __device__ __forceinline__ int c_diff(const uint8_t* a, const uint8_t* b) {
@@ -113,16 +116,19 @@ __device__ __forceinline__ int it_pseq(const Dev& D, int x) {
     return D.cls[x] == C_F ? D.pseq[x] : 0;
 }
 
-// x pops before y in the merged phase-1 flood (x != y); sets FB_CMP if the walk is too long
+// x pops before y in the merged phase-1 flood (x != y); sets FB_CMP if the walk is too long.
+// Reads the packed item records (k_pack_items), so a step is two dependent loads: the records of
+// x and y, then those of their chains' largest-key items.
 __device__ bool before(const Dev& D, int x, int y) {
     for (int step = 0; step < CMP_MAX_STEPS; ++step) {
         if (x == y) return false;
-        const int rx = it_run(D, x), ry = it_run(D, y);
-        if (rx == ry) return it_popidx(D, x) < it_popidx(D, y);
-        const int a = it_bnode(D, x), b = it_bnode(D, y);
-        const int la = it_lvl(D, a), lb = it_lvl(D, b);
+        const int4 rx = D.wr[x], ry = D.wr[y];  // {run, bnode, popidx, parent}
+        if (rx.x == ry.x) return rx.z < ry.z;
+        const int a = rx.y, b = ry.y;
+        const int4 ra = D.wr[a], rb = D.wr[b];
+        const int la = D.wl[a], lb = D.wl[b];
         if (la != lb) return la < lb;
-        const int pa = it_parent(D, a), pb = it_parent(D, b);
+        const int pa = ra.w, pb = rb.w;
         if (pa < 0 && pb < 0) return a < b;  // both queued in the initial raster scan
         if (pa < 0) return true;
         if (pb < 0) return false;
@@ -549,6 +555,34 @@ __global__ void k_link_leaves(Dev D, int* __restrict__ blist, int* __restrict__ 
             }
         }
         wave_append(first, b, blist, bcount);
+    }
+}
+
+// The item records before() walks, from the phase-1 results: F pixels (run, chain's largest-key
+// item, pop index, pusher, level), B pixels queued in the initial scan (run -(x + 2), themselves,
+// pop index 0, no pusher, initial level) and used leaf slots (ids N + slot).  Other ids are never
+// visited by a walk and keep whatever the memory held.
+__global__ void k_pack_items(Dev D) {
+    const int N = D.N;
+    const int nl = min(*D.nleaf, D.leafcap);
+    const size_t total = (size_t)N + nl;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        if (t < (size_t)N) {
+            const int x = (int)t;
+            const uint8_t c = D.cls[x];
+            if (c == C_F) {
+                D.wr[x] = make_int4(D.run[x], D.bnode[x], D.popidx[x], D.parent[x]);
+                D.wl[x] = D.ilvl[x];
+            } else if (c == C_B && D.lvl0[x] != NOPUSH) {
+                D.wr[x] = make_int4(-(x + 2), x, 0, -1);
+                D.wl[x] = (uint8_t)D.lvl0[x];
+            }
+        } else {
+            const int sl = (int)(t - N);
+            if (D.lf_b[sl] < 0) continue;
+            D.wr[t] = make_int4(D.lf_run[sl], D.lf_bnode[sl], D.lf_popidx[sl], D.lf_parent[sl]);
+            D.wl[t] = D.lf_lvl[sl];
+        }
     }
 }
 
@@ -1373,6 +1407,12 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     int* blist = qnext;  // N + leafcap ints
     hipLaunchKernelGGL(k_collect_b0, grid1(N), dim3(256), 0, s, D, blist, &small[35]);
     if (nrun > 0) hipLaunchKernelGGL(k_link_leaves, grid1(N), dim3(256), 0, s, D, blist, &small[35]);
+    // item records for the pop-order walks (every walk comes after this point)
+    Scratch sc_items;
+    NSC_CHECK(sc_items.alloc((sizeof(int4) + 1) * ((size_t)N + leafcap) + 256, s));
+    D.wr = sc_items.as<int4>();
+    D.wl = reinterpret_cast<uint8_t*>(D.wr + ((size_t)N + leafcap));
+    hipLaunchKernelGGL(k_pack_items, grid1(N), dim3(256), 0, s, D);
     tick("phase1", &small[2]);
     // B resolution and b1
     int* cand = lab2;  // reuse (N ints)

@@ -600,14 +600,17 @@ __global__ void k_collect_b0(Dev D, int* __restrict__ blist, int* __restrict__ b
 // kRunTier[b] pixels; the last bucket floods from global memory
 constexpr int kRunTiers = 6;
 __constant__ int kRunTier[kRunTiers - 1] = {512, 1024, 2048, 4096, 6400};
+// (one append per wave and bucket: an atomic per run on the few bucket counters serialises)
 __global__ void k_bucket_runs(const int32_t* __restrict__ box5, int nrun, int* __restrict__ lists, int* __restrict__ counts) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrun) return;
-    const long long L = (long long)(box5[2 * nrun + r] - box5[nrun + r] + 1 + 2 * P1_MARGIN) *
-                        (box5[4 * nrun + r] - box5[3 * nrun + r] + 1 + 2 * P1_MARGIN);
-    int b = 0;
-    while (b < kRunTiers - 1 && L > kRunTier[b]) ++b;
-    lists[(size_t)b * nrun + atomicAdd(&counts[b], 1)] = r;
+    int b = -1;
+    if (r < nrun) {
+        const long long L = (long long)(box5[2 * nrun + r] - box5[nrun + r] + 1 + 2 * P1_MARGIN) *
+                            (box5[4 * nrun + r] - box5[3 * nrun + r] + 1 + 2 * P1_MARGIN);
+        b = 0;
+        while (b < kRunTiers - 1 && L > kRunTier[b]) ++b;
+    }
+    for (int t = 0; t < kRunTiers; ++t) wave_append(b == t, r, lists + (size_t)t * nrun, &counts[t]);
 }
 
 // ---- B items: the real queue entry of each B pixel and its label decision at pop time ----

@@ -937,62 +937,149 @@ struct FloodQ {
     int* res;    // per-arrangement results of tie components (pool)
 };
 
+// boxes larger than this are scanned by the whole wave (64 pixels a step) instead of one lane, and
+// item lists longer than P2_COOP_SORT (up to 64 * P2_SORT_PER) are sorted by the whole wave
+constexpr int P2_COOP_BOX = 512, P2_COOP_SORT = 16, P2_SORT_PER = 8;
+
+// One lane per component; launched with 64-thread blocks (one wave), which the cooperative scan of
+// large boxes relies on.
 __global__ void k_phase2(Dev D, P2 Q, const int32_t* __restrict__ real, long long* __restrict__ heapk, TieRec* ties,
                          int* nties, int tiecap, long long* pool_used, long long poolcap, FloodQ F) {
+    const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Q.ncomp) return;
+    const bool valid = c < Q.ncomp;
     const int W = D.W;
     const int id = c + 1;
-    const int A = Q.area[c];
-    int32_t* items = Q.scratch + 3 * (size_t)Q.off[c];
+    const int cc = valid ? c : 0;  // a valid index for the loads of lanes without a component
+    const int A = Q.area[cc];
+    int32_t* items = Q.scratch + 3 * (size_t)Q.off[cc];
     int* hv = items + A;
-    long long* hk = heapk + (size_t)Q.off[c];
-    const int minx = Q.box5[Q.ncomp + c], maxx = Q.box5[2 * Q.ncomp + c];
-    const int miny = Q.box5[3 * Q.ncomp + c], maxy = Q.box5[4 * Q.ncomp + c];
-    // collect queued items (raster order), eight pixels' loads in flight at a time: one thread
-    // scans the whole box, and a load per pixel with its test in the same iteration waits on every
-    // pixel in turn
+    long long* hk = heapk + (size_t)Q.off[cc];
+    const int minx = Q.box5[Q.ncomp + cc], maxx = Q.box5[2 * Q.ncomp + cc];
+    const int miny = Q.box5[3 * Q.ncomp + cc], maxy = Q.box5[4 * Q.ncomp + cc];
+    const int bw = maxx - minx + 1, B = bw * (maxy - miny + 1);
+    // collect queued items (raster order).  Boxes above P2_COOP_BOX pixels: the wave scans each
+    // such box in turn, 64 box pixels a step, and compacts the hits with a ballot (a lane alone
+    // would wait on one memory round trip per eight pixels of a box of up to thousands).
     int ni = 0;
-    for (int y = miny; y <= maxy; ++y)
-        for (int x0 = minx; x0 <= maxx; x0 += 8) {
-            int cv[8], mv[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int i = y * W + min(x0 + j, maxx);
-                cv[j] = Q.comp[i];
-                mv[j] = D.m[i];
+    const bool coop = valid && B > P2_COOP_BOX;
+    unsigned long long pend = __ballot(coop);
+    while (pend) {
+        const int src = __ffsll((long long)pend) - 1;
+        pend &= pend - 1;
+        const int sc = __shfl(c, src), sminx = __shfl(minx, src), sminy = __shfl(miny, src);
+        const int sbw = __shfl(bw, src), sB = __shfl(B, src);
+        int32_t* sitems = Q.scratch + 3 * (size_t)Q.off[sc];
+        int cnt = 0;
+        for (int k0 = 0; k0 < sB; k0 += 64) {
+            const int k = k0 + lane;
+            bool take = false;
+            int gi = 0;
+            if (k < sB) {
+                gi = (sminy + k / sbw) * W + sminx + k % sbw;
+                take = Q.comp[gi] == sc + 1 && D.m[gi] == IN_QUEUE;
             }
-            __builtin_amdgcn_sched_barrier(0);
+            const unsigned long long tb = __ballot(take);
+            if (take) sitems[cnt + __popcll(tb & ((1ull << lane) - 1ull))] = gi;
+            cnt += __popcll(tb);
+        }
+        if (lane == src) ni = cnt;
+    }
+    // smaller boxes: the lane scans its own, eight pixels' loads in flight at a time (a load per
+    // pixel with its test in the same iteration waits on every pixel in turn)
+    if (valid && !coop)
+        for (int y = miny; y <= maxy; ++y)
+            for (int x0 = minx; x0 <= maxx; x0 += 8) {
+                int cv[8], mv[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (x0 + j <= maxx && cv[j] == id && mv[j] == IN_QUEUE) items[ni++] = y * W + x0 + j;
+                for (int j = 0; j < 8; ++j) {
+                    const int i = y * W + min(x0 + j, maxx);
+                    cv[j] = Q.comp[i];
+                    mv[j] = D.m[i];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (x0 + j <= maxx && cv[j] == id && mv[j] == IN_QUEUE) items[ni++] = y * W + x0 + j;
+            }
+    // Sort by (level, push time): the FIFO pop order, a strict total order.  Each item's level and
+    // push data are computed once (into hv[] and hk[], free until the flood below, and ni <= A)
+    // and move with the item.  Few items: the lane's own insertion sort.  More (the components of
+    // the LDS floods, up to ~100 items): the wave ranks each item by counting the items that
+    // precede it (one item per lane, all compared at once) and scatters the items to their ranks
+    // -- the same order, without one lane's ~n^2/2 dependent comparisons.
+    const bool coopsort = valid && ni > P2_COOP_SORT && ni <= 64 * P2_SORT_PER;
+    if (valid && !coopsort) {
+        for (int a = 0; a < ni; ++a) {
+            const int v = items[a];
+            hv[a] = item_level(D, real, Q.kind, v);
+            hk[a] = push_key(D, real, Q.kind, v);
         }
-    // insertion sort by (level, push time).  Each item's level and push data are computed once
-    // (into hv[] and hk[], free until the flood below, and ni <= A) and move with the item, so a
-    // comparison reads three cached values instead of re-deriving both items' data.
-    for (int a = 0; a < ni; ++a) {
-        const int v = items[a];
-        hv[a] = item_level(D, real, Q.kind, v);
-        hk[a] = push_key(D, real, Q.kind, v);
-    }
-    for (int a = 1; a < ni; ++a) {
-        const int v = items[a], lv = hv[a];
-        const long long kv = hk[a];
-        int b = a - 1;
-        while (b >= 0) {
-            const int u = items[b], lu = hv[b];
-            const long long ku = hk[b];
-            const bool gt = (lu > lv) || (lu == lv && push_before_key(D, real, v, kv, u, ku));
-            if (!gt) break;
-            items[b + 1] = u;
-            hv[b + 1] = lu;
-            hk[b + 1] = ku;
-            --b;
+        for (int a = 1; a < ni; ++a) {
+            const int v = items[a], lv = hv[a];
+            const long long kv = hk[a];
+            int b = a - 1;
+            while (b >= 0) {
+                const int u = items[b], lu = hv[b];
+                const long long ku = hk[b];
+                const bool gt = (lu > lv) || (lu == lv && push_before_key(D, real, v, kv, u, ku));
+                if (!gt) break;
+                items[b + 1] = u;
+                hv[b + 1] = lu;
+                hk[b + 1] = ku;
+                --b;
+            }
+            items[b + 1] = v;
+            hv[b + 1] = lv;
+            hk[b + 1] = kv;
         }
-        items[b + 1] = v;
-        hv[b + 1] = lv;
-        hk[b + 1] = kv;
     }
+    __syncthreads();  // every item list (stored by every lane in the scans) is complete
+    unsigned long long pend_sort = __ballot(coopsort);
+    while (pend_sort) {
+        const int src = __ffsll((long long)pend_sort) - 1;
+        pend_sort &= pend_sort - 1;
+        const int sc = __shfl(c, src), sni = __shfl(ni, src), sA = __shfl(A, src);
+        int32_t* sitems = Q.scratch + 3 * (size_t)Q.off[sc];
+        int* shv = sitems + sA;
+        long long* shk = heapk + (size_t)Q.off[sc];
+        for (int k = lane; k < sni; k += 64) {
+            const int v = sitems[k];
+            shv[k] = item_level(D, real, Q.kind, v);
+            shk[k] = push_key(D, real, Q.kind, v);
+        }
+        __syncthreads();
+        int rk[P2_SORT_PER], vv[P2_SORT_PER], lvv[P2_SORT_PER];
+        long long kvv[P2_SORT_PER];
+#pragma unroll
+        for (int q = 0; q < P2_SORT_PER; ++q) {
+            const int k = lane + 64 * q;
+            rk[q] = -1;
+            if (k >= sni) continue;
+            const int v = sitems[k], lv = shv[k];
+            const long long kv = shk[k];
+            int rank = 0;
+            for (int j = 0; j < sni; ++j) {
+                if (j == k) continue;
+                const int u = sitems[j], lu = shv[j];
+                rank += (lu < lv) || (lu == lv && push_before_key(D, real, u, shk[j], v, kv)) ? 1 : 0;
+            }
+            rk[q] = rank;
+            vv[q] = v;
+            lvv[q] = lv;
+            kvv[q] = kv;
+        }
+        __syncthreads();  // every comparison has read the lists before they are overwritten
+#pragma unroll
+        for (int q = 0; q < P2_SORT_PER; ++q) {
+            if (rk[q] < 0) continue;
+            sitems[rk[q]] = vv[q];
+            shv[rk[q]] = lvv[q];
+            shk[rk[q]] = kvv[q];
+        }
+        __syncthreads();  // the sorted list is complete before its owner reads it
+    }
+    if (!valid) return;
     // groups of items queued by the b1 flood at the same level and layer: order unknown
     TieRec R;
     R.c = c; R.ni = ni; R.ntie = 0;
@@ -1037,8 +1124,7 @@ __global__ void k_phase2(Dev D, P2 Q, const int32_t* __restrict__ real, long lon
         return;
     }
     // defer to the parallel permutation check (one wave per component)
-    const int B = (maxx - minx + 1) * (maxy - miny + 1);
-    const long long per = (long long)B + ni + A + 2LL * A + 4;
+    const long long per = (long long)B + ni + A + 2LL * A + 4;  // B: box pixels
     const long long need = 64 * per + B + 64;
     const long long base = atomicAdd((unsigned long long*)pool_used, (unsigned long long)need);
     const int k = atomicAdd(nties, 1);

@@ -935,6 +935,7 @@ struct FloodQ {
     int* nwork;  // [2]
     int workcap;
     int* res;    // per-arrangement results of tie components (pool)
+    int* recbad;  // per record: some arrangement differs from arrangement 0
 };
 
 // boxes larger than this are scanned by the whole wave (64 pixels a step) instead of one lane, and
@@ -1327,37 +1328,48 @@ __global__ __launch_bounds__(64) void k_flood_lds(Dev D, P2 Q, const int32_t* __
     }
 }
 
-// tie components flooded in LDS: every arrangement must give arrangement 0's labels, which are committed
+// Tie components flooded in LDS: every arrangement must give arrangement 0's labels, which are then
+// committed.  The comparison takes one workgroup per (record, arrangement) work item, across the
+// device (a workgroup per record would read every arrangement of a record through one CU).
+__global__ __launch_bounds__(256) void k_tie_check(P2 Q, FloodQ F, int W) {
+    const int n0 = min(F.nwork[0], F.workcap), n1 = min(F.nwork[1], F.workcap);
+    for (int t = blockIdx.x; t < n0 + n1; t += gridDim.x) {
+        const int w = t < n0 ? F.work[0][t] : F.work[1][t - n0];
+        const int ri = w / MAXCOMBO, combo = w % MAXCOMBO;
+        const TieRec R = F.recs[ri];
+        if (R.ncombo == 1 || combo == 0) continue;
+        const int c = R.c, id = c + 1;
+        const int ox = Q.box5[Q.ncomp + c] - 1, oy = Q.box5[3 * Q.ncomp + c] - 1;
+        const int lw = Q.box5[2 * Q.ncomp + c] - ox + 2, lh = Q.box5[4 * Q.ncomp + c] - oy + 2;
+        const int L = lw * lh;
+        const int* res = F.res + R.base;
+        bool diff = false;
+        for (int k = threadIdx.x; k < L; k += blockDim.x) {
+            const int g = (oy + k / lw) * W + ox + k % lw;
+            if (Q.comp[g] == id && res[(long long)combo * L + k] != res[k]) diff = true;
+        }
+        if (diff) atomicOr(&F.recbad[ri], 1);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_tie_commit(Dev D, P2 Q, FloodQ F) {
-    __shared__ int bad;
     const int nr = min(*F.nrec, F.reccap);
     for (int ri = blockIdx.x; ri < nr; ri += gridDim.x) {
-    const TieRec R = F.recs[ri];
-    if (R.ncombo == 1) continue;
-    const int c = R.c, id = c + 1, W = D.W;
-    const int ox = Q.box5[Q.ncomp + c] - 1, oy = Q.box5[3 * Q.ncomp + c] - 1;
-    const int lw = Q.box5[2 * Q.ncomp + c] - ox + 2, lh = Q.box5[4 * Q.ncomp + c] - oy + 2;
-    const int L = lw * lh;
-    const int* res = F.res + R.base;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        const int g = (oy + k / lw) * W + ox + k % lw;
-        if (Q.comp[g] != id) continue;
-        const int v0 = res[k];
-        for (int a = 1; a < R.ncombo; ++a)
-            if (res[(long long)a * L + k] != v0) { bad = 1; break; }
-    }
-    __syncthreads();
-    if (bad) {
-        if (threadIdx.x == 0) atomicOr(D.flags, FB_TIEPERM);
-    } else {
+        const TieRec R = F.recs[ri];
+        if (R.ncombo == 1) continue;
+        if (F.recbad[ri]) {
+            if (threadIdx.x == 0) atomicOr(D.flags, FB_TIEPERM);
+            continue;
+        }
+        const int c = R.c, id = c + 1, W = D.W;
+        const int ox = Q.box5[Q.ncomp + c] - 1, oy = Q.box5[3 * Q.ncomp + c] - 1;
+        const int lw = Q.box5[2 * Q.ncomp + c] - ox + 2, lh = Q.box5[4 * Q.ncomp + c] - oy + 2;
+        const int L = lw * lh;
+        const int* res = F.res + R.base;
         for (int k = threadIdx.x; k < L; k += blockDim.x) {
             const int g = (oy + k / lw) * W + ox + k % lw;
             if (Q.comp[g] == id) D.m[g] = res[k];
         }
-    }
-    __syncthreads();  // `bad` is reset for the next record
     }
 }
 
@@ -1571,7 +1583,8 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         // records and work lists of the LDS flood
         const int workcap = ncomp + (1 << 18);
         Scratch sc5;
-        NSC_CHECK(sc5.alloc(sizeof(TieRec) * (size_t)ncomp + 2 * sizeof(int) * (size_t)workcap + 512, s));
+        NSC_CHECK(sc5.alloc(sizeof(TieRec) * (size_t)ncomp + 2 * sizeof(int) * (size_t)workcap + sizeof(int) * (size_t)ncomp + 768,
+                            s));
         FloodQ F{};
         F.recs = reinterpret_cast<TieRec*>(sc5.p);
         F.reccap = ncomp;
@@ -1581,11 +1594,15 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         F.nrec = &small[32];
         F.nwork = &small[33];  // [33], [34]
         F.res = pool;
+        F.recbad = reinterpret_cast<int*>(reinterpret_cast<char*>(F.work[1] + workcap) + 256 -
+                                          (reinterpret_cast<uintptr_t>(F.work[1] + workcap) & 255));
+        NSC_HIP(hipMemsetAsync(F.recbad, 0, sizeof(int) * (size_t)ncomp, s));
         hipLaunchKernelGGL(k_phase2, dim3(cdiv(ncomp, 64)), dim3(64), 0, s, D, Q, real, hk, ties, &small[20], tiecap,
                            pool_used, poolcap, F);
         tick("phase2", &small[20]);
         hipLaunchKernelGGL(k_flood_lds<FL_BIG>, dim3(256), dim3(64), 0, s, D, Q, real, F, 1);
         hipLaunchKernelGGL(k_flood_lds<FL_SMALL>, dim3(1024), dim3(64), 0, s, D, Q, real, F, 0);
+        hipLaunchKernelGGL(k_tie_check, dim3(2048), dim3(256), 0, s, Q, F, W);
         hipLaunchKernelGGL(k_tie_commit, dim3(1024), dim3(256), 0, s, D, Q, F);
         tick("lds flood", &small[32]);
         hipLaunchKernelGGL(k_phase2_ties, dim3(tiecap), dim3(64), 0, s, D, Q, real, ties, &small[20], pool);

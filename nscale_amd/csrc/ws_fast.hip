@@ -140,7 +140,9 @@ __device__ bool before(const Dev& D, int x, int y) {
 }
 
 // ---- classification ----
-__global__ void k_classify(Dev D, uint8_t* cls, uint16_t* lvl0, uint8_t* fmask, int* nF) {
+// nF: this workgroup's count goes to partial[blockIdx.x] (summed by k_sum_partials): one atomic
+// per workgroup on a single counter would serialise over a grid this large.
+__global__ void k_classify(Dev D, uint8_t* cls, uint16_t* lvl0, uint8_t* fmask, int* partial) {
     const int W = D.W, H = D.H;
     int local = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < D.N; i += gridDim.x * blockDim.x) {
@@ -172,7 +174,21 @@ __global__ void k_classify(Dev D, uint8_t* cls, uint16_t* lvl0, uint8_t* fmask, 
         D.m[i] = frame ? WSHED : (v < 0 ? 0 : v);
         local += (c == C_F);
     }
-    block_count_add(local, nF);
+    const int bs = block_sum(local);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+
+// *out = sum of partial[0..n) (one workgroup of 1024 threads)
+__global__ __launch_bounds__(1024) void k_sum_partials(const int* __restrict__ partial, int n, int* out) {
+    int v = 0;
+    for (int i0 = threadIdx.x; i0 < n; i0 += 4 * blockDim.x) {  // four loads in flight per thread
+        int p[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) p[j] = i0 + j * (int)blockDim.x < n ? partial[i0 + j * blockDim.x] : 0;
+        v += p[0] + p[1] + p[2] + p[3];
+    }
+    const int t = block_sum(v);
+    if (threadIdx.x == 0) *out = t;
 }
 
 // ---- phase 1: one lane floods one run ----
@@ -1443,7 +1459,11 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     D.img = img; D.W = W; D.H = H; D.N = (int)N; D.m0 = m0; D.m = m; D.cls = cls; D.lvl0 = lvl0; D.run = run;
     D.st = st; D.ilvl = ilvl; D.pseq = pseq; D.parent = parent; D.bnode = bnode; D.popidx = popidx; D.lab = lab;
     D.head = head; D.nleaf = &small[2]; D.flags = &small[3];
-    hipLaunchKernelGGL(k_classify, grid_counted(N), dim3(256), 0, s, D, cls, lvl0, fmask, &small[0]);
+    {  // real[] is unused until B resolution: it holds the per-workgroup counts
+        const unsigned nb = (unsigned)std::min<size_t>(cdiv(N, 256), 1u << 14);
+        hipLaunchKernelGGL(k_classify, dim3(nb), dim3(256), 0, s, D, cls, lvl0, fmask, real);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, real, (int)nb, &small[0]);
+    }
     tick("classify", &small[0]);
     NSC_CHECK(ccl_neq(fmask, 0, run, W, H, 4, s));
     NSC_CHECK(relabel_impl(run, W, H, -1, &small[1], lab2, s));

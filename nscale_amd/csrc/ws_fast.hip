@@ -555,11 +555,13 @@ __device__ __forceinline__ void wave_append(bool take, int v, int* __restrict__ 
 
 // every used leaf slot joins the list of its B pixel (list order is irrelevant to its readers);
 // the first leaf of a pixel also puts the pixel on the list of B pixels with queue entries
+// (one append per workgroup and iteration: an atomic per wave on the single list counter
+// serialises over tens of thousands of waves)
 __global__ void k_link_leaves(Dev D, int* __restrict__ blist, int* __restrict__ bcount) {
     const int n = min(*D.nleaf, D.leafcap);
     const int stride = gridDim.x * blockDim.x;
-    const int n_up = ((n + 63) / 64) * 64;  // whole waves take part in the append
-    for (int sl = blockIdx.x * blockDim.x + threadIdx.x; sl < n_up; sl += stride) {
+    for (int k0 = blockIdx.x * blockDim.x; k0 < n; k0 += stride) {  // block-uniform trip count
+        const int sl = k0 + threadIdx.x;
         bool first = false;
         int b = -1;
         if (sl < n) {
@@ -570,7 +572,7 @@ __global__ void k_link_leaves(Dev D, int* __restrict__ blist, int* __restrict__ 
                 first = old < 0;
             }
         }
-        wave_append(first, b, blist, bcount);
+        block_append(first, b, blist, bcount);
     }
 }
 

@@ -1341,9 +1341,10 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
     };
     launch(0, nullptr, nullptr, listA, &counts[0]);
     int p = 1;
-    // eight passes before the first host check, doubling up to 32: the front needs tens of passes
-    // to cross a tile, and an empty pass costs a few microseconds against a host round trip of tens
-    for (int batch = 8;; batch = std::min(2 * batch, 32)) {
+    // host checks after 8, 24 and 56 passes, then every 8: the front needs tens of passes to cross
+    // a tile (about 60 on a 4096^2 nuclei tile), an empty pass costs a few microseconds and a host
+    // round trip a few tens
+    for (int batch = 8;; batch = p > 56 ? 8 : 2 * batch) {
         for (int b = 0; b < batch && p <= max_pass; ++b, ++p) {
             int* cur = (p & 1) ? listA : listB;
             int* nxt = (p & 1) ? listB : listA;

@@ -148,9 +148,9 @@ inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 // Independent launches (e.g. long-latency work on a few runs beside many short ones) can overlap
 // on per-thread side streams (default priority); join() makes the caller's stream wait for them.
 struct SideStream {
-    static constexpr int K = 2;
-    hipStream_t side[K] = {nullptr, nullptr};
-    hipEvent_t forked = nullptr, joined[K] = {nullptr, nullptr};
+    static constexpr int K = 3;
+    hipStream_t side[K] = {nullptr, nullptr, nullptr};
+    hipEvent_t forked = nullptr, joined[K] = {nullptr, nullptr, nullptr};
     int fork(hipStream_t s) {
         if (!forked) {
             NSC_HIP(hipEventCreateWithFlags(&forked, hipEventDisableTiming));

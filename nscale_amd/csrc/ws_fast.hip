@@ -1520,7 +1520,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(std::min(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(4), bcount + 4);
         hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(5), bcount + 5, qh2, qt);
         hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(std::min(nrun, 256)), dim3(64), 0, ss.side[1], D, box5, nrun, L(3), bcount + 3);
-        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(std::min(nrun, 768)), dim3(64), 0, ss.side[1], D, box5, nrun, L(2), bcount + 2);
+        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(std::min(nrun, 768)), dim3(64), 0, ss.side[2], D, box5, nrun, L(2), bcount + 2);
         hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(std::min(nrun, 1536)), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1);
         hipLaunchKernelGGL(k_phase1_lds<512>, dim3(std::min(nrun, 3072)), dim3(64), 0, s, D, box5, nrun, L(0), bcount);
         NSC_CHECK(ss.join(s));

@@ -513,56 +513,93 @@ __global__ void k_and_nonzero(const uint8_t* a, const uint8_t* b, uint8_t* out, 
 }
 
 // ---- canonical relabel: rank of each root in raster order ----
-constexpr int SCAN_B = 1024;
-__global__ void k_root_block_count(const int32_t* __restrict__ L, size_t n, int32_t* __restrict__ bsum) {
-    __shared__ int s[SCAN_B / 64];
-    size_t i = (size_t)blockIdx.x * SCAN_B + threadIdx.x;
-    int c = (i < n && L[i] == (int32_t)i) ? 1 : 0;
-    unsigned long long bal = __ballot(c);
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) s[wid] = __popcll(bal);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int k = 0; k < SCAN_B / 64; ++k) t += s[k];
-        bsum[blockIdx.x] = t;
+// A workgroup of 1024 threads covers SCAN_B = 4096 consecutive pixels, four per thread (contiguous,
+// so a thread's roots are in raster order after those of the threads before it); few workgroups
+// keep the scan of their counts short.
+constexpr int SCAN_B = 4096;
+
+// exclusive prefix sum of v over the workgroup (at most 16 waves); *total = the sum of all v
+__device__ __forceinline__ int block_scan_excl(int v, int* total) {
+    __shared__ int wsum[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    int x = v;  // inclusive scan within the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
     }
+    __syncthreads();  // wsum may still be read by a previous call
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int k = 0; k < nw; ++k) {
+        const int t = wsum[k];
+        if (k < wv) before += t;
+        tot += t;
+    }
+    if (total) *total = tot;
+    return before + x - v;
 }
-__global__ void k_scan_small(int32_t* bsum, int nb, int* total) {
-    // single block exclusive scan of nb block sums
-    __shared__ int part[1024];
-    int per = (nb + 1023) / 1024;
-    int start = threadIdx.x * per, acc = 0;
-    for (int k = 0; k < per && start + k < nb; ++k) acc += bsum[start + k];
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int k = 0; k < 1024; ++k) { int v = part[k]; part[k] = run; run += v; }
-        if (total) *total = run;
+
+__global__ __launch_bounds__(1024) void k_root_block_count(const int32_t* __restrict__ L, size_t n, int32_t* __restrict__ bsum) {
+    const size_t i0 = (size_t)blockIdx.x * SCAN_B + 4 * threadIdx.x;
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c += (i0 + j < n && L[i0 + j] == (int32_t)(i0 + j)) ? 1 : 0;
+    const int t = block_sum(c);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = t;
+}
+// exclusive scan of nb block sums in place (one workgroup of 1024 threads, a contiguous chunk each)
+__global__ __launch_bounds__(1024) void k_scan_small(int32_t* bsum, int nb, int* total) {
+    const int per = (nb + 1023) / 1024, start = threadIdx.x * per;
+    const int end = min(start + per, nb);
+    int acc = 0;
+    for (int k = start; k < end; k += 4) {  // four loads in flight
+        int v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = k + j < end ? bsum[k + j] : 0;
+        acc += v[0] + v[1] + v[2] + v[3];
     }
-    __syncthreads();
-    int run = part[threadIdx.x];
-    for (int k = 0; k < per && start + k < nb; ++k) { int v = bsum[start + k]; bsum[start + k] = run; run += v; }
+    int tot = 0;
+    int run = block_scan_excl(acc, &tot);
+    if (threadIdx.x == 0 && total) *total = tot;
+    for (int k = start; k < end; k += 4) {
+        int v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = k + j < end ? bsum[k + j] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (k + j < end) bsum[k + j] = run;
+            run += v[j];
+        }
+    }
 }
 // rank[i] (stored in place of area scratch) for roots: 1 + number of roots before i
-__global__ void k_root_rank(const int32_t* __restrict__ L, size_t n, const int32_t* __restrict__ boff, int32_t* __restrict__ rank) {
-    __shared__ int s[SCAN_B / 64];
-    size_t i = (size_t)blockIdx.x * SCAN_B + threadIdx.x;
-    int c = (i < n && L[i] == (int32_t)i) ? 1 : 0;
-    unsigned long long bal = __ballot(c);
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) s[wid] = __popcll(bal);
-    __syncthreads();
-    int before = boff[blockIdx.x];
-    for (int k = 0; k < wid; ++k) before += s[k];
-    before += __popcll(bal & ((1ull << lane) - 1ull));
-    if (c) rank[i] = before + 1;
+__global__ __launch_bounds__(1024) void k_root_rank(const int32_t* __restrict__ L, size_t n, const int32_t* __restrict__ boff,
+                                                   int32_t* __restrict__ rank) {
+    const size_t i0 = (size_t)blockIdx.x * SCAN_B + 4 * threadIdx.x;
+    bool root[4];
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        root[j] = i0 + j < n && L[i0 + j] == (int32_t)(i0 + j);
+        c += root[j] ? 1 : 0;
+    }
+    int k = boff[blockIdx.x] + block_scan_excl(c, nullptr);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (root[j]) rank[i0 + j] = ++k;
 }
 __global__ void k_apply_rank(int32_t* L, const int32_t* __restrict__ rank, size_t n, int32_t bgval) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        int r = L[i];
-        L[i] = (r == bgval || r < 0) ? 0 : rank[r];
+    NSC_FOR4(n) {
+        int r[4], v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[j] = i0 + j * S < n ? L[i0 + j * S] : -1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = (r[j] == bgval || r[j] < 0) ? 0 : rank[r[j]];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (i0 + j * S < n) L[i0 + j * S] = v[j];
     }
 }
 
@@ -683,9 +720,9 @@ int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int3
     Scratch sc;
     NSC_CHECK(sc.alloc(sizeof(int32_t) * (nb + 1), s));
     int32_t* bsum = sc.as<int32_t>();
-    hipLaunchKernelGGL(k_root_block_count, dim3(nb), dim3(SCAN_B), 0, s, labels, n, bsum);
+    hipLaunchKernelGGL(k_root_block_count, dim3(nb), dim3(SCAN_B / 4), 0, s, labels, n, bsum);
     hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, bsum, nb, dcount);
-    hipLaunchKernelGGL(k_root_rank, dim3(nb), dim3(SCAN_B), 0, s, labels, n, bsum, rank);
+    hipLaunchKernelGGL(k_root_rank, dim3(nb), dim3(SCAN_B / 4), 0, s, labels, n, bsum, rank);
     hipLaunchKernelGGL(k_apply_rank, grid1(n), dim3(256), 0, s, labels, rank, n, bgval);
     NSC_HIP(hipGetLastError());
     return NSC_OK;

@@ -613,43 +613,37 @@ __global__ void k_bbox_init(int32_t* out5, int32_t* area, int n) {
     out5[4 * n + k] = -1;
     if (area) area[k] = 0;
 }
-// Each thread scans 16 consecutive pixels and issues the atomics once per run of equal labels
-// within a row (a run's x extent is its first and last pixel), instead of once per pixel.
-constexpr int BB_RUN = 16;
+// A wave takes 64 consecutive pixels (one coalesced load) and issues the atomics once per run of
+// equal labels in its window; a row start also starts a run (a run's x extent is its first and
+// last pixel).  Labels outside 1..n are ignored.
 __global__ void k_bbox(const int32_t* __restrict__ L, int w, int h, int n, int32_t* out5, int32_t* area) {
     const size_t total = (size_t)w * h;
-    const size_t chunks = (total + BB_RUN - 1) / BB_RUN;
-    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks; c += (size_t)gridDim.x * blockDim.x) {
-        const size_t i0 = c * BB_RUN, i1 = std::min(total, i0 + BB_RUN);
-        int x = (int)(i0 % w), y = (int)(i0 / w);
-        int cur = 0, x0 = 0, x1 = 0, cnt = 0;
-        auto flush = [&]() {
-            if (cur <= 0 || cur > n) return;
-            const int k = cur - 1;
-            atomicMin(&out5[n + k], x0);
-            atomicMax(&out5[2 * n + k], x1);
+    const int lane = threadIdx.x & 63;
+    const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const size_t nwave = ((size_t)gridDim.x * blockDim.x) >> 6;
+    for (size_t base = wave * 64; base < total; base += nwave * 64) {
+        const size_t i = base + lane;
+        const int r = i < total ? L[i] : -1;
+        // position: the window's first pixel once per wave, then at most a few row wraps
+        int y = (int)(base / w), x = (int)(base - (size_t)y * w) + lane;
+        while (x >= w) {
+            x -= w;
+            ++y;
+        }
+        const int prev = __shfl_up(r, 1);
+        const bool head = (lane == 0 || prev != r || x == 0) && i < total;
+        const unsigned long long heads = __ballot(head);
+        if (head && r >= 1 && r <= n) {
+            const unsigned long long later = lane == 63 ? 0ull : (heads >> (lane + 1));
+            int end = later ? lane + __ffsll((long long)later) : 64;  // first lane after the run
+            end = (int)std::min<size_t>((size_t)end, total - base);
+            const int k = r - 1, len = end - lane;
+            atomicMin(&out5[n + k], x);
+            atomicMax(&out5[2 * n + k], x + len - 1);
             atomicMin(&out5[3 * n + k], y);
             atomicMax(&out5[4 * n + k], y);
-            if (area) atomicAdd(&area[k], cnt);
-        };
-        for (size_t i = i0; i < i1; ++i) {
-            const int l = L[i];
-            if (l != cur) {
-                flush();
-                cur = l;
-                x0 = x;
-                cnt = 0;
-            }
-            x1 = x;
-            ++cnt;
-            if (++x == w) {  // row ends: the run ends with it
-                flush();
-                cur = 0;
-                x = 0;
-                ++y;
-            }
+            if (area) atomicAdd(&area[k], len);
         }
-        flush();
     }
 }
 
@@ -731,7 +725,7 @@ int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int3
 int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s) {
     if (n <= 0) return NSC_OK;
     hipLaunchKernelGGL(k_bbox_init, dim3(cdiv(n, 256)), dim3(256), 0, s, out5, area, n);
-    hipLaunchKernelGGL(k_bbox, grid1(cdiv((size_t)w * h, BB_RUN)), dim3(256), 0, s, canon, w, h, n, out5, area);
+    hipLaunchKernelGGL(k_bbox, grid1((size_t)w * h), dim3(256), 0, s, canon, w, h, n, out5, area);
     NSC_HIP(hipGetLastError());
     return NSC_OK;
 }

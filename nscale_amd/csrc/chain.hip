@@ -146,16 +146,21 @@ __global__ void k_norbc(uint8_t* __restrict__ seg, const uint8_t* __restrict__ r
 __global__ void k_dist_prep(const float* __restrict__ dist, const uint8_t* __restrict__ seg_big,
                             const uint8_t* __restrict__ bgr, size_t n, float* __restrict__ distance,
                             float* __restrict__ hmask, float* __restrict__ hmarker, uint8_t* __restrict__ nuclei) {
+    // every load unconditional, then selects (a load under the mask's condition compiles to a
+    // branch around it, and the loads of a pixel then wait for one another)
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const bool f = seg_big[i] != 0;
-        const float d = f ? -dist[i] : -FLT_MAX;
+        const uint8_t sb = seg_big[i];
+        const float dv = dist[i];
+        const uint8_t c0 = bgr[3 * i], c1 = bgr[3 * i + 1], c2 = bgr[3 * i + 2];
+        const bool f = sb != 0;
+        const float d = f ? -dv : -FLT_MAX;
         if (distance) distance[i] = d;
         const float m = -d;
         hmask[i] = m;
         hmarker[i] = m - 1.0f;
-        nuclei[3 * i] = f ? bgr[3 * i] : 0;
-        nuclei[3 * i + 1] = f ? bgr[3 * i + 1] : 0;
-        nuclei[3 * i + 2] = f ? bgr[3 * i + 2] : 0;
+        nuclei[3 * i] = f ? c0 : 0;
+        nuclei[3 * i + 1] = f ? c1 : 0;
+        nuclei[3 * i + 2] = f ? c2 : 0;
     }
 }
 

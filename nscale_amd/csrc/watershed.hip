@@ -132,10 +132,10 @@ __global__ void k_ws_border_crop(const int32_t* __restrict__ P, int32_t* __restr
         const int x = (int)(i % w) + 1, y = (int)(i / w) + 1;
         const int32_t* r = P + (size_t)y * W;
         const int32_t* u = r - W;
-        const int32_t p = r[x];
-        int32_t o = p;
-        if (u[x] == -1 && p != -1 && ((u[x - 1] != -1 && r[x - 1] == -1) || (u[x + 1] != -1 && r[x + 1] == -1))) o = -1;
-        out[i] = o;
+        // all six loads first: a short-circuit condition would put them behind branches
+        const int32_t p = r[x], rl = r[x - 1], rr = r[x + 1], uc = u[x], ul = u[x - 1], ur = u[x + 1];
+        const bool dam = (uc == -1) & (p != -1) & (((ul != -1) & (rl == -1)) | ((ur != -1) & (rr == -1)));
+        out[i] = dam ? -1 : p;
     }
 }
 
@@ -147,11 +147,14 @@ __global__ void k_ws_pad(const int32_t* __restrict__ lab, const uint8_t* __restr
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % W) - 1, y = (int)(i / W) - 1;
         const bool in = x >= 0 && y >= 0 && x < w && y < h;
-        const size_t j = (size_t)y * w + x;
-        P[i] = in ? lab[j] : 0;
-        Pimg[3 * i] = in ? bgr[3 * j] : 0;
-        Pimg[3 * i + 1] = in ? bgr[3 * j + 1] : 0;
-        Pimg[3 * i + 2] = in ? bgr[3 * j + 2] : 0;
+        // loads from a clamped (always valid) pixel, then selects: no branch around the loads
+        const size_t j = (size_t)clampi(y, 0, h - 1) * w + clampi(x, 0, w - 1);
+        const int32_t lv = lab[j];
+        const uint8_t c0 = bgr[3 * j], c1 = bgr[3 * j + 1], c2 = bgr[3 * j + 2];
+        P[i] = in ? lv : 0;
+        Pimg[3 * i] = in ? c0 : 0;
+        Pimg[3 * i + 1] = in ? c1 : 0;
+        Pimg[3 * i + 2] = in ? c2 : 0;
     }
 }
 

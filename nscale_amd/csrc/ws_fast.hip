@@ -802,10 +802,14 @@ __global__ void k_bfs_commit(Dev D, const int* __restrict__ info, const unsigned
                              uint8_t* __restrict__ umask) {
     const int b1 = info[0], Y1 = info[2];
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < D.N; p += gridDim.x * blockDim.x) {
-        int v = D.m[p];
-        const uint8_t c = D.cls[p];
-        if (b1 >= 0 && Y1 > 0 && c == C_B && v == 0 && dist[p] != 0xFFFFFFFFu) { v = Y1; D.m[p] = v; }
-        if (kind[p] == 2) { v = IN_QUEUE; D.m[p] = v; }
+        // the four loads first (under the conditions they would wait for one another)
+        const int m0 = D.m[p];
+        const uint8_t c = D.cls[p], kd = kind[p];
+        const unsigned dp = dist[p];
+        int v = m0;
+        if (b1 >= 0 && Y1 > 0 && c == C_B && v == 0 && dp != 0xFFFFFFFFu) v = Y1;
+        if (kd == 2) v = IN_QUEUE;
+        if (v != m0) D.m[p] = v;
         umask[p] = (c != C_FRAME && c != C_SEED && (v == 0 || v == IN_QUEUE)) ? 1 : 0;
     }
 }

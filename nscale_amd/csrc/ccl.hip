@@ -508,8 +508,10 @@ __global__ void k_bwselect_out(const int32_t* __restrict__ L, const uint8_t* __r
 }
 
 __global__ void k_and_nonzero(const uint8_t* a, const uint8_t* b, uint8_t* out, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        out[i] = (a[i] != 0 && b[i] != 0) ? 255 : 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint8_t av = a[i], bv = b[i];  // both loads unconditional (no branch around the second)
+        out[i] = (av != 0 && bv != 0) ? 255 : 0;
+    }
 }
 
 // ---- canonical relabel: rank of each root in raster order ----

@@ -171,8 +171,11 @@ __global__ void k_negate_f(float* a, size_t n) {
 // seg_nonoverlap = seg_big where watermask >= 0  (HE.cpp:423-424)
 __global__ void k_nonoverlap(const uint8_t* __restrict__ seg_big, const int32_t* __restrict__ wm, uint8_t* __restrict__ out,
                              size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        out[i] = wm[i] >= 0 ? seg_big[i] : 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int32_t m = wm[i];
+        const uint8_t sb = seg_big[i];  // unconditional: no branch around the load
+        out[i] = m >= 0 ? sb : 0;
+    }
 }
 
 }  // namespace

@@ -744,18 +744,20 @@ __global__ void k_t1_state(Dev D, const int32_t* __restrict__ real, const int* _
     const int b1 = info[0], b1i = info[1], Y1 = info[2];
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < D.N; p += gridDim.x * blockDim.x) {
         uint8_t pm = 0, kd = 0;
-        const uint8_t c = D.cls[p];
+        // the pixel's class, state, label, pusher and queue entry are loaded together (each one
+        // behind the class test would be another dependent round trip)
+        const uint8_t c = D.cls[p], stp = D.st[p];
+        const int labp = D.lab[p], parp = D.parent[p], r = real[p];
         if (c == C_F) {
             int v = 0;
-            if (D.st[p] == 2 && (b1 < 0 || before(D, p, b1i))) v = D.lab[p];
-            else if (D.st[p] >= 1 && (D.parent[p] < 0 || b1 < 0 || before(D, D.parent[p], b1i))) { v = IN_QUEUE; kd = 1; }
+            if (stp == 2 && (b1 < 0 || before(D, p, b1i))) v = labp;
+            else if (stp >= 1 && (parp < 0 || b1 < 0 || before(D, parp, b1i))) { v = IN_QUEUE; kd = 1; }
             D.m[p] = v;
         } else if (c == C_B) {
-            const int r = real[p];
             int v = 0;
             if (p == b1) v = Y1;
             else if (r >= 0) {
-                const int par = it_parent(D, r);
+                const int par = D.wr[r].w;  // the queue entry's pusher, from its packed record
                 const bool queued = par < 0 || b1 < 0 || before(D, par, b1i);
                 if (queued) {
                     if (b1 < 0 || before(D, r, b1i)) v = WSHED;

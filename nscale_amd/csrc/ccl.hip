@@ -241,7 +241,9 @@ __global__ __launch_bounds__(256) void ccl_local(Px P, int32_t* __restrict__ L, 
 template <int MODE, int CONN>
 __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h, int tiles_x, int ntiles) {
     constexpr int ND = CONN == 8 ? 4 : 2;
-    const int per_tile = CT_W + 2 * CT_H;
+    // with 4-connectivity a last-column pixel has no backward neighbour in another tile (left and
+    // up are in its own tile; the tile's top row covers the corner), so only 8 needs that column
+    const int per_tile = CONN == 8 ? CT_W + 2 * CT_H : CT_W + CT_H;
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= ntiles * per_tile) return;
     const int t = gid / per_tile, e = gid % per_tile;
@@ -331,14 +333,14 @@ template <int MODE>
 int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipStream_t s) {
     const int tiles_x = (int)cdiv(w, CT_W), tiles_y = (int)cdiv(h, CT_H);
     const int nt = tiles_x * tiles_y;
-    const int nseam = nt * (CT_W + 2 * CT_H);
+    const int nseam8 = nt * (CT_W + 2 * CT_H), nseam4 = nt * (CT_W + CT_H);
     const size_t n = (size_t)w * h;
     if (conn == 8) {
         hipLaunchKernelGGL((ccl_local<MODE, 8>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x);
-        hipLaunchKernelGGL((ccl_seams<MODE, 8>), dim3(cdiv(nseam, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
+        hipLaunchKernelGGL((ccl_seams<MODE, 8>), dim3(cdiv(nseam8, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
     } else {
         hipLaunchKernelGGL((ccl_local<MODE, 4>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x);
-        hipLaunchKernelGGL((ccl_seams<MODE, 4>), dim3(cdiv(nseam, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
+        hipLaunchKernelGGL((ccl_seams<MODE, 4>), dim3(cdiv(nseam4, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
     }
     hipLaunchKernelGGL(ccl_flatten, dim3(std::min<unsigned>(cdiv(n, 1024), 65535u)), dim3(256), 0, s, L, n, bgval);
     NSC_HIP(hipGetLastError());

@@ -36,17 +36,6 @@ struct Px {
     uint8_t key;
 };
 
-template <int MODE>
-__device__ __forceinline__ bool fg_of(const Px& P, size_t i) {
-    if (MODE == FG_NEQ) return P.img[i] != P.key;
-    return P.img[i] != 0;
-}
-template <int MODE>
-__device__ __forceinline__ bool joins(const Px& P, size_t i, size_t j) {
-    if (MODE == FG_NEQ) return true;  // both known foreground
-    if (MODE == FG_EQ) return P.img[i] == P.img[j];
-    return P.fv[i] == P.fv[j];
-}
 
 // find with intermediate pointer jumping (each visited node is re-pointed to its grandparent).
 // Parents are always smaller than children, and a non-root never becomes a root again, so
@@ -274,6 +263,23 @@ __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h,
         lj[d] = ld_wg(&L[j[d]]);
         ljp[d] = ld_wg(&L[jp[d]]);
     }
+    // the values the equal-value joins compare, loaded with the labels (inside the checks below
+    // each would be one more dependent round trip)
+    using V = typename std::conditional<MODE == FG_FEQ, float, uint8_t>::type;
+    V vi = 0, vp = 0, vj[ND], vjp[ND];
+    if (MODE != FG_NEQ) {
+        auto val = [&](size_t q) -> V {
+            if constexpr (MODE == FG_FEQ) return P.fv[q];
+            else return P.img[q];
+        };
+        vi = val(i);
+        vp = val(ip);
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            vj[d] = val(j[d]);
+            vjp[d] = val(jp[d]);
+        }
+    }
     if (li < 0) return;
     auto cross = [&](int qx, int qy) { return (qx / CT_W != tx) || (qy / CT_H != ty); };
     auto valid = [&](int qx, int qy) { return qx >= 0 && qy >= 0 && qx < w && qy < h && cross(qx, qy); };
@@ -285,10 +291,10 @@ __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h,
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         const int qx = x + dxs[d], qy = y + dys[d];
-        if (!valid(qx, qy) || lj[d] < 0 || !joins<MODE>(P, i, j[d])) continue;
+        if (!valid(qx, qy) || lj[d] < 0 || (MODE != FG_NEQ && !(vi == vj[d]))) continue;
         if (has_prev && lprev == li) {
             const int rx = px + dxs[d], ry = py + dys[d];
-            if (valid(rx, ry) && ljp[d] == lj[d] && joins<MODE>(P, ip, jp[d])) continue;  // same union
+            if (valid(rx, ry) && ljp[d] == lj[d] && (MODE == FG_NEQ || vp == vjp[d])) continue;  // same union
         }
         gunite(L, li, lj[d]);  // labels are ancestors of their pixels
     }

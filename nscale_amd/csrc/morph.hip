@@ -10,7 +10,8 @@
 //      its result by the extra taps only (disk19: 19 taps per pixel for all 5 distinct runs);
 //   3. the output is the min/max over kernel rows of the run result of the matching window row.
 // Traffic: 1 byte in + 1 byte out per pixel (plus the halo re-read, r/32 of the tile).
-// 3 x 3 elements take a direct kernel instead (morph3_kernel).
+// The window and run results are held as 16-bit pixel pairs where that fits in LDS (morph2_kernel),
+// else as bytes (morph_kernel).  3 x 3 elements take a direct kernel instead (morph3_kernel).
 #include <algorithm>
 #include <cstdint>
 #include <vector>
@@ -102,6 +103,117 @@ __global__ __launch_bounds__(256) void morph_kernel(const uint8_t* __restrict__ 
             v = pick<ERODE>(v, H[P.row_run[t] * WW * MT + (y + i) * MT + x]);
         }
         out[(size_t)gy * w + gx] = (uint8_t)v;
+    }
+}
+
+// Two pixels per instruction: the same three steps with the window and the run results held as
+// 16-bit values, so one aligned 32-bit LDS word is a pair of neighbouring pixels and one packed
+// u16 min/max serves both.  A tap at an odd offset takes the pair straddling two aligned words
+// (funnel shift of the words read for the neighbouring even taps), so a run of n taps costs
+// about n/2 LDS reads and n packed ops per two outputs.  Used where its LDS (twice the byte
+// kernel's) fits.
+__device__ __forceinline__ uint32_t lds_u32(const uint16_t* p) { return *reinterpret_cast<const uint32_t*>(p); }
+
+template <bool ERODE>
+__device__ __forceinline__ uint32_t pick2(uint32_t a, uint32_t b) {
+    typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+    const u16x2 va = __builtin_bit_cast(u16x2, a), vb = __builtin_bit_cast(u16x2, b);
+    return __builtin_bit_cast(uint32_t, ERODE ? __builtin_elementwise_min(va, vb) : __builtin_elementwise_max(va, vb));
+}
+
+// taps q0..q1 (pair offsets from an even pair base `p`, q0 >= 0) folded into `acc`; reads the
+// aligned words at even offsets q0 & ~1 .. (q1 + 1) & ~1 only
+template <bool ERODE>
+__device__ __forceinline__ uint32_t taps2(uint32_t acc, const uint16_t* p, int q0, int q1) {
+    int q = q0 & ~1;
+    uint32_t cur = lds_u32(p + q);
+    for (; q <= q1; q += 2) {
+        if (q >= q0) acc = pick2<ERODE>(acc, cur);          // pixels q, q+1
+        if (q + 1 <= q1) {
+            const uint32_t nxt = lds_u32(p + q + 2);
+            acc = pick2<ERODE>(acc, (cur >> 16) | (nxt << 16));  // pixels q+1, q+2
+            cur = nxt;
+        }
+    }
+    return acc;
+}
+
+constexpr int M2_THREADS = 512;
+
+template <bool ERODE>
+__global__ __launch_bounds__(M2_THREADS) void morph2_kernel(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, int w,
+                                                            int h, MorphPlan P, uint8_t outside, int tiles_x, bool words) {
+    extern __shared__ uint16_t sm16[];
+    const int r = P.r;
+    const int WW = MT + 2 * r;  // window side (even, so rows start on 4-byte boundaries)
+    const int WS = WW;          // row stride in pixels; the last pair read is (WW-2, WW-1)
+    uint16_t* win = sm16;       // WW x WS
+    uint16_t* H = sm16 + WW * WS;  // nrun x WW x MT
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int x0 = tx * MT - r, y0 = ty * MT - r;
+    const auto gin = as_global(in);
+    const int WN = WW * WW;
+    for (int k0 = 0; k0 < WN; k0 += 8 * M2_THREADS) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = min(k0 + (int)threadIdx.x + M2_THREADS * j, WN - 1);
+            const int lx = k % WW, ly = k / WW;
+            v[j] = gin[(size_t)clampi(y0 + ly, 0, h - 1) * w + clampi(x0 + lx, 0, w - 1)];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = k0 + (int)threadIdx.x + M2_THREADS * j;
+            if (k >= WN) continue;
+            const int lx = k % WW, ly = k / WW;
+            const int gx = x0 + lx, gy = y0 + ly;
+            const bool in_img = gx >= 0 && gx < w && gy >= 0 && gy < h;
+            win[ly * WS + lx] = (uint16_t)(in_img ? v[j] : (uint32_t)outside);
+        }
+    }
+    __syncthreads();
+    const uint32_t ident = ERODE ? 0x00FF00FFu : 0u;
+    constexpr int NP = MT / 2;  // output pairs per row
+    for (int d = 0; d < P.nrun; ++d) {
+        const int a = P.ra[d], b = P.rb[d], bs = P.base[d];
+        uint16_t* Hd = H + d * WW * MT;
+        for (int k = threadIdx.x; k < WW * NP; k += M2_THREADS) {
+            const int x = 2 * (k % NP), row = k / NP;
+            const uint16_t* src = win + row * WS + x;  // pair offset q = r + tap
+            uint32_t v;
+            if (bs >= 0) {
+                v = lds_u32(H + (bs * WW + row) * MT + x);
+                if (a < P.ra[bs]) v = taps2<ERODE>(v, src, r + a, r + P.ra[bs] - 1);
+                if (b > P.rb[bs]) v = taps2<ERODE>(v, src, r + P.rb[bs] + 1, r + b);
+            } else {
+                v = taps2<ERODE>(ident, src, r + a, r + b);
+            }
+            *reinterpret_cast<uint32_t*>(Hd + row * MT + x) = v;
+        }
+        __syncthreads();
+    }
+    // output: four pixels per thread (two pairs), one 8-byte LDS read per (kernel row, run)
+    constexpr int NQ = MT / 4;
+    for (int k = threadIdx.x; k < MT * NQ; k += M2_THREADS) {
+        const int x = 4 * (k % NQ), y = k / NQ;
+        uint32_t v0 = ident, v1 = ident;
+        for (int t = 0; t < P.nrow; ++t) {
+            const uint2 hv = *reinterpret_cast<const uint2*>(H + (P.row_run[t] * WW + y + P.row_i[t]) * MT + x);
+            v0 = pick2<ERODE>(v0, hv.x);
+            v1 = pick2<ERODE>(v1, hv.y);
+        }
+        const int gx = tx * MT + x, gy = ty * MT + y;
+        if (gy >= h || gx >= w) continue;
+        const uint32_t o = (v0 & 0xFFu) | ((v0 >> 8) & 0xFF00u) | ((v1 & 0xFFu) << 16) | ((v1 >> 16) << 24);
+        uint8_t* orow = out + (size_t)gy * w;
+        if (words && gx + 3 < w) {
+            *reinterpret_cast<uint32_t*>(orow + gx) = o;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (gx + j < w) orow[gx + j] = (uint8_t)(o >> (8 * j));
+        }
     }
 }
 
@@ -237,6 +349,24 @@ int morph_impl(const uint8_t* in, uint8_t* out, int w, int h, const uint8_t* ker
             hipLaunchKernelGGL((morph3_kernel<false>), grid, dim3(256), 0, s, in, out, w, h, mask, (uint8_t)outside, words);
         NSC_HIP(hipGetLastError());
         return NSC_OK;
+    }
+    const size_t lds2 = 2 * ((size_t)WW * WW + (size_t)P.nrun * WW * MT);
+    if (lds2 <= 160 * 1024) {
+        const bool words = (w & 3) == 0 && (reinterpret_cast<uintptr_t>(out) & 3) == 0;
+        const void* fn = erode ? (const void*)morph2_kernel<true> : (const void*)morph2_kernel<false>;
+        if (lds2 > 64 * 1024) NSC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+        if (erode)
+            hipLaunchKernelGGL((morph2_kernel<true>), dim3(tiles_x * tiles_y), dim3(M2_THREADS), lds2, s, in, out, w, h, P,
+                               (uint8_t)outside, tiles_x, words);
+        else
+            hipLaunchKernelGGL((morph2_kernel<false>), dim3(tiles_x * tiles_y), dim3(M2_THREADS), lds2, s, in, out, w, h, P,
+                               (uint8_t)outside, tiles_x, words);
+        NSC_HIP(hipGetLastError());
+        return NSC_OK;
+    }
+    if (lds > 64 * 1024) {
+        const void* fn = erode ? (const void*)morph_kernel<true> : (const void*)morph_kernel<false>;
+        NSC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     if (erode)
         hipLaunchKernelGGL((morph_kernel<true>), dim3(tiles_x * tiles_y), dim3(256), lds, s, in, out, w, h, P,

@@ -103,7 +103,7 @@ def test_imfill_matlab_doc_example():
     assert np.array_equal(host(G().imfill(dev(d["bw1"]), dev(seeds), 8)), d["bw2"])
 
 
-@pytest.mark.parametrize("kernel", ["disk19", "cross3", "rect5", "holes7", "rand3"])
+@pytest.mark.parametrize("kernel", ["disk19", "cross3", "rect5", "holes7", "rand3", "disk31", "holes11"])
 @pytest.mark.parametrize("shape", [(1, 1), (5, 300), (130, 129), (300, 257), (67, 256)])
 def test_morphology(kernel, shape):
     rng = np.random.default_rng(sum(shape))
@@ -114,6 +114,11 @@ def test_morphology(kernel, shape):
         k[3, 3] = 1
     if kernel == "rand3":  # 3 x 3 without its centre: the direct 3 x 3 kernel, any taps
         k = np.array([[1, 0, 1], [0, 0, 1], [1, 1, 0]], np.uint8)
+    if kernel == "disk31":  # the largest element: 16-bit tables above 64 KiB of LDS
+        yy, xx = np.mgrid[-15:16, -15:16]
+        k = (yy * yy + xx * xx <= 15.5 ** 2).astype(np.uint8)
+    if kernel == "holes11":  # many runs: too large for the 16-bit tables, byte tables above 64 KiB
+        k = (np.random.default_rng(11).random((11, 11)) < 0.75).astype(np.uint8)
     print("morph", kernel, shape, flush=True)
     for outside in (0, 255, 40):
         assert np.array_equal(host(G().erode(dev(img), k, outside)), o_morph("erode", img, k, outside))

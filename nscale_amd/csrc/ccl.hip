@@ -623,35 +623,76 @@ __global__ void k_bbox_init(int32_t* out5, int32_t* area, int n) {
     out5[4 * n + k] = -1;
     if (area) area[k] = 0;
 }
-// A wave takes 64 consecutive pixels (one coalesced load) and issues the atomics once per run of
-// equal labels in its window; a row start also starts a run (a run's x extent is its first and
-// last pixel).  Labels outside 1..n are ignored.
+// A wave takes 64 consecutive pixels (coalesced loads of the window and of the rows above and
+// below it) and visits each run of equal labels in its window once; a row start also starts a
+// run.  A run issues an extent atomic only where no pixel of its component is known to beat it:
+//   y0: no pixel of the run has its label directly above;      y1: likewise below;
+//   x0: the pixels above its first pixel, and left of it in the rows y-1, y and y+1, lack its label;
+//   x1: likewise above its last pixel, and right of it in the rows y-1, y and y+1.
+// Every skip names a pixel of the component at a smaller row (y0), larger row (y1), smaller
+// column or the same column one row up (x0), larger column or the same column one row up (x1).
+// The extreme row's runs have no such pixel, and the topmost pixel of the extreme column has none
+// either, so each extreme is still issued; the skips drop the ~4 atomics per row of a component to
+// the few rows that hold an extreme.  Area stays one atomic per run.  Labels outside 1..n are ignored.
 __global__ void k_bbox(const int32_t* __restrict__ L, int w, int h, int n, int32_t* out5, int32_t* area) {
     const size_t total = (size_t)w * h;
     const int lane = threadIdx.x & 63;
     const size_t wave = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const size_t nwave = ((size_t)gridDim.x * blockDim.x) >> 6;
+    constexpr int NONE = INT_MIN;  // never a label in 1..n
     for (size_t base = wave * 64; base < total; base += nwave * 64) {
         const size_t i = base + lane;
-        const int r = i < total ? L[i] : -1;
+        const bool valid = i < total;
         // position: the window's first pixel once per wave, then at most a few row wraps
         int y = (int)(base / w), x = (int)(base - (size_t)y * w) + lane;
         while (x >= w) {
             x -= w;
             ++y;
         }
-        const int prev = __shfl_up(r, 1);
-        const bool head = (lane == 0 || prev != r || x == 0) && i < total;
+        // the pixel and its neighbours above and below: clamped (always valid) addresses, all
+        // three loads issued together, out-of-image values replaced afterwards
+        const size_t ic = valid ? i : total - 1;
+        const size_t iu = (valid && y > 0) ? ic - w : ic, id = (valid && y < h - 1) ? ic + w : ic;
+        const int r0 = L[ic], u0 = L[iu], d0 = L[id];
+        const int r = valid ? r0 : -1;
+        const int ru = (valid && y > 0) ? u0 : NONE, rd = (valid && y < h - 1) ? d0 : NONE;
+        // left and right neighbours (same row only): from the adjacent lanes, or loaded by the
+        // window's edge lanes
+        int lr = __shfl_up(r, 1), lu = __shfl_up(ru, 1), ld = __shfl_up(rd, 1);
+        int rr = __shfl_down(r, 1), ruu = __shfl_down(ru, 1), rdd = __shfl_down(rd, 1);
+        if (lane == 0 || lane == 63) {
+            const bool left = lane == 0;
+            const int xn = left ? x - 1 : x + 1;
+            int v = NONE, vu = NONE, vd = NONE;
+            if (valid && xn >= 0 && xn < w) {  // same row, so inside the image
+                const size_t j = left ? i - 1 : i + 1;
+                v = L[j];
+                if (y > 0) vu = L[j - w];
+                if (y < h - 1) vd = L[j + w];
+            }
+            if (left) {
+                lr = v; lu = vu; ld = vd;
+            } else {
+                rr = v; ruu = vu; rdd = vd;
+            }
+        }
+        const bool has_l = x > 0, has_r = x < w - 1;
+        const bool up = valid && ru == r, down = valid && rd == r;
+        const bool x1_beaten = valid && (up || (has_r && (rr == r || ruu == r || rdd == r)));
+        const unsigned long long m_up = __ballot(up), m_down = __ballot(down), m_x1 = __ballot(x1_beaten);
+        const bool head = (lane == 0 || lr != r || x == 0) && valid;
         const unsigned long long heads = __ballot(head);
         if (head && r >= 1 && r <= n) {
             const unsigned long long later = lane == 63 ? 0ull : (heads >> (lane + 1));
             int end = later ? lane + __ffsll((long long)later) : 64;  // first lane after the run
             end = (int)std::min<size_t>((size_t)end, total - base);
             const int k = r - 1, len = end - lane;
-            atomicMin(&out5[n + k], x);
-            atomicMax(&out5[2 * n + k], x + len - 1);
-            atomicMin(&out5[3 * n + k], y);
-            atomicMax(&out5[4 * n + k], y);
+            const unsigned long long run = (len == 64 ? ~0ull : ((1ull << len) - 1)) << lane;
+            const bool x0_beaten = up || (has_l && (lr == r || lu == r || ld == r));
+            if (!x0_beaten) atomicMin(&out5[n + k], x);
+            if (!((m_x1 >> (end - 1)) & 1ull)) atomicMax(&out5[2 * n + k], x + len - 1);
+            if (!(m_up & run)) atomicMin(&out5[3 * n + k], y);
+            if (!(m_down & run)) atomicMax(&out5[4 * n + k], y);
             if (area) atomicAdd(&area[k], len);
         }
     }

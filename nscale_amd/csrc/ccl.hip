@@ -649,34 +649,29 @@ __global__ void k_bbox(const int32_t* __restrict__ L, int w, int h, int n, int32
             x -= w;
             ++y;
         }
-        // the pixel and its neighbours above and below: clamped (always valid) addresses, all
-        // three loads issued together, out-of-image values replaced afterwards
+        // the pixel and its neighbours above and below, and for the window's first / last lane the
+        // same three one column further out (same row only; other lanes re-read their own): clamped
+        // (always valid) addresses, all six loads issued together, out-of-image values replaced
+        // afterwards
+        const bool has_l = x > 0, has_r = x < w - 1;
+        const bool has_u = valid && y > 0, has_d = valid && y < h - 1;
         const size_t ic = valid ? i : total - 1;
-        const size_t iu = (valid && y > 0) ? ic - w : ic, id = (valid && y < h - 1) ? ic + w : ic;
-        const int r0 = L[ic], u0 = L[iu], d0 = L[id];
+        const bool edge = valid && ((lane == 0 && has_l) || (lane == 63 && has_r));
+        const size_t ie = edge ? (lane == 0 ? ic - 1 : ic + 1) : ic;
+        const int r0 = L[ic], u0 = L[has_u ? ic - w : ic], d0 = L[has_d ? ic + w : ic];
+        const int e0 = L[ie], eu0 = L[has_u ? ie - w : ie], ed0 = L[has_d ? ie + w : ie];
         const int r = valid ? r0 : -1;
-        const int ru = (valid && y > 0) ? u0 : NONE, rd = (valid && y < h - 1) ? d0 : NONE;
-        // left and right neighbours (same row only): from the adjacent lanes, or loaded by the
-        // window's edge lanes
+        const int ru = has_u ? u0 : NONE, rd = has_d ? d0 : NONE;
+        const int ev = edge ? e0 : NONE, evu = (edge && has_u) ? eu0 : NONE, evd = (edge && has_d) ? ed0 : NONE;
+        // left and right neighbours: from the adjacent lanes, or the edge lanes' own loads
         int lr = __shfl_up(r, 1), lu = __shfl_up(ru, 1), ld = __shfl_up(rd, 1);
         int rr = __shfl_down(r, 1), ruu = __shfl_down(ru, 1), rdd = __shfl_down(rd, 1);
-        if (lane == 0 || lane == 63) {
-            const bool left = lane == 0;
-            const int xn = left ? x - 1 : x + 1;
-            int v = NONE, vu = NONE, vd = NONE;
-            if (valid && xn >= 0 && xn < w) {  // same row, so inside the image
-                const size_t j = left ? i - 1 : i + 1;
-                v = L[j];
-                if (y > 0) vu = L[j - w];
-                if (y < h - 1) vd = L[j + w];
-            }
-            if (left) {
-                lr = v; lu = vu; ld = vd;
-            } else {
-                rr = v; ruu = vu; rdd = vd;
-            }
+        if (lane == 0) {
+            lr = ev; lu = evu; ld = evd;
         }
-        const bool has_l = x > 0, has_r = x < w - 1;
+        if (lane == 63) {
+            rr = ev; ruu = evu; rdd = evd;
+        }
         const bool up = valid && ru == r, down = valid && rd == r;
         const bool x1_beaten = valid && (up || (has_r && (rr == r || ruu == r || rdd == r)));
         const unsigned long long m_up = __ballot(up), m_down = __ballot(down), m_x1 = __ballot(x1_beaten);

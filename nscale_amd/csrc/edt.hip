@@ -9,7 +9,7 @@
 //
 // Column pass (segmented for parallelism): per 64-row column segment, first/last zero; per column,
 // carry of the nearest zero above/below each segment; per segment, the final vertical distance g
-// (uint16, 0xFFFF = no zero in the column).
+// (uint16, 0xFFFF = no zero in the column) from a 64-bit mask of the segment's zero rows.
 // Row pass: one workgroup per row, g staged in LDS; each pixel searches outward while dx^2 is
 // below its best squared distance (work ~ distance value).  A pixel whose search would exceed
 // SEARCH_CAP columns flags its row; flagged rows are recomputed with the exact integer
@@ -33,16 +33,16 @@ __global__ void k_seg_zeros(const uint8_t* __restrict__ img, int w, int h, int n
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int sg = blockIdx.y;
     if (x >= w) return;
-    const int y0 = sg * SEG, y1 = min(h, y0 + SEG);
-    int f = -1, l = -1;
-    for (int y = y0; y < y1; ++y) {
-        if (img[(size_t)y * w + x] == 0) {
-            if (f < 0) f = y - y0;
-            l = y - y0;
-        }
-    }
-    fz[(size_t)sg * w + x] = (int16_t)f;
-    lz[(size_t)sg * w + x] = (int16_t)l;
+    const int y0 = sg * SEG, n = min(h, y0 + SEG) - y0;
+    // 64 loads in flight (rows past the image read the last row and are masked off), a bit per zero row
+    uint32_t v[SEG];
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) v[j] = img[(size_t)min(y0 + j, h - 1) * w + x];
+    unsigned long long zm = 0;
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) zm |= (unsigned long long)(v[j] == 0 && j < n) << j;
+    fz[(size_t)sg * w + x] = (int16_t)(zm ? __ffsll((long long)zm) - 1 : -1);
+    lz[(size_t)sg * w + x] = (int16_t)(zm ? 63 - __clzll((long long)zm) : -1);
 }
 
 // per column: row of the nearest zero strictly before / after each segment (-1 / INT_MAX-ish)
@@ -50,17 +50,33 @@ __global__ void k_seg_carry(const int16_t* __restrict__ fz, const int16_t* __res
                             int32_t* __restrict__ above, int32_t* __restrict__ below) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= w) return;
+    // sixteen segments' entries loaded at a time (clamped indices), then scanned
+    constexpr int CH = 16;
     int last = -1;
-    for (int s = 0; s < nseg; ++s) {
-        above[(size_t)s * w + x] = last;
-        int l = lz[(size_t)s * w + x];
-        if (l >= 0) last = s * SEG + l;
+    for (int s0 = 0; s0 < nseg; s0 += CH) {
+        int l[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) l[j] = lz[(size_t)min(s0 + j, nseg - 1) * w + x];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int s = s0 + j;
+            if (s >= nseg) break;
+            above[(size_t)s * w + x] = last;
+            if (l[j] >= 0) last = s * SEG + l[j];
+        }
     }
     int first = -1;
-    for (int s = nseg - 1; s >= 0; --s) {
-        below[(size_t)s * w + x] = first;
-        int f = fz[(size_t)s * w + x];
-        if (f >= 0) first = s * SEG + f;
+    for (int s1 = nseg - 1; s1 >= 0; s1 -= CH) {
+        int f[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) f[j] = fz[(size_t)max(s1 - j, 0) * w + x];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int s = s1 - j;
+            if (s < 0) break;
+            below[(size_t)s * w + x] = first;
+            if (f[j] >= 0) first = s * SEG + f[j];
+        }
     }
 }
 
@@ -69,20 +85,27 @@ __global__ void k_col_dist(const uint8_t* __restrict__ img, int w, int h, const 
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int sg = blockIdx.y;
     if (x >= w) return;
-    const int y0 = sg * SEG, y1 = min(h, y0 + SEG);
-    int up = above[(size_t)sg * w + x];  // row of the nearest zero above (-1: none)
-    for (int y = y0; y < y1; ++y) {
-        if (img[(size_t)y * w + x] == 0) up = y;
-        g[(size_t)y * w + x] = (up >= 0) ? (uint16_t)(y - up) : NONE;
-    }
-    int dn = below[(size_t)sg * w + x];  // row of the nearest zero below (-1: none)
-    for (int y = y1 - 1; y >= y0; --y) {
-        if (img[(size_t)y * w + x] == 0) dn = y;
-        if (dn >= 0) {
-            const size_t i = (size_t)y * w + x;
-            const uint16_t d = (uint16_t)(dn - y);
-            if (d < g[i]) g[i] = d;
-        }
+    const int y0 = sg * SEG, n = min(h, y0 + SEG) - y0;
+    // the segment's column at once: 64 loads in flight (rows past the image read the last row and
+    // are masked off), then a bit per zero row; each row's nearest zero above / below inside the
+    // segment is the highest set bit at or below it / the lowest at or above it
+    uint32_t v[SEG];
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) v[j] = img[(size_t)min(y0 + j, h - 1) * w + x];
+    unsigned long long zm = 0;
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) zm |= (unsigned long long)(v[j] == 0 && j < n) << j;
+    const int up = above[(size_t)sg * w + x];  // row of the nearest zero above the segment (-1: none)
+    const int dn = below[(size_t)sg * w + x];  // row of the nearest zero below it (-1: none)
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) {
+        if (j >= n) break;
+        const unsigned long long le = zm & (j == SEG - 1 ? ~0ull : ((2ull << j) - 1)), ge = zm >> j;
+        const int zu = le ? y0 + 63 - __clzll((long long)le) : up;
+        const int zd = ge ? y0 + j + __ffsll((long long)ge) - 1 : dn;
+        const int y = y0 + j;
+        const uint32_t du = zu >= 0 ? (uint32_t)(y - zu) : NONE, dd = zd >= 0 ? (uint32_t)(zd - y) : NONE;
+        g[(size_t)y * w + x] = (uint16_t)min(du, dd);
     }
 }
 
@@ -102,34 +125,32 @@ __global__ __launch_bounds__(256) void k_row_search(const uint16_t* __restrict__
     if (threadIdx.x == 0) need = 0;
     __syncthreads();
     for (int x = threadIdx.x; x < w; x += blockDim.x) {
-        long long best = (grow[x] == NONE) ? LLONG_MAX : (long long)grow[x] * grow[x];
+        // squared distances in 32 bits, exactly: g <= 65534 so g^2 < 2^32 and UINT_MAX marks "no
+        // zero"; a candidate dx^2 + v^2 is compared as v^2 < best - dx^2 (dx^2 < best), which can
+        // neither overflow nor underflow, and a winner is below the old best
+        const uint32_t gx = grow[x];
+        uint32_t best = (gx == NONE) ? UINT_MAX : gx * gx;
         int bdx = 0;
-        long long bg = grow[x];
+        uint32_t bg = gx;
         if (best != 0) {
             int dx = 1;
             for (; dx <= SEARCH_CAP; ++dx) {
-                const long long dx2 = (long long)dx * dx;
+                const uint32_t dx2 = (uint32_t)(dx * dx);
                 if (dx2 >= best) break;
                 if (x - dx < 0 && x + dx >= w) { dx = SEARCH_CAP + 2; break; }
                 if (x - dx >= 0) {
-                    const uint16_t v = grow[x - dx];
-                    if (v != NONE) {
-                        long long c = dx2 + (long long)v * v;
-                        if (c < best) { best = c; bdx = dx; bg = v; }
-                    }
+                    const uint32_t v = grow[x - dx];
+                    if (v != NONE && v * v < best - dx2) { best = dx2 + v * v; bdx = dx; bg = v; }
                 }
                 if (x + dx < w) {
-                    const uint16_t v = grow[x + dx];
-                    if (v != NONE) {
-                        long long c = dx2 + (long long)v * v;
-                        if (c < best) { best = c; bdx = dx; bg = v; }
-                    }
+                    const uint32_t v = grow[x + dx];
+                    if (v != NONE && v * v < best - dx2) { best = dx2 + v * v; bdx = dx; bg = v; }
                 }
             }
             if (dx == SEARCH_CAP + 1) need = 1;  // search cut short: exact envelope later
         }
         float o;
-        if (best == LLONG_MAX) o = sqrtf(1e15f);
+        if (best == UINT_MAX) o = sqrtf(1e15f);
         else o = edt_value((long long)bdx * bdx, bg);
         out[(size_t)y * w + x] = o;
     }

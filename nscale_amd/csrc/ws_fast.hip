@@ -148,24 +148,32 @@ __global__ void k_classify(Dev D, uint8_t* cls, uint16_t* lvl0, uint8_t* fmask, 
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < D.N; i += gridDim.x * blockDim.x) {
         const int x = i % W, y = i / W;
         const bool frame = (x == 0 || y == 0 || x == W - 1 || y == H - 1);
+        // the pixel's and its four neighbours' markers and colours, all loads issued together
+        // (a frame pixel, whose neighbours may leave the image, reads itself five times and
+        // ignores the copies); the decisions below are selects on loaded values
+        const int nb[4] = {frame ? i : i - 1, frame ? i : i + 1, frame ? i : i - W, frame ? i : i + W};
         const int v = D.m0[i];
-        uint8_t c;
-        uint16_t L = NOPUSH;
-        if (frame) c = C_FRAME;
-        else if (v > 0) c = C_SEED;
-        else {
-            const uint8_t* p = D.img + 3 * (size_t)i;
-            c = (p[0] == 0 && p[1] == 0 && p[2] == 0) ? C_B : C_F;
-            int best = 256;
-            auto pos = [&](int j, int xx, int yy) {
-                return !(xx == 0 || yy == 0 || xx == W - 1 || yy == H - 1) && D.m0[j] > 0;
-            };
-            if (pos(i - 1, x - 1, y)) best = min(best, c_diff(p, p - 3));
-            if (pos(i + 1, x + 1, y)) best = min(best, c_diff(p, p + 3));
-            if (pos(i - W, x, y - 1)) best = min(best, c_diff(p, p - 3 * (size_t)W));
-            if (pos(i + W, x, y + 1)) best = min(best, c_diff(p, p + 3 * (size_t)W));
-            if (best < 256) L = (uint16_t)best;
+        int vn[4];
+        uint8_t pc[3], nc[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) vn[k] = D.m0[nb[k]];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            pc[ch] = D.img[3 * (size_t)i + ch];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) nc[k][ch] = D.img[3 * (size_t)nb[k] + ch];
         }
+        int best = 256;  // smallest colour step to a neighbouring seed off the frame
+        const int nx[4] = {x - 1, x + 1, x, x}, ny[4] = {y, y, y - 1, y + 1};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool pos = !(nx[k] == 0 || ny[k] == 0 || nx[k] == W - 1 || ny[k] == H - 1) && vn[k] > 0;
+            const int d = c_diff(pc, nc[k]);
+            best = pos ? min(best, d) : best;
+        }
+        const bool black = pc[0] == 0 && pc[1] == 0 && pc[2] == 0;
+        const uint8_t c = frame ? C_FRAME : (v > 0 ? C_SEED : (black ? C_B : C_F));
+        const uint16_t L = (frame || v > 0 || best >= 256) ? NOPUSH : (uint16_t)best;
         cls[i] = c;
         lvl0[i] = L;
         fmask[i] = (c == C_F) ? 1 : 0;

@@ -126,7 +126,7 @@ def test_morphology(kernel, shape):
     assert np.array_equal(host(G().morph_open(dev(img), k)), o_morph("open", img, k))
 
 
-@pytest.mark.parametrize("case", ["blobs", "sparse_zeros", "no_zero_rows", "single_zero", "big"])
+@pytest.mark.parametrize("case", ["blobs", "sparse_zeros", "no_zero_rows", "single_zero", "big", "tallest"])
 def test_distance_transform(case):
     rng = np.random.default_rng(len(case))
     if case == "blobs":
@@ -141,6 +141,9 @@ def test_distance_transform(case):
     elif case == "single_zero":
         img = np.full((120, 1500), 255, np.uint8)  # distances beyond the search cap
         img[60, 1499] = 0
+    elif case == "tallest":  # the largest vertical distance (65533): squared distances near 2^32
+        img = np.full((65534, 3), 255, np.uint8)
+        img[0, 0] = 0
     else:
         from nscale_amd.synth import synth_he
         bgr = synth_he(1024, 1024, 9)

@@ -119,8 +119,8 @@ __device__ __forceinline__ int it_pseq(const Dev& D, int x) {
 // x pops before y in the merged phase-1 flood (x != y); sets FB_CMP if the walk is too long.
 // Reads the packed item records (k_pack_items), so a step is two dependent loads: the records of
 // x and y, then those of their chains' largest-key items.
-__device__ bool before(const Dev& D, int x, int y) {
-    for (int step = 0; step < CMP_MAX_STEPS; ++step) {
+__device__ bool before_from(const Dev& D, int x, int y, int step0) {
+    for (int step = step0; step < CMP_MAX_STEPS; ++step) {
         if (x == y) return false;
         const int4 rx = D.wr[x], ry = D.wr[y];  // {run, bnode, popidx, parent}
         if (rx.x == ry.x) return rx.z < ry.z;
@@ -137,6 +137,29 @@ __device__ bool before(const Dev& D, int x, int y) {
     }
     atomicOr(D.flags, FB_CMP);
     return false;
+}
+__device__ __forceinline__ bool before(const Dev& D, int x, int y) { return before_from(D, x, y, 0); }
+
+// What the first step of before() reads about an item: its run and pop index, its chain's
+// largest-key item, that item's level and pusher.  A tournament that compares the same winners
+// level after level loads this once per candidate and carries it along.
+struct ItemKey {
+    int item, run, popidx, top, lvl, par;
+};
+__device__ __forceinline__ ItemKey item_key(const Dev& D, int x) {
+    const int4 rx = D.wr[x];
+    const int4 ra = D.wr[rx.y];
+    return ItemKey{x, rx.x, rx.z, rx.y, (int)D.wl[rx.y], ra.w};
+}
+// before(D, kx.item, ky.item), its first step taken from the keys
+__device__ __forceinline__ bool before_keyed(const Dev& D, const ItemKey& kx, const ItemKey& ky) {
+    if (kx.item == ky.item) return false;
+    if (kx.run == ky.run) return kx.popidx < ky.popidx;
+    if (kx.lvl != ky.lvl) return kx.lvl < ky.lvl;
+    if (kx.par < 0 && ky.par < 0) return kx.top < ky.top;
+    if (kx.par < 0) return true;
+    if (ky.par < 0) return false;
+    return before_from(D, kx.par, ky.par, 1);
 }
 
 // ---- classification ----
@@ -683,6 +706,7 @@ constexpr int B1_BLOCKS = 1024;
 __global__ __launch_bounds__(256) void k_b1_partial(Dev D, const int32_t* __restrict__ real, const int* __restrict__ cand,
                                                     const int* __restrict__ ncand, int* __restrict__ partial) {
     __shared__ int best[256];
+    __shared__ ItemKey bkey[256];
     const int n = *ncand;
     int mine = -1;
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += B1_BLOCKS * 256) {
@@ -690,11 +714,15 @@ __global__ __launch_bounds__(256) void k_b1_partial(Dev D, const int32_t* __rest
         if (mine < 0 || before(D, real[b], real[mine])) mine = b;
     }
     best[threadIdx.x] = mine;
+    if (mine >= 0) bkey[threadIdx.x] = item_key(D, real[mine]);
     __syncthreads();
     for (int stride = 128; stride > 0; stride >>= 1) {
         if ((int)threadIdx.x < stride) {
             const int a = best[threadIdx.x], c = best[threadIdx.x + stride];
-            if (a < 0 || (c >= 0 && before(D, real[c], real[a]))) best[threadIdx.x] = c;
+            if (a < 0 || (c >= 0 && before_keyed(D, bkey[threadIdx.x + stride], bkey[threadIdx.x]))) {
+                best[threadIdx.x] = c;
+                bkey[threadIdx.x] = bkey[threadIdx.x + stride];
+            }
         }
         __syncthreads();
     }
@@ -706,6 +734,7 @@ __global__ __launch_bounds__(256) void k_b1_partial(Dev D, const int32_t* __rest
 __global__ __launch_bounds__(1024) void k_b1_reduce(Dev D, const int32_t* __restrict__ real, const int* __restrict__ cand,
                                                     int* __restrict__ info) {
     __shared__ int best[1024];
+    __shared__ ItemKey bkey[1024];
     const int n = B1_BLOCKS;
     int mine = -1;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -714,11 +743,15 @@ __global__ __launch_bounds__(1024) void k_b1_reduce(Dev D, const int32_t* __rest
         if (mine < 0 || before(D, real[b], real[mine])) mine = b;
     }
     best[threadIdx.x] = mine;
+    if (mine >= 0) bkey[threadIdx.x] = item_key(D, real[mine]);
     __syncthreads();
     for (int stride = 512; stride > 0; stride >>= 1) {
         if ((int)threadIdx.x < stride) {
             const int a = best[threadIdx.x], c = best[threadIdx.x + stride];
-            if (a < 0 || (c >= 0 && before(D, real[c], real[a]))) best[threadIdx.x] = c;
+            if (a < 0 || (c >= 0 && before_keyed(D, bkey[threadIdx.x + stride], bkey[threadIdx.x]))) {
+                best[threadIdx.x] = c;
+                bkey[threadIdx.x] = bkey[threadIdx.x + stride];
+            }
         }
         __syncthreads();
     }

@@ -640,7 +640,10 @@ __global__ void k_collect_b0(Dev D, int* __restrict__ blist, int* __restrict__ b
     const int stride = gridDim.x * blockDim.x;
     const int n_up = ((D.N + 63) / 64) * 64;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_up; i += stride) {
-        const bool take = i < D.N && D.cls[i] == C_B && D.lvl0[i] != NOPUSH;
+        const int j = min(i, D.N - 1);  // both loads unconditional (clamped), then the test
+        const uint8_t c = D.cls[j];
+        const uint16_t l0 = D.lvl0[j];
+        const bool take = i < D.N && c == C_B && l0 != NOPUSH;
         wave_append(take, i, blist, bcount);
     }
 }
@@ -660,6 +663,35 @@ __global__ void k_bucket_runs(const int32_t* __restrict__ box5, int nrun, int* _
         while (b < kRunTiers - 1 && L > kRunTier[b]) ++b;
     }
     for (int t = 0; t < kRunTiers; ++t) wave_append(b == t, r, lists + (size_t)t * nrun, &counts[t]);
+}
+
+// Label a B pixel b (never on the frame) takes when its queue entry r pops: the agreeing positive
+// label of its seed neighbours and of its F neighbours popped (labelled) before r, WSHED where
+// they disagree, 0 for none.  The four neighbours' class, marker, state and label are loaded
+// together and used through selects (loads behind the class test were a dependent round trip
+// per neighbour); only the pop-order walk stays conditional.
+__device__ int b_label(const Dev& D, int b, int r, int W) {
+    const int dirs[4] = {-1, 1, -W, W};
+    uint8_t cs[4], ss[4];
+    int ms[4], ls[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const int s = b + dirs[d];
+        cs[d] = D.cls[s];
+        ss[d] = D.st[s];
+        ms[d] = D.m0[s];
+        ls[d] = D.lab[s];
+    }
+    int lab = 0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const bool seed = cs[d] == C_SEED;
+        const bool fcand = cs[d] == C_F && ss[d] == 2 && ls[d] > 0;
+        int t = seed ? ms[d] : 0;
+        if (!seed && fcand && before(D, b + dirs[d], r)) t = ls[d];
+        if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
+    }
+    return lab;
 }
 
 // ---- B items: the real queue entry of each B pixel and its label decision at pop time ----
@@ -684,17 +716,7 @@ __global__ void k_bresolve(Dev D, int32_t* __restrict__ real, int* __restrict__ 
                 }
             }
             real[b] = r;
-            if (r >= 0) {
-                const int dirs[4] = {-1, 1, -W, W};
-                for (int d = 0; d < 4; ++d) {
-                    const int s = b + dirs[d];
-                    int t = 0;
-                    const uint8_t cs = D.cls[s];
-                    if (cs == C_SEED) t = D.m0[s];
-                    else if (cs == C_F && D.st[s] == 2 && D.lab[s] > 0 && before(D, s, r)) t = D.lab[s];
-                    if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
-                }
-            }
+            if (r >= 0) lab = b_label(D, b, r, W);
         }
         block_append(lab > 0, b, cand, ncand);
     }
@@ -760,19 +782,7 @@ __global__ __launch_bounds__(1024) void k_b1_reduce(Dev D, const int32_t* __rest
         info[0] = b1;
         info[1] = b1 >= 0 ? real[b1] : -1;
         int lab = 0;
-        if (b1 >= 0) {
-            const int W = D.W;
-            const int dirs[4] = {-1, 1, -W, W};
-            const int r = real[b1];
-            for (int d = 0; d < 4; ++d) {
-                const int s = b1 + dirs[d];
-                int t = 0;
-                const uint8_t cs = D.cls[s];
-                if (cs == C_SEED) t = D.m0[s];
-                else if (cs == C_F && D.st[s] == 2 && D.lab[s] > 0 && before(D, s, r)) t = D.lab[s];
-                if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
-            }
-        }
+        if (b1 >= 0) lab = b_label(D, b1, real[b1], D.W);
         info[2] = lab;
         // a level-0 b1 could share the level-0 FIFO with other items: outside the decomposition
         if (b1 >= 0 && it_lvl(D, real[b1]) == 0) atomicOr(D.flags, FB_TIES);

@@ -175,7 +175,26 @@ __global__ __launch_bounds__(M2_THREADS) void morph2_kernel(const uint8_t* __res
     __syncthreads();
     const uint32_t ident = ERODE ? 0x00FF00FFu : 0u;
     constexpr int NP = MT / 2;  // output pairs per row
-    for (int d = 0; d < P.nrun; ++d) {
+    // nested runs (each extends the one before it, as for disks): a thread carries its running
+    // result from run to run in a register and writes every run's table once, one barrier in all
+    bool chain = true;
+    for (int d = 0; d < P.nrun; ++d) chain = chain && P.base[d] == d - 1;
+    if (chain) {
+        for (int k = threadIdx.x; k < WW * NP; k += M2_THREADS) {
+            const int x = 2 * (k % NP), row = k / NP;
+            const uint16_t* src = win + row * WS + x;
+            uint32_t v = taps2<ERODE>(ident, src, r + P.ra[0], r + P.rb[0]);
+            *reinterpret_cast<uint32_t*>(H + row * MT + x) = v;
+            for (int d = 1; d < P.nrun; ++d) {
+                const int a = P.ra[d], b = P.rb[d], pa = P.ra[d - 1], pb = P.rb[d - 1];
+                if (a < pa) v = taps2<ERODE>(v, src, r + a, r + pa - 1);
+                if (b > pb) v = taps2<ERODE>(v, src, r + pb + 1, r + b);
+                *reinterpret_cast<uint32_t*>(H + (d * WW + row) * MT + x) = v;
+            }
+        }
+        __syncthreads();
+    }
+    for (int d = 0; d < (chain ? 0 : P.nrun); ++d) {
         const int a = P.ra[d], b = P.rb[d], bs = P.base[d];
         uint16_t* Hd = H + d * WW * MT;
         for (int k = threadIdx.x; k < WW * NP; k += M2_THREADS) {

@@ -303,7 +303,9 @@ __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h,
 // Four pixels per thread, i, i + S, i + 2S, i + 3S (S = threads in the grid): each of the four
 // loads is coalesced over the wave, and the four labels and the four first hops are each issued
 // together instead of one dependent round trip after another.
-__global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval) {
+// `zero_at_roots` (optional): entry i cleared for every root i, so a per-root table (areas) needs
+// no image-sized fill before it is accumulated.
+__global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval, int32_t* __restrict__ zero_at_roots) {
     const size_t S = (size_t)gridDim.x * blockDim.x;
     for (size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * S) {
         int p[4], q[4];
@@ -325,6 +327,7 @@ __global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval) {
             // read-only walk: a path-jumping find here could re-point a pixel whose final root
             // another thread has already written back to an intermediate ancestor.  Roots are final
             // in this kernel and every value an entry takes is an ancestor, so cached loads are safe.
+            if (zero_at_roots && p[j] == (int)i) zero_at_roots[i] = 0;
             int r = p[j], nx = q[j];
             while (nx != r) {
                 r = nx;
@@ -336,7 +339,8 @@ __global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval) {
 }
 
 template <int MODE>
-int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipStream_t s) {
+int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipStream_t s,
+            int32_t* zero_at_roots = nullptr) {
     const int tiles_x = (int)cdiv(w, CT_W), tiles_y = (int)cdiv(h, CT_H);
     const int nt = tiles_x * tiles_y;
     const int nseam8 = nt * (CT_W + 2 * CT_H), nseam4 = nt * (CT_W + CT_H);
@@ -348,7 +352,8 @@ int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipS
         hipLaunchKernelGGL((ccl_local<MODE, 4>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x);
         hipLaunchKernelGGL((ccl_seams<MODE, 4>), dim3(cdiv(nseam4, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
     }
-    hipLaunchKernelGGL(ccl_flatten, dim3(std::min<unsigned>(cdiv(n, 1024), 65535u)), dim3(256), 0, s, L, n, bgval);
+    hipLaunchKernelGGL(ccl_flatten, dim3(std::min<unsigned>(cdiv(n, 1024), 65535u)), dim3(256), 0, s, L, n, bgval,
+                       zero_at_roots);
     NSC_HIP(hipGetLastError());
     return NSC_OK;
 }
@@ -718,8 +723,8 @@ int ccl_mask(const uint8_t* fg, const float* vals, int32_t* labels, int w, int h
 int bwareaopen_impl(const uint8_t* img, uint8_t* out, int w, int h, int lo, int hi, int conn, int* dcount,
                     int32_t* labels, int32_t* area, hipStream_t s) {
     const size_t n = (size_t)w * h;
-    NSC_CHECK(ccl_u8(img, labels, w, h, conn, true, -1, s));
-    NSC_HIP(hipMemsetAsync(area, 0, n * sizeof(int32_t), s));
+    // equal-value labelling whose flatten clears the roots' area entries (the only ones read)
+    NSC_CHECK(ccl_run<FG_EQ>(Px{img, nullptr, 0}, labels, w, h, conn, -1, s, area));
     hipLaunchKernelGGL(k_area, grid1(n), dim3(256), 0, s, labels, area, n);
     if (dcount) NSC_HIP(hipMemsetAsync(dcount, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_area_select, grid_counted(n), dim3(256), 0, s, labels, area, out, n, lo, hi, dcount);

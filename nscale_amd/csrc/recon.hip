@@ -701,11 +701,10 @@ static int recon_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* til
     const size_t bytes = sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2);
     NSC_CHECK(sc.alloc(bytes, s));
     unsigned* stamp = sc.as<unsigned>();
-    int* listA = reinterpret_cast<int*>(stamp + nt);
+    int* counts = reinterpret_cast<int*>(stamp + nt);  // next to the stamps: one fill clears both
+    int* listA = counts + max_pass + 2;
     int* listB = listA + nt;
-    int* counts = listB + nt;
-    NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt, s));
-    NSC_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (max_pass + 2), s));
+    NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt + sizeof(int) * (max_pass + 2), s));
     const int grid = std::min(nt, 1 << 16);
     auto launch = [&](int pass, const int* lst, const int* cnt, int* nl, int* nc) {
         switch (G) {
@@ -1324,11 +1323,10 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
                        s));
     uint4* flags = sc.as<uint4>();
     unsigned* stamp = reinterpret_cast<unsigned*>(flags + TW * (size_t)nwin);
-    int* listA = reinterpret_cast<int*>(stamp + nt);
+    int* counts = reinterpret_cast<int*>(stamp + nt);  // next to the stamps: one fill clears both
+    int* listA = counts + max_pass + 2;
     int* listB = listA + nt;
-    int* counts = listB + nt;
-    NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt, s));
-    NSC_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (max_pass + 2), s));
+    NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt + sizeof(int) * (max_pass + 2), s));
     hipLaunchKernelGGL(geo_flags, dim3(nwin), dim3(TW), 0, s, pass, flags, w, h, tiles_x);
     const unsigned grid = (unsigned)std::min(nt, 1 << 16);
     auto launch = [&](int p, const int* lst, const int* cnt, int* nl, int* nc) {

@@ -303,9 +303,10 @@ __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h,
 // Four pixels per thread, i, i + S, i + 2S, i + 3S (S = threads in the grid): each of the four
 // loads is coalesced over the wave, and the four labels and the four first hops are each issued
 // together instead of one dependent round trip after another.
-// `zero_at_roots` (optional): entry i cleared for every root i, so a per-root table (areas) needs
-// no image-sized fill before it is accumulated.
-__global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval, int32_t* __restrict__ zero_at_roots) {
+// `zero_at_roots` / `zero8_at_roots` (optional): entry i cleared for every root i, so a per-root
+// table (areas, selection flags) needs no image-sized fill before it is filled in.
+__global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval, int32_t* __restrict__ zero_at_roots,
+                            uint8_t* __restrict__ zero8_at_roots) {
     const size_t S = (size_t)gridDim.x * blockDim.x;
     for (size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * S) {
         int p[4], q[4];
@@ -328,6 +329,7 @@ __global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval, int32_t* __rest
             // another thread has already written back to an intermediate ancestor.  Roots are final
             // in this kernel and every value an entry takes is an ancestor, so cached loads are safe.
             if (zero_at_roots && p[j] == (int)i) zero_at_roots[i] = 0;
+            if (zero8_at_roots && p[j] == (int)i) zero8_at_roots[i] = 0;
             int r = p[j], nx = q[j];
             while (nx != r) {
                 r = nx;
@@ -340,7 +342,7 @@ __global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval, int32_t* __rest
 
 template <int MODE>
 int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipStream_t s,
-            int32_t* zero_at_roots = nullptr) {
+            int32_t* zero_at_roots = nullptr, uint8_t* zero8_at_roots = nullptr) {
     const int tiles_x = (int)cdiv(w, CT_W), tiles_y = (int)cdiv(h, CT_H);
     const int nt = tiles_x * tiles_y;
     const int nseam8 = nt * (CT_W + 2 * CT_H), nseam4 = nt * (CT_W + CT_H);
@@ -353,7 +355,7 @@ int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipS
         hipLaunchKernelGGL((ccl_seams<MODE, 4>), dim3(cdiv(nseam4, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
     }
     hipLaunchKernelGGL(ccl_flatten, dim3(std::min<unsigned>(cdiv(n, 1024), 65535u)), dim3(256), 0, s, L, n, bgval,
-                       zero_at_roots);
+                       zero_at_roots, zero8_at_roots);
     NSC_HIP(hipGetLastError());
     return NSC_OK;
 }
@@ -736,8 +738,8 @@ int bwareaopen_impl(const uint8_t* img, uint8_t* out, int w, int h, int lo, int 
 static int select_components(const uint8_t* img, uint8_t key, const uint8_t* src, int32_t* labels, uint8_t* flag,
                              int w, int h, int conn, hipStream_t s) {
     const size_t n = (size_t)w * h;
-    NSC_CHECK(ccl_neq(img, key, labels, w, h, conn, s));
-    NSC_HIP(hipMemsetAsync(flag, 0, n, s));
+    // labelling whose flatten clears the roots' flags (the only entries read or set)
+    NSC_CHECK(ccl_run<FG_NEQ>(Px{img, nullptr, key}, labels, w, h, conn, -1, s, nullptr, flag));
     hipLaunchKernelGGL(k_flag_roots, grid1(n), dim3(256), 0, s, labels, src, flag, w, h);
     NSC_HIP(hipGetLastError());
     return NSC_OK;

@@ -722,6 +722,14 @@ int ccl_mask(const uint8_t* fg, const float* vals, int32_t* labels, int w, int h
     return ccl_run<FG_FEQ>(P, labels, w, h, conn, -1, s);
 }
 
+// ccl_mask whose flatten also clears flags[root] for every root (a per-root flag table then
+// needs no image-sized fill)
+int ccl_mask_clear_root_flags(const uint8_t* fg, const float* vals, int32_t* labels, int w, int h, int conn,
+                              uint8_t* flags, hipStream_t s) {
+    Px P{fg, vals, 0};
+    return ccl_run<FG_FEQ>(P, labels, w, h, conn, -1, s, nullptr, flags);
+}
+
 int bwareaopen_impl(const uint8_t* img, uint8_t* out, int w, int h, int lo, int hi, int conn, int* dcount,
                     int32_t* labels, int32_t* area, hipStream_t s) {
     const size_t n = (size_t)w * h;

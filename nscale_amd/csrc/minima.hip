@@ -18,6 +18,8 @@
 namespace nsc {
 
 int ccl_mask(const uint8_t* fg, const float* vals, int32_t* labels, int w, int h, int conn, hipStream_t s);
+int ccl_mask_clear_root_flags(const uint8_t* fg, const float* vals, int32_t* labels, int w, int h, int conn,
+                              uint8_t* flags, hipStream_t s);
 
 namespace {
 
@@ -152,8 +154,7 @@ int local_max_impl(const float* img, uint8_t* out, int w, int h, int conn, bool 
         hipLaunchKernelGGL((k_lmax_cand<8>), grid1(n), dim3(256), 0, s, c, marker, rec, cand, greater, w, h);
     else
         hipLaunchKernelGGL((k_lmax_cand<4>), grid1(n), dim3(256), 0, s, c, marker, rec, cand, greater, w, h);
-    NSC_CHECK(ccl_mask(cand, c, L, w, h, conn, s));
-    NSC_HIP(hipMemsetAsync(flag, 0, n, s));
+    NSC_CHECK(ccl_mask_clear_root_flags(cand, c, L, w, h, conn, flag, s));  // flag[] is read at roots only
     hipLaunchKernelGGL(k_flag_from, grid1(n), dim3(256), 0, s, L, greater, flag, n);
     hipLaunchKernelGGL(k_lmax_out, grid1(n), dim3(256), 0, s, L, flag, out, n);
     NSC_HIP(hipGetLastError());

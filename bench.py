@@ -36,45 +36,80 @@ os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def cpu_baseline(size, seconds_budget=20.0):
-    """Oracle chain (single-threaded restatement of the reference CPU path) on the host cores,
-    one tile per thread (the reference scales out one tile per process).  Bounded sample."""
+def host_cpus():
+    """Cores this process may use: the affinity set, capped by a cgroup CPU quota when one is set."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except Exception:
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    usable = min(aff, quota) if quota else aff
+    return {"nproc": os.cpu_count(), "affinity": aff, "cgroup_quota_cpus": quota, "usable": usable, "model": model}
+
+
+def oracle_cxxflags():
+    try:
+        for line in open(os.path.join(ROOT, "oracle", "Makefile")):
+            if line.startswith("CXXFLAGS"):
+                return "g++ " + line.split("=", 1)[1].strip()
+    except Exception:
+        pass
+    return None
+
+
+def cpu_baseline(size, tiles_per_thread=2):
+    """Oracle chain (single-threaded restatement of the reference CPU path, which is itself
+    single-threaded per tile) on the host cores: one thread per usable core, each running
+    tiles_per_thread whole tiles (the reference scales out one tile per process), plus the
+    single-core latency of one tile.  Bounded sample: about (1 + tiles_per_thread) tile times."""
     import numpy as np
     from nscale_amd.synth import synth_he
     from tests._lib import SegParams, oracle, ptr
 
     o = oracle()
-    cores = max(1, min(16, os.cpu_count() or 1))
-    tiles = [synth_he(size, size, 2026 + k) for k in range(cores)]
+    cpus = host_cpus()
+    cores = cpus["usable"]
+    tiles = [synth_he(size, size, 2026 + k) for k in range(min(cores, 8))]
     p = SegParams()
     o.orc_seg_params_default(ctypes.byref(p))
 
-    def run(k, out):
+    def run(k, out, reps=1):
         lab = np.empty((size, size), np.int32)
         t = time.perf_counter()
-        o.orc_segment_nuclei(ptr(tiles[k]), size, size, ctypes.byref(p), ptr(lab), None, None)
+        for r in range(reps):
+            o.orc_segment_nuclei(ptr(tiles[(k + r) % len(tiles)]), size, size, ctypes.byref(p), ptr(lab), None, None)
         out[k] = time.perf_counter() - t
 
-    # single core latency
-    lat = [0.0] * cores
-    run(0, lat)
+    lat = [0.0]
+    run(0, lat)  # single-core latency of one tile (the process is otherwise idle)
     one_core = size * size / lat[0] / 1e6
-    # all cores, one tile each (skipped when the single tile already used most of the budget)
-    agg = None
-    if lat[0] * 1.5 < seconds_budget:
-        times = [0.0] * cores
-        ths = [threading.Thread(target=run, args=(k, times)) for k in range(cores)]
-        t0 = time.perf_counter()
-        for th in ths:
-            th.start()
-        for th in ths:
-            th.join()
-        agg = cores * size * size / (time.perf_counter() - t0) / 1e6
-    return {"value": round(agg if agg is not None else one_core, 3), "unit": "Mpix/s",
-            "cores": cores if agg is not None else 1, "kind": "port",
-            "sample": "oracle segmentNuclei chain on %dx%d synthetic H&E tiles, one tile per thread "
-                      "(%d threads); single-core %.3f Mpix/s (%.2f s/tile)" % (size, size, cores, one_core, lat[0]),
-            "one_core_mpix_s": round(one_core, 3)}
+    times = [0.0] * cores
+    ths = [threading.Thread(target=run, args=(k, times, tiles_per_thread)) for k in range(cores)]
+    t0 = time.perf_counter()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    wall = time.perf_counter() - t0
+    agg = cores * tiles_per_thread * size * size / wall / 1e6
+    return {"value": round(agg, 3), "unit": "Mpix/s", "cores": cores, "kind": "port",
+            "sample": "oracle segmentNuclei chain on %dx%d synthetic H&E tiles: %d threads x %d tiles each "
+                      "(%.1f s wall); single-core %.3f Mpix/s (%.2f s/tile)"
+                      % (size, size, cores, tiles_per_thread, wall, one_core, lat[0]),
+            "one_core_mpix_s": round(one_core, 3), "tiles_per_thread": tiles_per_thread,
+            "nproc": cpus["nproc"], "affinity_cpus": cpus["affinity"], "cgroup_quota_cpus": cpus["cgroup_quota_cpus"],
+            "cpu_model": cpus["model"], "compiler": oracle_cxxflags()}
 
 
 def bench_recon(args):
@@ -138,16 +173,8 @@ def pct(xs, q):
 
 
 def serve_sizes(n, seed):
-    """Tile sizes of the batched/served workloads: short side log-uniform in [512, 4096], long side =
-    short * U[1, 2], random orientation."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n):
-        a = int(round(float(np.exp(rng.uniform(np.log(512), np.log(4096))))))
-        b = int(round(a * float(rng.uniform(1.0, 2.0))))
-        out.append((a, b) if rng.random() < 0.5 else (b, a))
-    return out
+    from nscale_amd.synth import serve_sizes as sizes
+    return sizes(n, seed)
 
 
 def bench_serve(args):

@@ -81,12 +81,18 @@ typedef struct nsc_recon_desc {
 int nsc_imreconstruct_u8_batched(const nsc_recon_desc* descs, int n, void* stream);
 int nsc_imreconstruct_f32_batched(const nsc_recon_desc* descs, int n, void* stream);
 /* Autotuner of the reconstruction schedule (no reference counterpart; the reference has fixed
-   shapes): times each candidate (windows per workgroup side 1 or 2, 4 or 8 passes between host
-   checks) on this batch -- outputs are written each time -- and keeps the fastest for later
-   calls of the same element type (is_float 0: uint8, 1: float).  region / passes (may be NULL)
-   receive the choice, timings (may be NULL, 4 floats) the measured ms per candidate. */
+   shapes): times each candidate -- windows per workgroup side 1 or 2 with 4 or 8 passes between
+   host checks, and the device-side work queue for the residual wavefront -- on this batch (outputs
+   are written each time) and keeps the fastest for the calling thread's later calls of the same
+   element type (is_float 0: uint8, 1: float); other threads keep their own schedule.  region /
+   passes (may be NULL) receive the choice (passes 0: the work queue), timings (may be NULL, 5
+   floats) the measured ms per candidate in that order. */
 int nsc_recon_autotune(const nsc_recon_desc* descs, int n, int is_float, void* stream, int* region, int* passes,
                        float* timings);
+/* Sets the calling thread's reconstruction schedule explicitly (no reference counterpart): region
+   side 1..4 (0 = default), passes between host checks (0 = default), queue 1 = device-side work
+   queue after the first pass, 0 = host-launched passes, -1 = the NSC_RECON_DRAIN default. */
+int nsc_recon_set_schedule(int is_float, int region, int passes, int queue);
 /* nscale::imhmin<float>(image, h, conn) on n images (MorphologicOperations.cpp:3342-3364):
    descs[i].mask = input image, descs[i].out = output, descs[i].marker unused */
 int nsc_imhmin_f32_batched(const nsc_recon_desc* descs, int n, float hval, void* stream);
@@ -124,6 +130,14 @@ int nsc_bwareaopen2_u8(const uint8_t* img, uint8_t* out, int w, int h, int minSi
 int nsc_bounding_boxes(const int32_t* canon_labels, int w, int h, int n, int32_t* out5, int32_t* area,
                        void* stream);
 
+/* ConnComponents::boundingBox(w, h, label, bgval, compcount)  ConnComponents.h:43, .cpp:241-284,
+   literally: labels are scanned in raster order, bgval skipped, the scan stops at the first -1 or 0
+   label; *compcount = number of labels met, out5 (DEVICE, may be NULL) receives the reference's
+   block of 5 x compcount ints [label | minx | maxx | miny | maxy] (column stride compcount, labels in
+   first-seen order) when compcount <= cap.  Labels met before the stop must lie in 1..w*h-1. */
+int nsc_cc_bounding_box(const int32_t* labels, int w, int h, int32_t bgval, int32_t* out5, int cap, int* compcount,
+                        void* stream);
+
 /* ---- morphology, distance, minima, watershed -------------------------------------------- */
 /* cv::erode / cv::dilate with a k x k structuring element (HOST array, k odd, k <= 31), anchor at the
    centre, and `outside` as the value of pixels beyond the image (morphOpen pads with 255 for the
@@ -148,6 +162,10 @@ int nsc_cv_watershed(const uint8_t* bgr, int32_t* markers, int w, int h, void* s
    2 comparison walk, 4 tie group too large / level-0 first background pixel, 8 heap capacity,
    16 tie order matters); no reference counterpart, reporting only */
 void nsc_watershed_stats(int* fast, int* fallback, int* last_reason);
+/* process-wide sums of the decomposed flood's phase-1 runs per tier of LDS frame size: frames of
+   at most 512, 1024, 2048, 4096 and 6400 pixels, then the global-memory bucket; writes min(n, 6)
+   counters and returns 6 (no reference counterpart, reporting only) */
+int nsc_watershed_tier_runs(long long* runs, int n);
 /* nscale::watershed2(origImage, image, conn)  MorphologicOperations.h:103, .cpp:3717-3760 */
 int nsc_watershed2(const uint8_t* bgr, const float* img, int32_t* out, int w, int h, int conn, void* stream);
 
@@ -182,6 +200,82 @@ int nsc_workspace_destroy(nsc_workspace* ws);
    the chain status code. */
 int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p, int32_t* labels,
                        int* compcount, nsc_workspace* ws, void** stage, double* stage_ms, void* stream);
+
+/* Bounding-box outputs of the chain (nsc_seg_io.bbox_mode). */
+#define NSC_BBOX_NONE 0
+/* ConnComponents::boundingBox(w, h, labels, bgval 0, compcount) exactly as segmentNuclei calls it
+   (HistologicalEntities.cpp:590-591, ConnComponents.cpp:241-284): the scan skips label 0 and stops
+   at the first background (-1) pixel in raster order, so on a tile whose first pixel is background
+   compcount is 0 and no box is written.  Boxes come in first-seen order, label = the component's
+   label (its smallest raster index), extents over the scanned pixels only. */
+#define NSC_BBOX_REFERENCE_CPU 1
+/* one box per nucleus in ascending label order, the result of nscale::gpu::boundingBox2
+   (cuda/ccl_uf.cu:993-1107) on the returned label image: label column = the nucleus's label in
+   the returned image (smallest raster index, or 1..N when canonical), extents over all its pixels;
+   area (optional) receives the pixel counts.  compcount = number of nuclei. */
+#define NSC_BBOX_ALL 2
+
+/* Pitched images and box outputs of nsc_segment_nuclei_io (the reference's Mat/GpuMat arguments
+   carry a row step; its GPU operators make continuous copies internally, MorphologicOperationsGPU.cpp:180-186). */
+typedef struct nsc_seg_io {
+    /* inputs */
+    size_t bgr_step;      /* bytes between rows of bgr; 0 = 3*w */
+    size_t labels_step;   /* bytes between rows of labels; 0 = 4*w */
+    int canonical;        /* 0: the reference CPU labels (smallest raster index, background -1);
+                             1: 1..N in raster order of first pixel, background 0 (the GPU path's relabel) */
+    int bbox_mode;        /* NSC_BBOX_NONE / NSC_BBOX_REFERENCE_CPU / NSC_BBOX_ALL */
+    int32_t* bbox5;       /* DEVICE, room for 5 * max_boxes ints: receives the reference's n x 5 block
+                             [label | minx | maxx | miny | maxy], structure of arrays, column stride n = compcount */
+    int32_t* area;        /* DEVICE, optional, max_boxes ints (NSC_BBOX_ALL): pixels per box */
+    int max_boxes;        /* capacity; a larger compcount returns NSC_ERR_LIMIT with compcount set */
+    /* outputs */
+    int compcount;        /* the number of boxes the mode defines (the reference's compcount) */
+    int nuclei;           /* number of nuclei (labelled components) */
+} nsc_seg_io;
+/* HistologicalEntities::segmentNuclei(img, output, compcount, bbox, ...) with pitched images and
+   the box outputs selected by io (may be NULL: then as nsc_segment_nuclei without compcount). */
+int nsc_segment_nuclei_io(const uint8_t* bgr, int w, int h, const nsc_seg_params* p, int32_t* labels, nsc_seg_io* io,
+                          nsc_workspace* ws, void** stage, double* stage_ms, void* stream);
+
+/* HistologicalEntities::plFindNucleusCandidates(img, seg_norbc, ...) (stage S1),
+   HistologicalEntities.h:63-64, .cpp:115-304: seg_norbc (DEVICE u8, row step seg_step bytes; 0 =
+   w) receives the candidate mask.  Returns NSC_SEG_CONTINUE when the chain goes on, or
+   NSC_SEG_BACKGROUND / NSC_SEG_BACKGROUND_LIKELY / NSC_SEG_NO_CANDIDATES_LEFT (seg_norbc then
+   unspecified), or a negative error. */
+int nsc_pl_find_nucleus_candidates(const uint8_t* bgr, size_t bgr_step, int w, int h, const nsc_seg_params* p,
+                                   uint8_t* seg_norbc, size_t seg_step, nsc_workspace* ws, void* stream);
+/* HistologicalEntities::plSeparateNuclei(img, seg_open, seg_nonoverlap, minSizePl,
+   watershedConnectivity) (stage A4), HistologicalEntities.h:65-66, .cpp:308-449: seg_open and
+   seg_nonoverlap are DEVICE u8 planes with row steps (0 = w).  Returns NSC_SEG_CONTINUE or a
+   negative error. */
+int nsc_pl_separate_nuclei(const uint8_t* bgr, size_t bgr_step, const uint8_t* seg_open, size_t open_step, int w, int h,
+                           int minSizePl, int watershedConnectivity, uint8_t* seg_nonoverlap, size_t out_step,
+                           nsc_workspace* ws, void* stream);
+
+/* ---- per-nucleus features -------------------------------------------------------------- */
+/* Outputs of nsc_object_features: DEVICE arrays of n entries (intensity: n x 8, gradient: n x 6,
+   row-major per nucleus); intensity / gradient may be NULL. */
+typedef struct nsc_object_features_out {
+    int32_t* area;          /* ObjFeatures::area           ObjFeatures.h:38, .cpp:15-45 */
+    double* perimeter;      /* ObjFeatures::perimeter      ObjFeatures.h:41, .cpp:144-278 */
+    double* major_axis;     /* ObjFeatures::ellipse        ObjFeatures.h:39, .cpp:48-116 */
+    double* minor_axis;
+    double* eccentricity;
+    double* extent_ratio;   /* ObjFeatures::extent_ratio   ObjFeatures.h:40, .cpp:118-141 */
+    double* circularity;    /* ObjFeatures::circularity    ObjFeatures.h:42, .cpp:281-293 */
+    float* intensity;       /* ObjFeatures::intensityFeatures ObjFeatures.h:49, .cpp:342-375: mean, max,
+                               min, std, entropy, energy, skewness, kurtosis of the plane's values */
+    float* gradient;        /* ObjFeatures::gradientFeatures  ObjFeatures.h:50, .cpp:377-418: mean, std,
+                               entropy, energy, skewness, kurtosis of Operators::gradient's magnitude */
+} nsc_object_features_out;
+/* Features of the n objects of a DEVICE label image described by a DEVICE box block (5 x n,
+   column stride n: [label | minx | maxx | miny | maxy], as nsc_bounding_boxes or the chain's
+   NSC_BBOX_ALL output give it); img (DEVICE, may be NULL without intensity / gradient outputs) is
+   an interleaved image of `channels` bytes per pixel whose plane `channel` is measured (the
+   reference splits BGR and passes one plane, ObjFeatures.cpp:540-560).  The canny features
+   (ObjFeatures.h:51) are not provided. */
+int nsc_object_features(const int32_t* labels, int w, int h, const int32_t* bbox5, int n, const uint8_t* img,
+                        int channels, int channel, nsc_object_features_out* out, void* stream);
 
 /* ---- continuous-batching tile server --------------------------------------------------- */
 /* Replaces the reference's one-tile-per-MPI-process scale-out of segmentNuclei

@@ -36,6 +36,22 @@ class Request(ctypes.Structure):
                 ("bbox5", _P), ("area", _P), ("max_labels", _I), ("params", _P)]
 
 
+class SegIO(ctypes.Structure):
+    """nsc_seg_io: row steps, label form and the bounding-box outputs of nsc_segment_nuclei_io."""
+    _fields_ = [("bgr_step", ctypes.c_size_t), ("labels_step", ctypes.c_size_t), ("canonical", _I),
+                ("bbox_mode", _I), ("bbox5", _P), ("area", _P), ("max_boxes", _I),
+                ("compcount", _I), ("nuclei", _I)]
+
+
+BBOX_NONE, BBOX_REFERENCE_CPU, BBOX_ALL = 0, 1, 2
+
+
+class ObjFeaturesOut(ctypes.Structure):
+    """nsc_object_features_out: device arrays of the per-nucleus features."""
+    _fields_ = [(k, _P) for k in ("area", "perimeter", "major_axis", "minor_axis", "eccentricity", "extent_ratio",
+                                  "circularity", "intensity", "gradient")]
+
+
 class ReconDesc(ctypes.Structure):
     _fields_ = [("marker", _P), ("mask", _P), ("out", _P), ("w", _I), ("h", _I), ("conn", _I)]
 
@@ -70,6 +86,12 @@ def lib():
         L.nsc_imreconstruct_u8_batched.argtypes = [ctypes.POINTER(ReconDesc), _I, _P]
         L.nsc_imreconstruct_f32_batched.argtypes = [ctypes.POINTER(ReconDesc), _I, _P]
         L.nsc_imhmin_f32_batched.argtypes = [ctypes.POINTER(ReconDesc), _I, ctypes.c_float, _P]
+        L.nsc_segment_nuclei_io.argtypes = [_P, _I, _I, _P, _P, ctypes.POINTER(SegIO), _P, _P, _P, _P]
+        L.nsc_pl_find_nucleus_candidates.argtypes = [_P, ctypes.c_size_t, _I, _I, _P, _P, ctypes.c_size_t, _P, _P]
+        L.nsc_pl_separate_nuclei.argtypes = [_P, ctypes.c_size_t, _P, ctypes.c_size_t, _I, _I, _I, _I, _P,
+                                             ctypes.c_size_t, _P, _P]
+        L.nsc_object_features.argtypes = [_P, _I, _I, _P, _I, _P, _I, _I, ctypes.POINTER(ObjFeaturesOut), _P]
+        L.nsc_watershed_tier_runs.argtypes = [ctypes.POINTER(ctypes.c_longlong), _I]
         L.nsc_recon_autotune.argtypes = [ctypes.POINTER(ReconDesc), _I, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                          ctypes.POINTER(ctypes.c_float)]
         _lib = L

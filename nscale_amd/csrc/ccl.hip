@@ -702,6 +702,74 @@ __global__ void k_bbox(const int32_t* __restrict__ L, int w, int h, int n, int32
 
 inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u)); }
 
+// ---- ConnComponents::boundingBox, literally (ConnComponents.cpp:241-284) -----------------------
+// The reference scans labels in raster order, skips bgval, stops at the first -1 or 0 label, and
+// lists the labels it met in first-seen order.  Here: the stop index, each label's first position
+// (atomicMin), a rank of the first positions (the relabel kernels), then the box kernel on the
+// ranks.  Labels met before the stop must lie in 1 .. w*h-1 (as ConnComponents::label makes them).
+__global__ void k_bb_stop(const int32_t* __restrict__ L, int n, int32_t bgval, int* __restrict__ stop) {
+    int best = INT_MAX;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t v = L[i];
+        if (v != bgval && (v == -1 || v == 0)) {  // a thread's first hit in its stride is its smallest
+            best = i;
+            break;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) best = min(best, __shfl_down(best, off));
+    __shared__ int part[16];
+    const int wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) part[wv] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = INT_MAX;
+        for (int k = 0; k < nw; ++k) b = min(b, part[k]);
+        if (b != INT_MAX) atomicMin(stop, b);
+    }
+}
+__global__ void k_bb_first(const int32_t* __restrict__ L, int n, int32_t bgval, const int* __restrict__ stop,
+                           int* __restrict__ firstpos, int* __restrict__ bad) {
+    const int i0 = min(*stop, n);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < i0; i += gridDim.x * blockDim.x) {
+        const int32_t v = L[i];
+        if (v == bgval) continue;
+        if (v < 1 || v >= n) atomicOr(bad, 1);
+        else atomicMin(&firstpos[v], i);
+    }
+}
+// R[i] = i at first positions (the roots the rank kernels count), -1 elsewhere
+__global__ void k_bb_roots(const int32_t* __restrict__ L, int n, int32_t bgval, const int* __restrict__ stop,
+                           const int* __restrict__ firstpos, int32_t* __restrict__ R) {
+    const int i0 = *stop;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t v = L[i];
+        const bool ok = i < i0 && v != bgval && v >= 1 && v < n;
+        const int f = ok ? firstpos[v] : -1;
+        R[i] = (ok && f == i) ? i : -1;
+    }
+}
+// canon[i] = rank of the first position of L[i] (1..count) over the scanned pixels, -1 elsewhere
+__global__ void k_bb_canon(const int32_t* __restrict__ L, int n, int32_t bgval, const int* __restrict__ stop,
+                           const int* __restrict__ firstpos, const int32_t* __restrict__ rank, int32_t* __restrict__ canon) {
+    const int i0 = *stop;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t v = L[i];
+        const bool ok = i < i0 && v != bgval && v >= 1 && v < n;
+        canon[i] = ok ? rank[firstpos[v]] : -1;
+    }
+}
+// label column: the box of rank k holds the label first seen at the k-th first position
+// (rk: the rank at first positions, 0 elsewhere)
+__global__ void k_bb_labels(const int32_t* __restrict__ L, int n, const int32_t* __restrict__ rk, int32_t* __restrict__ out5) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t k = rk[i];
+        if (k > 0) out5[k - 1] = L[i];
+    }
+}
+__global__ void k_fill_i32(int32_t* p, int32_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -783,6 +851,39 @@ int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int3
     return NSC_OK;
 }
 
+int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s);
+
+// ConnComponents::boundingBox(w, h, labels, bgval, compcount) with the reference's result block
+// (5 x compcount, column stride compcount) in out5 when compcount <= cap.  Scratch: firstpos, R,
+// rank (w*h ints each) and cnt (4 ints).  Synchronises the stream (compcount is a host value).
+int bbox_literal_impl(const int32_t* L, int w, int h, int32_t bgval, int32_t* out5, int cap, int* compcount,
+                      int32_t* firstpos, int32_t* R, int32_t* rank, int* cnt, hipStream_t s) {
+    const int n = w * h;
+    hipLaunchKernelGGL(k_fill_i32, grid1(n), dim3(256), 0, s, firstpos, INT_MAX, (size_t)n);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, cnt, 0, (size_t)4);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, cnt, n, (size_t)1);  // stop = n: no stop
+    hipLaunchKernelGGL(k_bb_stop, grid1(n), dim3(256), 0, s, L, n, bgval, &cnt[0]);
+    hipLaunchKernelGGL(k_bb_first, grid1(n), dim3(256), 0, s, L, n, bgval, &cnt[0], firstpos, &cnt[2]);
+    hipLaunchKernelGGL(k_bb_roots, grid1(n), dim3(256), 0, s, L, n, bgval, &cnt[0], firstpos, R);
+    // ranks of the first positions (relabel's root rank: 1 + first positions before it)
+    NSC_CHECK(relabel_impl(R, w, h, -1, &cnt[1], rank, s));
+    int hc[4] = {0, 0, 0, 0};
+    NSC_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+    NSC_HIP(stream_sync(s));
+    if (hc[2]) {
+        set_error("boundingBox: a label before the first background pixel lies outside 1..w*h-1");
+        return NSC_ERR_ARG;
+    }
+    *compcount = hc[1];
+    if (hc[1] == 0 || !out5 || hc[1] > cap) return NSC_OK;
+    // R now holds the rank at each first position (0 elsewhere); rank[] becomes the rank image
+    hipLaunchKernelGGL(k_bb_canon, grid1(n), dim3(256), 0, s, L, n, bgval, &cnt[0], firstpos, R, rank);
+    NSC_CHECK(bbox_impl(rank, w, h, hc[1], out5, nullptr, s));
+    hipLaunchKernelGGL(k_bb_labels, grid1(n), dim3(256), 0, s, L, n, R, out5);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
 int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s) {
     if (n <= 0) return NSC_OK;
     hipLaunchKernelGGL(k_bbox_init, dim3(cdiv(n, 256)), dim3(256), 0, s, out5, area, n);
@@ -799,6 +900,7 @@ extern "C" {
 
 int nsc_cc_label_u8(const uint8_t* img, int32_t* labels, int w, int h, int32_t bgval, int conn, void* stream) {
     NSC_ARG(img && labels && w > 0 && h > 0 && valid_conn(conn), "cc_label_u8");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     return ccl_u8(img, labels, w, h, conn, true, bgval, as_stream(stream));
 }
@@ -809,6 +911,7 @@ int nsc_bwlabel2_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn,
 
 int nsc_cc_relabel(int32_t* labels, int w, int h, int32_t bgval, int* count, void* stream) {
     NSC_ARG(labels && w > 0 && h > 0, "cc_relabel");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     Scratch sc;
@@ -826,6 +929,7 @@ int nsc_cc_relabel(int32_t* labels, int w, int h, int32_t bgval, int* count, voi
 int nsc_bwareaopen2_u8(const uint8_t* img, uint8_t* out, int w, int h, int minSize, int maxSize, int conn, int* count,
                        void* stream) {
     NSC_ARG(img && out && w > 0 && h > 0 && valid_conn(conn), "bwareaopen2_u8");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     const size_t n = (size_t)w * h;
@@ -845,6 +949,7 @@ int nsc_bwareaopen2_u8(const uint8_t* img, uint8_t* out, int w, int h, int minSi
 int nsc_imreconstruct_binary_u8(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int w, int h, int conn,
                                 void* stream) {
     NSC_ARG(marker && mask && out && w > 0 && h > 0 && valid_conn(conn), "imreconstruct_binary_u8");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     const size_t n = (size_t)w * h;
@@ -862,6 +967,7 @@ int nsc_imreconstruct_binary_u8(const uint8_t* marker, const uint8_t* mask, uint
 
 int nsc_imfill_u8(const uint8_t* img, const uint8_t* seeds, uint8_t* out, int w, int h, int conn, void* stream) {
     NSC_ARG(img && seeds && out && w > 0 && h > 0 && valid_conn(conn), "imfill_u8");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     const size_t n = (size_t)w * h;
@@ -877,6 +983,7 @@ int nsc_imfill_u8(const uint8_t* img, const uint8_t* seeds, uint8_t* out, int w,
 
 int nsc_imfill_holes_u8(const uint8_t* img, uint8_t* out, int w, int h, int conn, void* stream) {
     NSC_ARG(img && out && w > 0 && h > 0 && valid_conn(conn), "imfill_holes_u8");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     const size_t n = (size_t)w * h;
@@ -888,6 +995,7 @@ int nsc_imfill_holes_u8(const uint8_t* img, uint8_t* out, int w, int h, int conn
 
 int nsc_bwselect_u8(const uint8_t* bw, const uint8_t* seeds, uint8_t* out, int w, int h, int conn, void* stream) {
     NSC_ARG(bw && seeds && out && w > 0 && h > 0 && valid_conn(conn), "bwselect_u8");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     const size_t n = (size_t)w * h;
@@ -897,8 +1005,23 @@ int nsc_bwselect_u8(const uint8_t* bw, const uint8_t* seeds, uint8_t* out, int w
     return bwselect_impl(bw, seeds, out, w, h, conn, labels, reinterpret_cast<uint8_t*>(labels + n), s);
 }
 
+int nsc_cc_bounding_box(const int32_t* labels, int w, int h, int32_t bgval, int32_t* out5, int cap, int* compcount,
+                        void* stream) {
+    NSC_ARG(labels && compcount && w > 0 && h > 0 && cap >= 0, "cc_bounding_box");
+    NSC_LABEL_SIZE(w, h);
+    NSC_CHECK(ensure_init());
+    hipStream_t s = as_stream(stream);
+    const size_t n = (size_t)w * h;
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(int32_t) * (3 * n + 4), s));
+    int32_t* firstpos = sc.as<int32_t>();
+    return bbox_literal_impl(labels, w, h, bgval, out5, cap, compcount, firstpos, firstpos + n, firstpos + 2 * n,
+                             reinterpret_cast<int*>(firstpos + 3 * n), s);
+}
+
 int nsc_bounding_boxes(const int32_t* canon_labels, int w, int h, int n, int32_t* out5, int32_t* area, void* stream) {
     NSC_ARG(canon_labels && out5 && w > 0 && h > 0 && n >= 0, "bounding_boxes");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     return bbox_impl(canon_labels, w, h, n, out5, area, as_stream(stream));
 }

@@ -30,6 +30,9 @@ size_t ws2_scratch(int w, int h);
 int ws2_from_labels(const uint8_t* bgr, const int32_t* labels, int32_t* out, int w, int h, void* scratch, hipStream_t s);
 int ccl_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn, bool equal_values, int32_t bgval, hipStream_t s);
 int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int32_t* rank, hipStream_t s);
+int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s);
+int bbox_literal_impl(const int32_t* L, int w, int h, int32_t bgval, int32_t* out5, int cap, int* compcount,
+                      int32_t* firstpos, int32_t* R, int32_t* rank, int* cnt, hipStream_t s);
 
 namespace {
 
@@ -273,6 +276,7 @@ void nsc_seg_params_default(nsc_seg_params* p) {
 
 int nsc_workspace_create(int max_w, int max_h, nsc_workspace** ws) {
     NSC_ARG(ws && max_w > 0 && max_h > 0, "workspace_create");
+    NSC_LABEL_SIZE(max_w, max_h);
     NSC_CHECK(ensure_init());
     auto* W = new nsc_workspace();
     int rc = ws_alloc(*W, max_w, max_h);
@@ -289,37 +293,51 @@ int nsc_workspace_destroy(nsc_workspace* ws) {
     return NSC_OK;
 }
 
-int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* prm, int32_t* labels, int* compcount,
-                       nsc_workspace* ws, void** stage, double* stage_ms, void* stream) {
-    if (!bgr || w <= 0 || h <= 0) return NSC_SEG_INVALID_IMAGE;
-    NSC_ARG(labels && prm && ws, "segment_nuclei");
-    NSC_ARG(workspace_fits(ws, w, h), "tile larger than the workspace");
-    NSC_ARG(valid_conn(prm->fillHolesConnectivity) && valid_conn(prm->reconConnectivity) &&
-                valid_conn(prm->watershedConnectivity), "connectivity");
-    NSC_CHECK(ensure_init());
-    hipStream_t s = as_stream(stream);
-    Workspace& W = *ws;
-    const size_t n = (size_t)w * h;
-    const nsc_seg_params& p = *prm;
-    W.arena.top = 0;
-    W.arena.depth = 0;
-    ArenaScope arena_scope(&W.arena);
+}  // extern "C"
 
-    // ---- optional per-stage timing (HIP events on the stream) ----
+// ------------------------------------------------------------------------------------------
+// The chain, stage by stage.  A call's context carries the workspace, the stream and the optional
+// per-stage dumps and timers (the reference's IntermediateResultHandler ids and logger stages).
+namespace nsc {
+namespace {
+
+// label column of a box block built on canonical labels: the box of canonical label k = canon[r]
+// of root r (L[r] == r) gets label r
+__global__ void k_root_label_column(const int32_t* __restrict__ L, const int32_t* __restrict__ canon, int n,
+                                    int32_t* __restrict__ out5) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (L[i] == i) {
+            const int k = canon[i];
+            if (k >= 1) out5[k - 1] = i;
+        }
+    }
+}
+
+struct Chain {
+    Workspace& W;
+    hipStream_t s;
+    void** stage;
+    double* stage_ms;
     std::vector<std::pair<int, hipEvent_t>> marks;
-    auto mark = [&](int id) {
+    Chain(Workspace& w, hipStream_t st, void** sg, double* ms) : W(w), s(st), stage(sg), stage_ms(ms) {}
+    ~Chain() {
+        for (auto& m : marks) (void)hipEventDestroy(m.second);
+    }
+    // timing mark: the time since the previous mark is charged to stage id
+    void mark(int id) {
         if (!stage_ms) return;
         hipEvent_t e;
         if (hipEventCreate(&e) == hipSuccess) {
             (void)hipEventRecord(e, s);
             marks.push_back({id, e});
         }
-    };
-    auto save = [&](int id, const void* src, size_t bytes) -> int {
+    }
+    int save(int id, const void* src, size_t bytes) {
         if (stage && stage[id]) NSC_HIP(hipMemcpyAsync(stage[id], src, bytes, hipMemcpyDeviceToDevice, s));
         return NSC_OK;
-    };
-    auto finish_timing = [&]() {
+    }
+    bool want(int id) const { return stage && stage[id]; }
+    void finish_timing() {
         if (!stage_ms || marks.empty()) return;
         (void)hipEventSynchronize(marks.back().second);
         for (size_t k = 1; k < marks.size(); ++k) {
@@ -328,112 +346,323 @@ int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* p
             stage_ms[marks[k].first] += ms;
         }
         for (auto& m : marks) (void)hipEventDestroy(m.second);
-    };
-    mark(0);
-    NSC_CHECK(save(0, bgr, 3 * n));
+        marks.clear();
+    }
+};
 
-    // ---- S1: plFindNucleusCandidates ----
+// plFindNucleusCandidates (HistologicalEntities.cpp:115-304): candidates in W.seg_norbc.
+// Returns NSC_SEG_CONTINUE or the early status (background, no candidates).
+int stage_s1(Chain& C, const uint8_t* bgr, int w, int h, const nsc_seg_params& p) {
+    Workspace& W = C.W;
+    hipStream_t s = C.s;
+    const size_t n = (size_t)w * h;
+    C.mark(0);
+    NSC_CHECK(C.save(0, bgr, 3 * n));
     NSC_HIP(hipMemsetAsync(W.counts, 0, 4 * sizeof(int), s));
     hipLaunchKernelGGL(k_s1_pixels, grid_counted(cdiv(n, 4)), dim3(256), 0, s, bgr, n, p.blue, p.green, p.red, (float)p.T1,
-                       (float)p.T2, (stage && stage[1]) ? W.bgm : nullptr, W.bw1, W.bw2, W.r2b, W.rc, &W.counts[0],
+                       (float)p.T2, C.want(1) ? W.bgm : nullptr, W.bw1, W.bw2, W.r2b, W.rc, &W.counts[0],
                        (reinterpret_cast<uintptr_t>(bgr) & 3) == 0 ? 1 : 0);
     NSC_HIP(hipGetLastError());
     int bgArea = 0;
     NSC_HIP(hipMemcpyAsync(&bgArea, &W.counts[0], sizeof(int), hipMemcpyDeviceToHost, s));
     NSC_HIP(stream_sync(s));
     const float ratio = (float)bgArea / (float)(w * h);  // HE.cpp:125
-    mark(1);
-    if (ratio >= 0.99) { finish_timing(); return NSC_SEG_BACKGROUND; }
-    if (ratio >= 0.9) { finish_timing(); return NSC_SEG_BACKGROUND_LIKELY; }
-    NSC_CHECK(save(1, W.bgm, n));
+    C.mark(1);
+    if (ratio >= 0.99) return NSC_SEG_BACKGROUND;
+    if (ratio >= 0.9) return NSC_SEG_BACKGROUND_LIKELY;
+    NSC_CHECK(C.save(1, W.bgm, n));
 
     // getRBC: bwselect(bw2, bw1, 8) & imR2B -- an empty bw1 selects nothing, which is the
     // reference's zero branch (HE.cpp:74-80)
     NSC_CHECK(bwselect_impl(W.bw2, W.bw1, W.rbcsel, w, h, 8, W.L, W.tmp8, s));
-    mark(2);
+    C.mark(2);
     // rc_open = morphOpen(rc, disk19); rc_recon = imreconstruct(rc_open, rc, reconConnectivity)
     NSC_CHECK(morph_open_impl(W.rc, W.rc_open, W.tmp8, w, h, kDisk19, 19, s));
-    NSC_CHECK(save(3, W.rc_open, n));
-    mark(3);
+    NSC_CHECK(C.save(3, W.rc_open, n));
+    C.mark(3);
     NSC_CHECK(recon_impl<uint8_t>(W.rc_open, W.rc, W.rc_recon, w, h, p.reconConnectivity, s));
-    mark(26);  // timing id 26: the imreconstruct<uchar>(rc_open, rc) call alone
-    NSC_CHECK(save(4, W.rc_recon, n));
+    C.mark(26);  // timing id 26: the imreconstruct<uchar>(rc_open, rc) call alone
+    NSC_CHECK(C.save(4, W.rc_recon, n));
     hipLaunchKernelGGL(k_diff_thr, grid1(n), dim3(256), 0, s, W.rc, W.rc_recon, n, p.G1, p.G2, W.diff, W.t1, W.t2);
-    NSC_CHECK(save(5, W.diff, n));
-    NSC_CHECK(save(6, W.t1, n));
-    mark(4);
+    NSC_CHECK(C.save(5, W.diff, n));
+    NSC_CHECK(C.save(6, W.t1, n));
+    C.mark(4);
     NSC_CHECK(fill_holes_impl(W.t1, W.fill1, w, h, p.fillHolesConnectivity, W.L, W.tmp8, s));
-    NSC_CHECK(save(7, W.fill1, n));
-    mark(7);
+    NSC_CHECK(C.save(7, W.fill1, n));
+    C.mark(7);
     NSC_CHECK(bwareaopen_impl(W.fill1, W.bw1_t, w, h, p.minSize, p.maxSize, 8, &W.counts[1], W.L, W.area, s));
-    NSC_CHECK(save(8, W.bw1_t, n));
-    NSC_CHECK(save(9, W.t2, n));
-    mark(8);
+    NSC_CHECK(C.save(8, W.bw1_t, n));
+    NSC_CHECK(C.save(9, W.t2, n));
+    C.mark(8);
     NSC_CHECK(bwselect_impl(W.t2, W.bw1_t, W.seg_norbc, w, h, 8, W.L, W.tmp8, s));
-    NSC_CHECK(save(10, W.seg_norbc, n));
-    hipLaunchKernelGGL(k_norbc, grid1(n), dim3(256), 0, s, W.seg_norbc, W.rbcsel, W.r2b,
-                       (stage && stage[2]) ? W.rbc : nullptr, n);
-    NSC_CHECK(save(2, W.rbc, n));
-    NSC_CHECK(save(11, W.seg_norbc, n));
-    mark(10);
+    NSC_CHECK(C.save(10, W.seg_norbc, n));
+    hipLaunchKernelGGL(k_norbc, grid1(n), dim3(256), 0, s, W.seg_norbc, W.rbcsel, W.r2b, C.want(2) ? W.rbc : nullptr, n);
+    NSC_CHECK(C.save(2, W.rbc, n));
+    NSC_CHECK(C.save(11, W.seg_norbc, n));
+    C.mark(10);
+    int kept = 0;  // bwareaopen2(fill1, 11..1000) count (HE.cpp:272-274)
+    NSC_HIP(hipMemcpyAsync(&kept, &W.counts[1], sizeof(int), hipMemcpyDeviceToHost, s));
+    NSC_HIP(stream_sync(s));
+    return kept == 0 ? NSC_SEG_NO_CANDIDATES_LEFT : NSC_SEG_CONTINUE;
+}
 
-    // ---- segmentNuclei body ----
-    NSC_CHECK(fill_holes_impl(W.seg_norbc, W.seg_nohole, w, h, 4, W.L, W.tmp8, s));  // HE.cpp:494
-    NSC_CHECK(save(12, W.seg_nohole, n));
-    mark(12);
-    NSC_CHECK(morph_open_impl(W.seg_nohole, W.seg_open, W.tmp8, w, h, kCross3, 3, s));  // HE.cpp:498-499
-    NSC_CHECK(save(13, W.seg_open, n));
-    mark(13);
-
-    // ---- A4: plSeparateNuclei ----
-    NSC_CHECK(bwareaopen_impl(W.seg_open, W.seg_big_t, w, h, p.minSizePl, INT_MAX, 8, nullptr, W.L, W.area, s));
-    NSC_CHECK(save(14, W.seg_big_t, n));
-    mark(14);
+// plSeparateNuclei (HistologicalEntities.cpp:308-449) on seg_open: seg_nonoverlap in W.nonov.
+int stage_a4(Chain& C, const uint8_t* bgr, const uint8_t* seg_open, int w, int h, int minSizePl, int wsConn) {
+    Workspace& W = C.W;
+    hipStream_t s = C.s;
+    const size_t n = (size_t)w * h;
+    NSC_CHECK(bwareaopen_impl(seg_open, W.seg_big_t, w, h, minSizePl, INT_MAX, 8, nullptr, W.L, W.area, s));
+    NSC_CHECK(C.save(14, W.seg_big_t, n));
+    C.mark(14);
     NSC_CHECK(morph_impl(W.seg_big_t, W.seg_big, w, h, kCross3, 3, 0, false, s));  // HE.cpp:330
-    NSC_CHECK(save(15, W.seg_big, n));
-    mark(15);
+    NSC_CHECK(C.save(15, W.seg_big, n));
+    C.mark(15);
     NSC_CHECK(edt_impl(W.seg_big, W.dist, w, h, s));  // HE.cpp:359
-    NSC_CHECK(save(16, W.dist, 4 * n));
-    hipLaunchKernelGGL(k_dist_prep, grid1(n), dim3(256), 0, s, W.dist, W.seg_big, bgr, n,
-                       (stage && stage[17]) ? W.distance : nullptr, W.hmask, W.hmarker, W.nuclei);
-    NSC_CHECK(save(17, W.distance, 4 * n));
-    mark(16);
+    NSC_CHECK(C.save(16, W.dist, 4 * n));
+    hipLaunchKernelGGL(k_dist_prep, grid1(n), dim3(256), 0, s, W.dist, W.seg_big, bgr, n, C.want(17) ? W.distance : nullptr,
+                       W.hmask, W.hmarker, W.nuclei);
+    NSC_CHECK(C.save(17, W.distance, 4 * n));
+    C.mark(16);
     NSC_CHECK(recon_impl<float>(W.hmarker, W.hmask, W.d2, w, h, 8, s));  // imhmin(distance, 1, 8) HE.cpp:384
     hipLaunchKernelGGL(k_negate_f, grid1(n), dim3(256), 0, s, W.d2, n);
-    NSC_CHECK(save(18, W.d2, 4 * n));
-    NSC_CHECK(save(19, W.nuclei, 3 * n));
-    mark(18);
+    NSC_CHECK(C.save(18, W.d2, 4 * n));
+    NSC_CHECK(C.save(19, W.nuclei, 3 * n));
+    C.mark(18);
     // watershed2(nuclei, distance2, watershedConnectivity)  HE.cpp:414
-    NSC_CHECK(local_max_impl(W.d2, W.tmp8, w, h, p.watershedConnectivity, true, W.big, s));
-    NSC_CHECK(ccl_u8(W.tmp8, W.L, w, h, p.watershedConnectivity, true, -1, s));
+    NSC_CHECK(local_max_impl(W.d2, W.tmp8, w, h, wsConn, true, W.big, s));
+    NSC_CHECK(ccl_u8(W.tmp8, W.L, w, h, wsConn, true, -1, s));
     NSC_CHECK(ws2_from_labels(W.nuclei, W.L, W.wm, w, h, W.big, s));
-    NSC_CHECK(save(20, W.wm, 4 * n));
+    NSC_CHECK(C.save(20, W.wm, 4 * n));
     hipLaunchKernelGGL(k_nonoverlap, grid1(n), dim3(256), 0, s, W.seg_big, W.wm, W.nonov, n);
-    NSC_CHECK(save(21, W.nonov, n));
-    mark(20);
+    NSC_CHECK(C.save(21, W.nonov, n));
+    C.mark(20);
+    NSC_HIP(hipGetLastError());
+    return NSC_SEG_CONTINUE;
+}
+
+// boxes of the final labels (contiguous lab) as io->bbox_mode asks; nuclei = number of roots.
+// Called after the chain's last synchronisation, with W.L holding the canonical relabel of lab
+// unless lab itself was relabelled (io->canonical).
+int chain_boxes(Chain& C, int32_t* lab, int w, int h, nsc_seg_io* io, int nuclei) {
+    Workspace& W = C.W;
+    hipStream_t s = C.s;
+    const int n = w * h;
+    io->compcount = 0;
+    if (io->bbox_mode == NSC_BBOX_REFERENCE_CPU) {
+        // ConnComponents::boundingBox(labels, bgval 0) on the reference CPU labels (HE.cpp:590-591)
+        int cnt = 0;
+        NSC_CHECK(bbox_literal_impl(lab, w, h, 0, io->bbox5, io->max_boxes, &cnt, W.L, W.area, W.wm, &W.counts[4], s));
+        io->compcount = cnt;
+        if (cnt > io->max_boxes || (cnt > 0 && !io->bbox5)) {
+            set_error("segment_nuclei: %d boxes exceed the capacity of %d", cnt, io->max_boxes);
+            return NSC_ERR_LIMIT;
+        }
+        NSC_HIP(stream_sync(s));
+        return NSC_OK;
+    }
+    if (io->bbox_mode == NSC_BBOX_ALL) {
+        io->compcount = nuclei;
+        if (nuclei == 0) return NSC_OK;
+        if (nuclei > io->max_boxes || !io->bbox5) {
+            set_error("segment_nuclei: %d boxes exceed the capacity of %d", nuclei, io->max_boxes);
+            return NSC_ERR_LIMIT;
+        }
+        if (io->canonical) {
+            NSC_CHECK(bbox_impl(lab, w, h, nuclei, io->bbox5, io->area, s));
+        } else {
+            // W.L holds the canonical copy the nuclei count was taken from
+            NSC_CHECK(bbox_impl(W.L, w, h, nuclei, io->bbox5, io->area, s));
+            hipLaunchKernelGGL(k_root_label_column, grid1(n), dim3(256), 0, s, lab, W.L, n, io->bbox5);
+        }
+        NSC_HIP(hipGetLastError());
+        NSC_HIP(stream_sync(s));
+    }
+    return NSC_OK;
+}
+
+int segment_body(const uint8_t* bgr_in, int w, int h, const nsc_seg_params& p, int32_t* labels_out, nsc_seg_io* io,
+                 Workspace& W, void** stage, double* stage_ms, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    W.arena.top = 0;
+    W.arena.depth = 0;
+    ArenaScope arena_scope(&W.arena);
+    Chain C(W, s, stage, stage_ms);
+    const size_t bgr_step = (io && io->bgr_step) ? io->bgr_step : (size_t)3 * w;
+    const size_t lab_step = (io && io->labels_step) ? io->labels_step : (size_t)4 * w;
+    NSC_ARG(bgr_step >= (size_t)3 * w && lab_step >= (size_t)4 * w && lab_step % 4 == 0, "segment_nuclei: row steps");
+    // pitched images: continuous copies in the workspace arena (released in reverse order at return)
+    Scratch bgr_c, lab_c;
+    const uint8_t* bgr = bgr_in;
+    if (bgr_step != (size_t)3 * w) {
+        NSC_CHECK(bgr_c.alloc(3 * n, s));
+        NSC_HIP(hipMemcpy2DAsync(bgr_c.p, (size_t)3 * w, bgr_in, bgr_step, (size_t)3 * w, h, hipMemcpyDeviceToDevice, s));
+        bgr = bgr_c.as<uint8_t>();
+    }
+    int32_t* labels = labels_out;
+    if (lab_step != (size_t)4 * w) {
+        NSC_CHECK(lab_c.alloc(4 * n, s));
+        labels = lab_c.as<int32_t>();
+    }
+
+    int st = stage_s1(C, bgr, w, h, p);
+    if (st != NSC_SEG_CONTINUE) {
+        C.finish_timing();
+        return st;
+    }
+    // ---- segmentNuclei body (HE.cpp:494-499) ----
+    NSC_CHECK(fill_holes_impl(W.seg_norbc, W.seg_nohole, w, h, 4, W.L, W.tmp8, s));  // HE.cpp:494
+    NSC_CHECK(C.save(12, W.seg_nohole, n));
+    C.mark(12);
+    NSC_CHECK(morph_open_impl(W.seg_nohole, W.seg_open, W.tmp8, w, h, kCross3, 3, s));  // HE.cpp:498-499
+    NSC_CHECK(C.save(13, W.seg_open, n));
+    C.mark(13);
+    st = stage_a4(C, bgr, W.seg_open, w, h, p.minSizePl, p.watershedConnectivity);
+    if (st != NSC_SEG_CONTINUE) return st;
 
     NSC_CHECK(bwareaopen_impl(W.nonov, W.seg, w, h, p.minSizeSeg, p.maxSizeSeg, 4, &W.counts[2], W.L, W.area, s));  // :548
-    NSC_CHECK(save(23, W.seg, n));
-    mark(23);
+    NSC_CHECK(C.save(23, W.seg, n));
+    C.mark(23);
     NSC_CHECK(fill_holes_impl(W.seg, W.fin, w, h, p.fillHolesConnectivity, W.L, W.tmp8, s));  // :573
-    NSC_CHECK(save(24, W.fin, n));
-    mark(24);
+    NSC_CHECK(C.save(24, W.fin, n));
+    C.mark(24);
     NSC_CHECK(ccl_u8(W.fin, labels, w, h, 8, true, -1, s));  // bwlabel2(final, 8) :585
-    NSC_CHECK(save(25, labels, 4 * n));
-    if (compcount) {  // number of nuclei = number of roots
+    NSC_CHECK(C.save(25, labels, 4 * n));
+    // number of nuclei = number of roots (rank scratch in W.area)
+    const bool canonical = io && io->canonical;
+    const bool cpu_boxes = io && io->bbox_mode == NSC_BBOX_REFERENCE_CPU;
+    if (canonical && !cpu_boxes) {
+        NSC_CHECK(relabel_impl(labels, w, h, -1, &W.counts[3], W.area, s));
+    } else {
         NSC_HIP(hipMemcpyAsync(W.L, labels, 4 * n, hipMemcpyDeviceToDevice, s));
         NSC_CHECK(relabel_impl(W.L, w, h, -1, &W.counts[3], W.area, s));
     }
-    mark(25);
+    C.mark(25);
     NSC_HIP(hipGetLastError());
     int c[4] = {0, 0, 0, 0};
     NSC_HIP(hipMemcpyAsync(c, W.counts, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
     NSC_HIP(stream_sync(s));
-    finish_timing();
-    if (c[1] == 0) return NSC_SEG_NO_CANDIDATES_LEFT;  // HE.cpp:272-274
+    C.finish_timing();
     if (c[2] == 0) return NSC_SEG_NO_CANDIDATES_LEFT;  // HE.cpp:554-556
-    if (compcount) *compcount = c[3];
+    if (io) {
+        io->nuclei = c[3];
+        NSC_CHECK(chain_boxes(C, labels, w, h, io, c[3]));
+        if (canonical && cpu_boxes) NSC_CHECK(relabel_impl(labels, w, h, -1, &W.counts[3], W.area, s));
+    }
+    if (labels != labels_out)
+        NSC_HIP(hipMemcpy2DAsync(labels_out, lab_step, labels, (size_t)4 * w, (size_t)4 * w, h, hipMemcpyDeviceToDevice, s));
+    NSC_HIP(stream_sync(s));
     return NSC_SEG_SUCCESS;
+}
+
+// A failed call may leave work queued on the stream against the workspace: wait for it before
+// returning, so a retry (possibly on another stream) cannot reuse buffers still in use.
+int settle(int rc, hipStream_t s) {
+    if (rc <= NSC_ERR_ARG) (void)hipStreamSynchronize(s);
+    return rc;
+}
+
+int check_chain_args(const uint8_t* bgr, int w, int h, const nsc_seg_params* prm, const nsc_workspace* ws) {
+    NSC_ARG(bgr && prm && ws && w > 0 && h > 0, "segment_nuclei");
+    NSC_ARG(workspace_fits(ws, w, h), "tile larger than the workspace");
+    NSC_LABEL_SIZE(w, h);
+    NSC_ARG(valid_conn(prm->fillHolesConnectivity) && valid_conn(prm->reconConnectivity) &&
+                valid_conn(prm->watershedConnectivity), "connectivity");
+    return ensure_init();
+}
+
+}  // namespace
+}  // namespace nsc
+
+extern "C" {
+
+int nsc_segment_nuclei_io(const uint8_t* bgr, int w, int h, const nsc_seg_params* prm, int32_t* labels, nsc_seg_io* io,
+                          nsc_workspace* ws, void** stage, double* stage_ms, void* stream) {
+    if (!bgr || w <= 0 || h <= 0) return NSC_SEG_INVALID_IMAGE;  // HE.cpp:477 (no image data)
+    NSC_ARG(labels, "segment_nuclei: labels");
+    NSC_CHECK(check_chain_args(bgr, w, h, prm, ws));
+    NSC_ARG(!io || (io->bbox_mode >= NSC_BBOX_NONE && io->bbox_mode <= NSC_BBOX_ALL), "segment_nuclei: bbox_mode");
+    if (io) {
+        io->compcount = 0;
+        io->nuclei = 0;
+    }
+    hipStream_t s = as_stream(stream);
+    return settle(segment_body(bgr, w, h, *prm, labels, io, *ws, stage, stage_ms, s), s);
+}
+
+int nsc_segment_nuclei(const uint8_t* bgr, int w, int h, const nsc_seg_params* prm, int32_t* labels, int* compcount,
+                       nsc_workspace* ws, void** stage, double* stage_ms, void* stream) {
+    nsc_seg_io io;
+    memset(&io, 0, sizeof(io));
+    const int st = nsc_segment_nuclei_io(bgr, w, h, prm, labels, &io, ws, stage, stage_ms, stream);
+    if (compcount && st == NSC_SEG_SUCCESS) *compcount = io.nuclei;
+    return st;
+}
+
+int nsc_pl_find_nucleus_candidates(const uint8_t* bgr, size_t bgr_step, int w, int h, const nsc_seg_params* prm,
+                                   uint8_t* seg_norbc, size_t seg_step, nsc_workspace* ws, void* stream) {
+    if (!bgr || w <= 0 || h <= 0) return NSC_SEG_INVALID_IMAGE;
+    NSC_ARG(seg_norbc, "plFindNucleusCandidates: seg_norbc");
+    NSC_CHECK(check_chain_args(bgr, w, h, prm, ws));
+    if (!bgr_step) bgr_step = (size_t)3 * w;
+    if (!seg_step) seg_step = (size_t)w;
+    NSC_ARG(bgr_step >= (size_t)3 * w && seg_step >= (size_t)w, "plFindNucleusCandidates: row steps");
+    hipStream_t s = as_stream(stream);
+    Workspace& W = *ws;
+    auto body = [&]() -> int {
+        W.arena.top = 0;
+        W.arena.depth = 0;
+        ArenaScope arena_scope(&W.arena);
+        Chain C(W, s, nullptr, nullptr);
+        Scratch bgr_c;
+        const uint8_t* in = bgr;
+        if (bgr_step != (size_t)3 * w) {
+            NSC_CHECK(bgr_c.alloc((size_t)3 * w * h, s));
+            NSC_HIP(hipMemcpy2DAsync(bgr_c.p, (size_t)3 * w, bgr, bgr_step, (size_t)3 * w, h, hipMemcpyDeviceToDevice, s));
+            in = bgr_c.as<uint8_t>();
+        }
+        const int st = stage_s1(C, in, w, h, *prm);
+        if (st != NSC_SEG_CONTINUE) return st;
+        NSC_HIP(hipMemcpy2DAsync(seg_norbc, seg_step, W.seg_norbc, (size_t)w, (size_t)w, h, hipMemcpyDeviceToDevice, s));
+        NSC_HIP(stream_sync(s));
+        return st;
+    };
+    return settle(body(), s);
+}
+
+int nsc_pl_separate_nuclei(const uint8_t* bgr, size_t bgr_step, const uint8_t* seg_open, size_t open_step, int w, int h,
+                           int minSizePl, int watershedConnectivity, uint8_t* seg_nonoverlap, size_t out_step,
+                           nsc_workspace* ws, void* stream) {
+    if (!bgr || w <= 0 || h <= 0) return NSC_SEG_INVALID_IMAGE;
+    NSC_ARG(seg_open && seg_nonoverlap && ws && valid_conn(watershedConnectivity), "plSeparateNuclei");
+    NSC_ARG(workspace_fits(ws, w, h), "tile larger than the workspace");
+    NSC_LABEL_SIZE(w, h);
+    NSC_CHECK(ensure_init());
+    if (!bgr_step) bgr_step = (size_t)3 * w;
+    if (!open_step) open_step = (size_t)w;
+    if (!out_step) out_step = (size_t)w;
+    NSC_ARG(bgr_step >= (size_t)3 * w && open_step >= (size_t)w && out_step >= (size_t)w, "plSeparateNuclei: row steps");
+    hipStream_t s = as_stream(stream);
+    Workspace& W = *ws;
+    auto body = [&]() -> int {
+        W.arena.top = 0;
+        W.arena.depth = 0;
+        ArenaScope arena_scope(&W.arena);
+        Chain C(W, s, nullptr, nullptr);
+        Scratch bgr_c;
+        const uint8_t* in = bgr;
+        if (bgr_step != (size_t)3 * w) {
+            NSC_CHECK(bgr_c.alloc((size_t)3 * w * h, s));
+            NSC_HIP(hipMemcpy2DAsync(bgr_c.p, (size_t)3 * w, bgr, bgr_step, (size_t)3 * w, h, hipMemcpyDeviceToDevice, s));
+            in = bgr_c.as<uint8_t>();
+        }
+        NSC_HIP(hipMemcpy2DAsync(W.seg_open, (size_t)w, seg_open, open_step, (size_t)w, h, hipMemcpyDeviceToDevice, s));
+        const int st = stage_a4(C, in, W.seg_open, w, h, minSizePl, watershedConnectivity);
+        if (st != NSC_SEG_CONTINUE) return st;
+        NSC_HIP(hipMemcpy2DAsync(seg_nonoverlap, out_step, W.nonov, (size_t)w, (size_t)w, h, hipMemcpyDeviceToDevice, s));
+        NSC_HIP(stream_sync(s));
+        return st;
+    };
+    return settle(body(), s);
 }
 
 }  // extern "C"

@@ -39,6 +39,11 @@ void clear_error();
         }                                                    \
     } while (0)
 
+// int32 labels and root indices address pixels, so entry points that produce or consume them
+// reject images of INT_MAX pixels or more
+#define NSC_LABEL_SIZE(w, h) \
+    NSC_ARG((size_t)(w) * (size_t)(h) < (size_t)0x7FFFFFFF, "image too large for int32 labels (w*h must be below 2^31-1)")
+
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 int ensure_init();
@@ -275,7 +280,5 @@ template <typename T>
 int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s);
 template <typename T>
 int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s);
-// per-thread reconstruction schedule: -1 default, 0 one launch per pass, 1 one cooperative launch
-void set_thread_recon_schedule(int schedule);
 
 }  // namespace nsc

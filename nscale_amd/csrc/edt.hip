@@ -208,8 +208,13 @@ __global__ __launch_bounds__(256) void k_row_envelope(const uint16_t* __restrict
 }  // namespace
 
 int edt_impl(const uint8_t* img, float* out, int w, int h, hipStream_t s) {
+    // size limits are checked before anything is queued on the stream
     if (h >= 0xFFFF) {
         set_error("distance transform supports heights below 65535");
+        return NSC_ERR_LIMIT;
+    }
+    if (w > 32768) {
+        set_error("distance transform supports widths up to 32768");
         return NSC_ERR_LIMIT;
     }
     const int nseg = (int)cdiv(h, SEG);
@@ -240,10 +245,6 @@ int edt_impl(const uint8_t* img, float* out, int w, int h, hipStream_t s) {
     hipLaunchKernelGGL(k_seg_carry, dim3(cdiv(w, 64)), dim3(64), 0, s, fz, lz, w, nseg, above, below);
     hipLaunchKernelGGL(k_col_dist, gb, dim3(64), 0, s, img, w, h, above, below, g);
     const size_t lds = sizeof(uint16_t) * w;
-    if (w > 32768) {
-        set_error("distance transform supports widths up to 32768");
-        return NSC_ERR_LIMIT;
-    }
     hipLaunchKernelGGL(k_row_search, dim3(h), dim3(256), lds, s, g, w, h, out, rows, nrows);
     // rows whose search was cut short (distances beyond SEARCH_CAP): exact envelope; the launch
     // is unconditional and its blocks beyond the flagged count exit at once (no host sync)

@@ -215,6 +215,7 @@ int nsc_imhmin_f32(const float* img, float* out, int w, int h, float hval, int c
 
 int nsc_local_maxima_f32(const float* img, uint8_t* out, int w, int h, int conn, void* stream) {
     NSC_ARG(img && out && w > 0 && h > 0 && valid_conn(conn), "local_maxima_f32");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     Scratch sc;
@@ -224,6 +225,7 @@ int nsc_local_maxima_f32(const float* img, uint8_t* out, int w, int h, int conn,
 
 int nsc_local_minima_f32(const float* img, uint8_t* out, int w, int h, int conn, void* stream) {
     NSC_ARG(img && out && w > 0 && h > 0 && valid_conn(conn), "local_minima_f32");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     Scratch sc;

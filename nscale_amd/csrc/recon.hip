@@ -20,10 +20,9 @@
 //    the output: 3 bytes/px for uint8, the algorithmic minimum.  A tile whose owned border
 //    changed marks the neighbours whose halo it feeds; later passes re-run only marked tiles
 //    (reading out + mask) until no tile is marked.
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cstdlib>
 #include <mutex>
 #include <type_traits>
@@ -72,6 +71,23 @@ __device__ __forceinline__ void mark_neighbours(int t, unsigned b, int cx, int c
     const int nx = cx + dx, ny = cy + dy;
     if (need && nx >= 0 && nx < nx_max && ny >= 0 && ny < ny_max)
         mark_tile(base + ny * nx_max + nx, next_pass, stamp, next_list, next_count);
+}
+
+// Bitmap form of mark_neighbours for the work-queue drain: sets the dirty bit of each neighbour
+// whose halo the changed borders in b feed (global tile id = base + tile index).
+__device__ __forceinline__ void mark_neighbours_bits(int t, unsigned b, int cx, int cy, int nx_max, int ny_max, int base,
+                                                     bool edges_only, unsigned long long* dirty) {
+    if (t >= 8) return;
+    const int k = t < 4 ? t : t + 1;
+    const int dx = k % 3 - 1, dy = k / 3 - 1;
+    if (edges_only && dx != 0 && dy != 0) return;
+    const bool need = (dy == -1 && (b & CH_TOP)) || (dy == 1 && (b & CH_BOT)) || (dx == -1 && (b & CH_LEFT)) ||
+                      (dx == 1 && (b & CH_RIGHT));
+    const int nx = cx + dx, ny = cy + dy;
+    if (need && nx >= 0 && nx < nx_max && ny >= 0 && ny < ny_max) {
+        const int g = base + ny * nx_max + nx;
+        atomicOr(&dirty[g >> 6], 1ull << (g & 63));
+    }
 }
 
 // one image of a reconstruction launch: its planes, size, connectivity and first global tile id
@@ -255,7 +271,8 @@ __device__ __forceinline__ auto col_sum(const R (&v)[TH]) {
 // one tile of one image: load window, sweep to stability, write owned pixels, mark neighbours
 template <typename T, int CONN>
 __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass, RShared<T>& sh, int* __restrict__ next_list,
-                                           int* __restrict__ next_count, unsigned* __restrict__ stamp) {
+                                           int* __restrict__ next_count, unsigned* __restrict__ stamp,
+                                           unsigned long long* dirty = nullptr) {
     using R = typename RegOf<T>::type;
     constexpr bool U8 = std::is_same<T, uint8_t>::value;
     auto& lm = sh.lm;
@@ -367,10 +384,14 @@ __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass
         for (int y = 1; y < TH - 1; ++y)
             if (v[y] != init_r[y]) bits |= CH_RIGHT;
     }
+    if (dirty) __threadfence();  // the owned pixels are visible before any neighbour is marked
     if (bits) atomicOr(&chg_bits, bits);
     __syncthreads();
-    mark_neighbours(lane, chg_bits, tx, ty, tiles_x, tiles_y, im.base, false, (unsigned)pass + 1, stamp, next_list,
-                    next_count);
+    if (dirty)
+        mark_neighbours_bits(lane, chg_bits, tx, ty, tiles_x, tiles_y, im.base, false, dirty);
+    else
+        mark_neighbours(lane, chg_bits, tx, ty, tiles_x, tiles_y, im.base, false, (unsigned)pass + 1, stamp, next_list,
+                        next_count);
 }
 
 // Pass 0 runs every tile of every image; pass k > 0 runs the tiles listed by pass k - 1.  Blocks
@@ -380,19 +401,117 @@ template <typename T, int CONN>
 __global__ __launch_bounds__(64, RECON_WAVES) void recon_tiles(const RImg<T>* __restrict__ imgs, RImg<T> one, const int* __restrict__ tile_img,
                                                   int ntiles, const int* __restrict__ list, const int* __restrict__ list_count,
                                                   int* __restrict__ next_list, int* __restrict__ next_count,
-                                                  unsigned* __restrict__ stamp, int pass) {
+                                                  int* __restrict__ clear_count, unsigned* __restrict__ stamp, int pass,
+                                                  unsigned long long* __restrict__ dirty) {
     __shared__ RShared<T> sh;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && clear_count) *clear_count = 0;  // the counter of pass + 2 (ring)
     const int n = (pass == 0) ? ntiles : *list_count;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         const int g = (pass == 0) ? k : list[k];
         const RImg<T> im = imgs ? imgs[tile_img[g]] : one;
         const int local = g - im.base;
         if (CONN == 8 || (CONN == 0 && im.conn == 8))
-            recon_tile<T, 8>(im, local, pass, sh, next_list, next_count, stamp);
+            recon_tile<T, 8>(im, local, pass, sh, next_list, next_count, stamp, dirty);
         else
-            recon_tile<T, 4>(im, local, pass, sh, next_list, next_count, stamp);
+            recon_tile<T, 4>(im, local, pass, sh, next_list, next_count, stamp, dirty);
         __syncthreads();  // the shared window is reused by the next tile
     }
+}
+
+// ---- the residual wavefront as a device-side work queue ----------------------------------------
+// After pass 0 (every tile once, marking the neighbours whose halo changed in the `dirty` bitmap),
+// one launch drains the bitmap: a wave claims a dirty tile that no other wave is solving (the
+// `busy` bitmap makes a tile's solves exclusive, so a tile's pixels only ever rise), clears its
+// dirty bit, re-solves it from the current output and marks the neighbours it changed.  No pass
+// structure, no host round trip, no grid-wide barrier: a tile re-runs as soon as a neighbour
+// feeds it.  The claim is an acquire (the fence after it discards stale L1 lines), the marking a
+// release (recon_tile fences its stores first).  A wave leaves when a scan finds no dirty bit
+// while no tile is being solved (a wave that solved a tile always scans again before leaving, so
+// the last marks are drained), or after `spin_limit` empty scans (harmless: others drain).  The
+// host then checks that the bitmap is empty.
+// One wave looks for a dirty tile that no other wave is solving and claims it: returns its id
+// (the same in every lane) or -1; `seen` tells whether any dirty bit was found.  The running
+// count is raised before a claim attempt, so a claimed tile is always counted.
+__device__ __forceinline__ int drain_claim(int lane, int nwords, int start, unsigned long long* dirty,
+                                           unsigned long long* busy, int* running, bool& seen) {
+    int got = -1;
+    seen = false;
+    for (int k0 = 0; k0 < nwords && got < 0; k0 += 64) {
+        const int k = k0 + lane;
+        int wi = start + k;
+        if (wi >= nwords) wi -= nwords;
+        unsigned long long d = 0;
+        if (k < nwords) d = __hip_atomic_load(&dirty[wi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long cand = __ballot(d != 0);
+        seen |= cand != 0;
+        while (cand && got < 0) {
+            const int src = __ffsll((long long)cand) - 1;
+            cand &= cand - 1;
+            int mine = -1;
+            if (lane == src) {
+                atomicAdd(running, 1);
+                unsigned long long bits = d;
+                while (bits) {
+                    const int b = __ffsll((long long)bits) - 1;
+                    bits &= bits - 1;
+                    const unsigned long long m = 1ull << b;
+                    if (atomicOr(&busy[wi], m) & m) continue;  // another wave is solving it
+                    if (atomicAnd(&dirty[wi], ~m) & m) {
+                        mine = wi * 64 + b;
+                        break;
+                    }
+                    atomicAnd(&busy[wi], ~m);
+                }
+                if (mine < 0) atomicSub(running, 1);
+            }
+            got = __shfl(mine, src);
+        }
+    }
+    return got;
+}
+
+// end of a claimed tile's solve (one lane): its pixels and marks are out, it may be claimed again
+__device__ __forceinline__ void drain_release(int got, unsigned long long* busy, int* running) {
+    __threadfence();
+    atomicAnd(&busy[got >> 6], ~(1ull << (got & 63)));
+    atomicSub(running, 1);
+}
+
+template <typename T, int CONN>
+__global__ __launch_bounds__(64, RECON_WAVES) void recon_drain(const RImg<T>* __restrict__ imgs, RImg<T> one,
+                                                               const int* __restrict__ tile_img, int nwords,
+                                                               unsigned long long* dirty, unsigned long long* busy,
+                                                               int* running, int spin_limit) {
+    __shared__ RShared<T> sh;
+    const int lane = threadIdx.x;
+    int idle = 0;
+    const int start = (int)((blockIdx.x * 97u) % (unsigned)nwords);
+    for (;;) {
+        bool seen = false;
+        const int got = drain_claim(lane, nwords, start, dirty, busy, running, seen);
+        if (got < 0) {
+            const int r = __hip_atomic_load(running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((!seen && r == 0) || ++idle > spin_limit) return;
+            __builtin_amdgcn_s_sleep(4);
+            continue;
+        }
+        idle = 0;
+        __threadfence();  // acquire: the neighbours' latest pixels, not stale L1 lines
+        const RImg<T> im = imgs ? imgs[tile_img[got]] : one;
+        const int local = got - im.base;
+        if (CONN == 8 || (CONN == 0 && im.conn == 8))
+            recon_tile<T, 8>(im, local, 1, sh, nullptr, nullptr, nullptr, dirty);
+        else
+            recon_tile<T, 4>(im, local, 1, sh, nullptr, nullptr, nullptr, dirty);
+        __syncthreads();  // the shared window is reused by the next tile
+        if (lane == 0) drain_release(got, busy, running);
+    }
+}
+
+// 1 in *flag if any bit of the bitmap is set
+__global__ void k_any_bits(const unsigned long long* __restrict__ bits, int nwords, int* flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x)
+        if (bits[i]) atomicOr(flag, 1);
 }
 
 // ---- regions of G x G windows per workgroup --------------------------------------------
@@ -612,8 +731,10 @@ __global__ __launch_bounds__(64 * G * G) void recon_regions(const RImg<T>* __res
                                                             const int* __restrict__ tile_img, int ntiles,
                                                             const int* __restrict__ list, const int* __restrict__ list_count,
                                                             int* __restrict__ next_list, int* __restrict__ next_count,
-                                                            unsigned* __restrict__ stamp, int pass) {
+                                                            int* __restrict__ clear_count, unsigned* __restrict__ stamp,
+                                                            int pass) {
     __shared__ RRegion<T, G> sh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *clear_count = 0;  // the counter of pass + 2 (ring)
     const int n = (pass == 0) ? ntiles : *list_count;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         const int g = (pass == 0) ? k : list[k];
@@ -627,186 +748,226 @@ __global__ __launch_bounds__(64 * G * G) void recon_regions(const RImg<T>* __res
     }
 }
 
-// The whole reconstruction in one cooperative launch: every pass runs over a grid-stride loop and
-// ends at a grid-wide barrier (co-residency of all blocks is guaranteed by the cooperative launch),
-// so no host round trip sits between passes.  All blocks read the same pass count after the
-// barrier, hence leave the loop together.
-template <typename T, int CONN>
-__global__ __launch_bounds__(64) void recon_persistent(const RImg<T>* __restrict__ imgs, RImg<T> one,
-                                                       const int* __restrict__ tile_img, int ntiles, int* listA, int* listB,
-                                                       int* counts, unsigned* stamp, int max_pass, int* status) {
-    __shared__ RShared<T> sh;
-    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-    for (int pass = 0; pass <= max_pass; ++pass) {
-        const int* cur = (pass & 1) ? listB : listA;  // written by the previous pass
-        int* nxt = (pass & 1) ? listA : listB;
-        const int n = pass == 0 ? ntiles : __hip_atomic_load(&counts[pass - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (n == 0) return;
-        if (pass == max_pass) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(status, 1);
-            return;
-        }
-        for (int k = blockIdx.x; k < n; k += gridDim.x) {
-            const int g = (pass == 0) ? k : cur[k];
-            const RImg<T> im = imgs ? imgs[tile_img[g]] : one;
-            const int local = g - im.base;
-            if (CONN == 8 || (CONN == 0 && im.conn == 8))
-                recon_tile<T, 8>(im, local, pass, sh, nxt, &counts[pass], stamp);
-            else
-                recon_tile<T, 4>(im, local, pass, sh, nxt, &counts[pass], stamp);
-            __syncthreads();
-        }
-        grid.sync();
-    }
-}
-
 }  // namespace
 
 // Passes over a set of images until no tile is marked.  imgs_dev = nullptr: the single image `one`.
 template <typename T, int CONN, int G>
 static void launch_pass(int grid, hipStream_t s, const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt,
-                        const int* lst, const int* cnt, int* nl, int* nc, unsigned* stamp, int pass) {
+                        const int* lst, const int* cnt, int* nl, int* nc, int* clr, unsigned* stamp, int pass,
+                        unsigned long long* dirty = nullptr) {
     if (G == 1)
         hipLaunchKernelGGL((recon_tiles<T, CONN>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc,
-                           stamp, pass);
+                           clr, stamp, pass, dirty);
     else
         hipLaunchKernelGGL((recon_regions<T, CONN, G>), dim3(grid), dim3(64 * G * G), 0, s, imgs_dev, one, tile_img, nt, lst,
-                           cnt, nl, nc, stamp, pass);
+                           cnt, nl, nc, clr, stamp, pass);
 }
 
 template <typename T, int G>
 static void launch_conn(int conn, int grid, hipStream_t s, const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img,
-                        int nt, const int* lst, const int* cnt, int* nl, int* nc, unsigned* stamp, int pass) {
-    if (conn == 8) launch_pass<T, 8, G>(grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
-    else if (conn == 4) launch_pass<T, 4, G>(grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
-    else launch_pass<T, 0, G>(grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
+                        int nt, const int* lst, const int* cnt, int* nl, int* nc, int* clr, unsigned* stamp, int pass) {
+    if (conn == 8) launch_pass<T, 8, G>(grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, clr, stamp, pass);
+    else if (conn == 4) launch_pass<T, 4, G>(grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, clr, stamp, pass);
+    else launch_pass<T, 0, G>(grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, clr, stamp, pass);
 }
 
 // largest region side the workgroup's LDS holds for the element type
 template <typename T> constexpr int max_region() { return sizeof(T) == 1 ? 4 : 2; }
 
-// Tunables of the per-pass schedule, per element size (0: u8, 1: 4-byte): region side and the
-// number of passes launched between host checks.  Defaults measured on MI355X; the autotuner
-// (nsc_recon_autotune) replaces them with the fastest setting on a caller's batch.
+// Schedule of the per-pass driver, per element size (0: u8, 1: 4-byte): region side (0 = the
+// measured default) and the number of passes launched between host checks.  Per calling thread:
+// the autotuner (nsc_recon_autotune) sets the calling thread's schedule only, so concurrent callers
+// never change each other's.
 struct ReconTune {
-    std::atomic<int> region{0}, batch{4};
+    int region = 0, batch = 4;
+    int drain = -1;  // 1: device-side work queue for one-window regions, 0: passes, -1: NSC_RECON_DRAIN
 };
-static ReconTune g_tune[2];
-template <typename T> static ReconTune& tune() { return g_tune[sizeof(T) == 1 ? 0 : 1]; }
+static thread_local ReconTune tl_tune[2];
+template <typename T> static ReconTune& tune() { return tl_tune[sizeof(T) == 1 ? 0 : 1]; }
+
+// Per-pass counters live in a ring of kRing entries: pass p reads entry (p-1) % kRing (its list
+// length), adds to entry p % kRing, and its block 0 clears entry (p+1) % kRing for pass p + 1,
+// so no buffer grows with the number of passes.
+constexpr int kRing = 4;
+
+// Upper bound on the passes of a call: a pass that leaves a tile marked moved the front by at
+// least one pixel along some propagation path, and every path is shorter than the pixel count.
+static long long pass_limit(long long pixels) { return pixels + 8; }
+
+// Device buffers of a reconstruction call for nt tiles (regions), carved from one block: the drain's
+// dirty and busy bitmaps and its counters, the passes' stamps and counter ring (all of these start
+// at zero: one clear covers them), then the passes' two tile lists.
+struct ReconBufs {
+    unsigned long long *dirty, *busy;
+    int* flags;  // [0] tiles being solved by the drain, [1] bits left after the drain
+    unsigned* stamp;
+    int *counts, *listA, *listB;
+    size_t zero_bytes;  // leading bytes to clear
+};
+static size_t recon_words(int nt) { return ((size_t)nt + 63) / 64; }
+static size_t recon_scratch_bytes(int nt) {
+    return 2 * sizeof(unsigned long long) * recon_words(nt) + 4 * sizeof(int) + sizeof(unsigned) * (size_t)nt +
+           sizeof(int) * (kRing + 2 * (size_t)nt);
+}
+static ReconBufs recon_carve(void* base, int nt) {
+    ReconBufs B;
+    const size_t nw = recon_words(nt);
+    B.dirty = static_cast<unsigned long long*>(base);
+    B.busy = B.dirty + nw;
+    B.flags = reinterpret_cast<int*>(B.busy + nw);
+    B.stamp = reinterpret_cast<unsigned*>(B.flags + 4);
+    B.counts = reinterpret_cast<int*>(B.stamp + nt);
+    B.listA = B.counts + kRing;
+    B.listB = B.listA + nt;
+    B.zero_bytes = (size_t)(reinterpret_cast<uint8_t*>(B.listA) - static_cast<uint8_t*>(base));
+    return B;
+}
+
+// clears the leading words of the call's buffers (a kernel, not a runtime fill, so a captured graph
+// of a call replays it faithfully)
+__global__ void k_zero_words(uint32_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+static void recon_clear(const ReconBufs& B, hipStream_t s) {
+    const size_t words = B.zero_bytes / 4;
+    hipLaunchKernelGGL(k_zero_words, dim3(std::min<unsigned>(cdiv(words, 256), 1024u)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t*>(B.dirty), words);
+}
+
+// waves of the drain kernel that fit on the device at once (the grid of the drain launch)
+template <typename T>
+static int drain_grid() {
+    static int g = [] {
+        int dev = 0, per_cu = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, recon_drain<T, 8>, 64, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            return 2048;
+        }
+        return std::max(1, per_cu) * std::max(1, cus);
+    }();
+    return g;
+}
+
+// Schedule of the residual wavefront: host-launched passes (default) or the device-side work
+// queue (NSC_RECON_DRAIN=1; one-window regions only).  The queue is exact but measured far slower
+// on MI355X (imreconstruct<uchar> of the 4096^2 ellipse tile: 15.5 ms against 0.44 ms with passes):
+// every claim, mark and bitmap scan is a device-scope atomic or coherent load, which the eight
+// XCDs' separate L2 caches cannot serve locally, and thousands of waves contend for a few bitmap
+// lines, while a pass boundary makes the same hand-off once per launch.
+static bool drain_env() {
+    static bool v = [] {
+        const char* e = getenv("NSC_RECON_DRAIN");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+template <typename T>
+static bool drain_enabled() {
+    const int d = tune<T>().drain;
+    return d < 0 ? drain_env() : d != 0;
+}
 
 template <typename T>
-static int recon_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, int conn, int G, hipStream_t s) {
-    const int max_pass = 4 * nt + 8;
-    Scratch sc;
-    const size_t bytes = sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2);
-    NSC_CHECK(sc.alloc(bytes, s));
-    unsigned* stamp = sc.as<unsigned>();
-    int* counts = reinterpret_cast<int*>(stamp + nt);  // next to the stamps: one fill clears both
-    int* listA = counts + max_pass + 2;
-    int* listB = listA + nt;
-    NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt + sizeof(int) * (max_pass + 2), s));
+static int recon_passes(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, long long pixels, int conn,
+                        int G, const ReconBufs& B, hipStream_t s) {
+    const long long max_pass = pass_limit(pixels);
     const int grid = std::min(nt, 1 << 16);
-    auto launch = [&](int pass, const int* lst, const int* cnt, int* nl, int* nc) {
+    unsigned* stamp = B.stamp;
+    int* counts = B.counts;
+    auto launch = [&](int pass, const int* lst, const int* cnt, int* nl, int* nc, int* clr) {
         switch (G) {
-            case 2: launch_conn<T, 2>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass); break;
+            case 2: launch_conn<T, 2>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, clr, stamp, pass); break;
             case 3: launch_conn<T, (max_region<T>() >= 3 ? 3 : 2)>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl,
-                                                                    nc, stamp, pass); break;
-            case 4: launch_conn<T, max_region<T>()>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
+                                                                    nc, clr, stamp, pass); break;
+            case 4: launch_conn<T, max_region<T>()>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, clr, stamp,
+                                                    pass);
                     break;
-            default: launch_conn<T, 1>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, stamp, pass);
+            default: launch_conn<T, 1>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, clr, stamp, pass);
         }
     };
-    launch(0, nullptr, nullptr, listA, &counts[0]);
+    launch(0, nullptr, nullptr, B.listA, &counts[0], &counts[1]);
     NSC_HIP(hipGetLastError());
-    int pass = 1;
+    long long pass = 1;
     for (;;) {
-        const int batch = std::max(1, tune<T>().batch.load());
+        const int batch = std::max(1, tune<T>().batch);
         for (int b = 0; b < batch && pass <= max_pass; ++b, ++pass) {
-            int* cur = (pass & 1) ? listA : listB;
-            int* nxt = (pass & 1) ? listB : listA;
-            launch(pass, cur, &counts[pass - 1], nxt, &counts[pass]);
+            int* cur = (pass & 1) ? B.listA : B.listB;
+            int* nxt = (pass & 1) ? B.listB : B.listA;
+            launch((int)std::min<long long>(pass, INT_MAX - 1), cur, &counts[(pass - 1) % kRing], nxt, &counts[pass % kRing],
+                   &counts[(pass + 1) % kRing]);
         }
         NSC_HIP(hipGetLastError());
         int c = 0;
-        NSC_HIP(hipMemcpyAsync(&c, &counts[pass - 1], sizeof(int), hipMemcpyDeviceToHost, s));
+        NSC_HIP(hipMemcpyAsync(&c, &counts[(pass - 1) % kRing], sizeof(int), hipMemcpyDeviceToHost, s));
         NSC_HIP(stream_sync(s));
         if (c == 0) break;
         if (pass > max_pass) {
-            set_error("imreconstruct: no convergence after %d passes", max_pass);
+            set_error("imreconstruct: no convergence after %lld passes", max_pass);
             return NSC_ERR_LIMIT;
         }
     }
     return NSC_OK;
 }
 
-static std::mutex& coop_mutex() {
-    static std::mutex m;
-    return m;
-}
-
-// Same passes in one cooperative launch (see recon_persistent).
-template <typename T, int CONN>
-static int recon_coop_launch(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, hipStream_t s) {
-    static int max_blocks[2] = {0, 0};  // co-resident blocks of this kernel on the device (per process)
-    int& mb = max_blocks[0];
-    if (mb == 0) {
-        int dev = 0, per_cu = 0, cus = 0;
-        NSC_HIP(hipGetDevice(&dev));
-        NSC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        NSC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, recon_persistent<T, CONN>, 64, 0));
-        mb = std::max(1, per_cu) * cus;
-    }
-    const int max_pass = 4 * nt + 8;
-    Scratch sc;
-    const size_t bytes = sizeof(unsigned) * nt + 2 * sizeof(int) * nt + sizeof(int) * (max_pass + 2) + sizeof(int);
-    NSC_CHECK(sc.alloc(bytes, s));
-    unsigned* stamp = sc.as<unsigned>();
-    int* listA = reinterpret_cast<int*>(stamp + nt);
-    int* listB = listA + nt;
-    int* counts = listB + nt;
-    int* status = counts + max_pass + 2;
-    NSC_HIP(hipMemsetAsync(stamp, 0, bytes, s));
-    dim3 grid((unsigned)std::min(nt, mb)), block(64);
-    RImg<T> one_arg = one;
-    int ntiles = nt, mp = max_pass;
-    void* args[] = {(void*)&imgs_dev, (void*)&one_arg, (void*)&tile_img, (void*)&ntiles, (void*)&listA, (void*)&listB,
-                    (void*)&counts, (void*)&stamp, (void*)&mp, (void*)&status};
-    int st = 0;
-    {
-        // two grid-barrier kernels sharing the device could each hold CUs the other waits for: the
-        // process runs one at a time, each to completion
-        std::lock_guard<std::mutex> lk(coop_mutex());
-        NSC_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(recon_persistent<T, CONN>), grid, block, args, 0, s));
-        NSC_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
-        NSC_HIP(stream_sync(s));
-    }
-    if (st) {
-        set_error("imreconstruct: no convergence after %d passes", max_pass);
-        return NSC_ERR_LIMIT;
-    }
+// pass 0 marking into the dirty bitmap, then one drain launch; the bitmap must end empty
+template <typename T>
+static int recon_drain_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, int conn,
+                           const ReconBufs& B, hipStream_t s, bool* leftover) {
+    const int grid0 = std::min(nt, 1 << 16);
+    const int nw = (int)recon_words(nt);
+    if (conn == 8) launch_pass<T, 8, 1>(grid0, s, imgs_dev, one, tile_img, nt, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                        nullptr, 0, B.dirty);
+    else if (conn == 4) launch_pass<T, 4, 1>(grid0, s, imgs_dev, one, tile_img, nt, nullptr, nullptr, nullptr, nullptr,
+                                             nullptr, nullptr, 0, B.dirty);
+    else launch_pass<T, 0, 1>(grid0, s, imgs_dev, one, tile_img, nt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              0, B.dirty);
+    const int grid = std::min(nt, drain_grid<T>());
+    constexpr int kSpin = 4096;
+    if (conn == 8)
+        hipLaunchKernelGGL((recon_drain<T, 8>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nw, B.dirty, B.busy,
+                           &B.flags[0], kSpin);
+    else if (conn == 4)
+        hipLaunchKernelGGL((recon_drain<T, 4>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nw, B.dirty, B.busy,
+                           &B.flags[0], kSpin);
+    else
+        hipLaunchKernelGGL((recon_drain<T, 0>), dim3(grid), dim3(64), 0, s, imgs_dev, one, tile_img, nw, B.dirty, B.busy,
+                           &B.flags[0], kSpin);
+    hipLaunchKernelGGL(k_any_bits, dim3(std::min<unsigned>(cdiv(nw, 256), 256u)), dim3(256), 0, s, B.dirty, nw, &B.flags[1]);
+    NSC_HIP(hipGetLastError());
+    int left = 0;
+    NSC_HIP(hipMemcpyAsync(&left, &B.flags[1], sizeof(int), hipMemcpyDeviceToHost, s));
+    NSC_HIP(stream_sync(s));
+    *leftover = left != 0;
     return NSC_OK;
 }
 
+static std::atomic<long long> g_drain_redo{0};  // drains that left marked tiles (reporting)
+
+// the whole reconstruction on cleared buffers B
 template <typename T>
-static int recon_coop(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, int conn, hipStream_t s) {
-    if (conn == 8) return recon_coop_launch<T, 8>(imgs_dev, one, tile_img, nt, s);
-    if (conn == 4) return recon_coop_launch<T, 4>(imgs_dev, one, tile_img, nt, s);
-    return recon_coop_launch<T, 0>(imgs_dev, one, tile_img, nt, s);
+static int recon_run(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, long long pixels, int conn,
+                     int G, const ReconBufs& B, hipStream_t s) {
+    if (G == 1 && drain_enabled<T>()) {
+        bool leftover = false;
+        NSC_CHECK(recon_drain_run<T>(imgs_dev, one, tile_img, nt, conn, B, s, &leftover));
+        if (!leftover) return NSC_OK;
+        // a drain that ended with marked tiles (its waves gave up): the passes finish from scratch
+        ++g_drain_redo;
+        recon_clear(B, s);
+    }
+    return recon_passes<T>(imgs_dev, one, tile_img, nt, pixels, conn, G, B, s);
 }
 
-// schedule: 0 = one launch per pass (host checks every 4 passes), 1 = one cooperative launch
-// (NSC_RECON_SCHEDULE=coop).  Measured on MI355X the per-pass launches are faster (the grid
-// barrier costs more than a launch, and the cooperative grid is capped at the co-resident block
-// count), so they are the default.  Threads that run many chains side by side (the server's
-// workers) always use 0 so that their reconstructions overlap instead of queueing for the
-// cooperative launch.
-static thread_local int tl_schedule = -1;
-static int recon_schedule() {
-    static int v = [] {
-        const char* e = getenv("NSC_RECON_SCHEDULE");
-        return (e && e[0] == 'c') ? 1 : 0;
-    }();
-    return tl_schedule >= 0 ? tl_schedule : v;
+template <typename T>
+static int recon_run_scratch(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, long long pixels,
+                             int conn, int G, hipStream_t s) {
+    Scratch sc;
+    NSC_CHECK(sc.alloc(recon_scratch_bytes(nt), s));
+    const ReconBufs B = recon_carve(sc.p, nt);
+    recon_clear(B, s);
+    return recon_run<T>(imgs_dev, one, tile_img, nt, pixels, conn, G, B, s);
 }
 
 // region side (windows per workgroup side) for element type T: NSC_RECON_REGION overrides
@@ -819,7 +980,7 @@ static int region_side() {
     if (env) return env;
     // measured default 1: 2 ties with it on the 4096^2 workloads, 3 and 4 are slower (a region's
     // rounds wait on its slowest window)
-    const int t = tune<T>().region.load();
+    const int t = tune<T>().region;
     return t ? std::min(t, max_region<T>()) : 1;
 }
 
@@ -832,13 +993,9 @@ static void set_regions(RImg<T>& r, int G) {
 template <typename T>
 int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, hipStream_t s) {
     RImg<T> one{marker, mask, out, w, h, (int)cdiv(w, OW), (int)cdiv(h, OH), conn, 0, 0, 0};
-    if (recon_schedule() == 1) {
-        set_regions(one, 1);
-        return recon_coop<T>(nullptr, one, nullptr, one.tiles_x * one.tiles_y, conn, s);
-    }
     const int G = region_side<T>();
     set_regions(one, G);
-    return recon_run<T>(nullptr, one, nullptr, one.rgx * one.rgy, conn, G, s);
+    return recon_run_scratch<T>(nullptr, one, nullptr, one.rgx * one.rgy, (long long)w * h, conn, G, s);
 }
 
 // A batch of images in one launch per pass: the image table and the tile -> image map live in
@@ -846,9 +1003,9 @@ int recon_impl(const T* marker, const T* mask, T* out, int w, int h, int conn, h
 template <typename T>
 int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
     std::vector<RImg<T>> imgs(n);
-    long long nt = 0;
+    long long nt = 0, pixels = 0;
     int conn_all = d[0].conn;
-    const int G = recon_schedule() == 1 ? 1 : region_side<T>();
+    const int G = region_side<T>();
     for (int i = 0; i < n; ++i) {
         NSC_ARG(d[i].marker && d[i].mask && d[i].out && d[i].w > 0 && d[i].h > 0 && (d[i].conn == 4 || d[i].conn == 8),
                 "imreconstruct batched: descriptor");
@@ -864,6 +1021,7 @@ int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
         r.base = (int)nt;
         set_regions(r, G);
         nt += (long long)r.rgx * r.rgy;
+        pixels += (long long)r.w * r.h;
         if (d[i].conn != conn_all) conn_all = 0;
     }
     NSC_ARG(nt < (1ll << 30), "imreconstruct batched: too many tiles");
@@ -877,8 +1035,7 @@ int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
     int* tile_dev = reinterpret_cast<int*>(static_cast<char*>(tab.p) + ib);
     NSC_HIP(hipMemcpyAsync(imgs_dev, imgs.data(), sizeof(RImg<T>) * n, hipMemcpyHostToDevice, s));
     NSC_HIP(hipMemcpyAsync(tile_dev, tile_img.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice, s));
-    const int rc = recon_schedule() == 1 ? recon_coop<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, s)
-                                         : recon_run<T>(imgs_dev, imgs[0], tile_dev, (int)nt, conn_all, G, s);
+    const int rc = recon_run_scratch<T>(imgs_dev, imgs[0], tile_dev, (int)nt, pixels, conn_all, G, s);
     // the host tables must outlive the copies (pageable copies complete before returning; be explicit)
     NSC_HIP(stream_sync(s));
     return rc;
@@ -887,48 +1044,48 @@ int recon_batched_impl(const nsc_recon_desc* d, int n, hipStream_t s) {
 template int recon_impl<uint8_t>(const uint8_t*, const uint8_t*, uint8_t*, int, int, int, hipStream_t);
 template int recon_impl<int32_t>(const int32_t*, const int32_t*, int32_t*, int, int, int, hipStream_t);
 template int recon_impl<float>(const float*, const float*, float*, int, int, int, hipStream_t);
-void set_thread_recon_schedule(int schedule) { tl_schedule = schedule; }
 
 // Times every (region side, passes per host check) candidate on the caller's batch -- the
 // outputs are written each time, identically -- and keeps the fastest for later calls of the
 // same element type.  timings (optional, host) receives one entry per candidate in ms.
 template <typename T>
 static int recon_autotune(const nsc_recon_desc* d, int n, hipStream_t s, int* region, int* batch, float* timings) {
-    const int regions[2] = {1, 2}, batches[2] = {4, 8};
+    // candidates: (region side, passes per host check, work queue)
+    const int cand[5][3] = {{1, 4, 0}, {1, 8, 0}, {2, 4, 0}, {2, 8, 0}, {1, 4, 1}};
     ReconTune& t = tune<T>();
-    const int r0 = t.region.load(), b0 = t.batch.load();
+    const ReconTune t0 = t;
     hipEvent_t e0, e1;
     NSC_HIP(hipEventCreate(&e0));
     NSC_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int bi = 0, k = 0;
-    for (int ri = 0; ri < 2; ++ri)
-        for (int b = 0; b < 2; ++b, ++k) {
-            t.region = regions[ri];
-            t.batch = batches[b];
-            int rc = recon_batched_impl<T>(d, n, s);  // warm
-            float ms = 0.f;
-            if (rc == NSC_OK) {
-                (void)hipEventRecord(e0, s);
-                rc = recon_batched_impl<T>(d, n, s);
-                (void)hipEventRecord(e1, s);
-                (void)hipEventSynchronize(e1);
-                (void)hipEventElapsedTime(&ms, e0, e1);
-            }
-            if (rc != NSC_OK) {
-                t.region = r0;
-                t.batch = b0;
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                return rc;
-            }
-            if (timings) timings[k] = ms;
-            if (ms < best) { best = ms; bi = k; }
+    int bi = 0;
+    for (int k = 0; k < 5; ++k) {
+        t.region = cand[k][0];
+        t.batch = cand[k][1];
+        t.drain = cand[k][2];
+        int rc = recon_batched_impl<T>(d, n, s);  // warm
+        float ms = 0.f;
+        if (rc == NSC_OK) {
+            (void)hipEventRecord(e0, s);
+            rc = recon_batched_impl<T>(d, n, s);
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
         }
-    t.region = regions[bi / 2];
-    t.batch = batches[bi % 2];
+        if (rc != NSC_OK) {
+            t = t0;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
+        if (timings) timings[k] = ms;
+        if (ms < best) { best = ms; bi = k; }
+    }
+    t.region = cand[bi][0];
+    t.batch = cand[bi][1];
+    t.drain = cand[bi][2];
     if (region) *region = t.region;
-    if (batch) *batch = t.batch;
+    if (batch) *batch = t.drain ? 0 : t.batch;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return NSC_OK;
@@ -938,6 +1095,15 @@ template int recon_batched_impl<uint8_t>(const nsc_recon_desc*, int, hipStream_t
 template int recon_batched_impl<float>(const nsc_recon_desc*, int, hipStream_t);
 
 }  // namespace nsc
+
+extern "C" int nsc_recon_set_schedule(int is_float, int region, int passes, int queue) {
+    NSC_ARG(region >= 0 && region <= 4 && passes >= 0 && queue >= -1 && queue <= 1, "recon_set_schedule");
+    nsc::ReconTune& t = is_float ? nsc::tune<float>() : nsc::tune<uint8_t>();
+    t.region = region;
+    t.batch = passes ? passes : 4;
+    t.drain = queue;
+    return NSC_OK;
+}
 
 extern "C" int nsc_recon_autotune(const nsc_recon_desc* descs, int n, int is_float, void* stream, int* region, int* passes,
                                   float* timings) {
@@ -1107,11 +1273,12 @@ __device__ __forceinline__ void geo_store(unsigned* dist, int w, int h, int tx, 
 __global__ __launch_bounds__(64, 2) void geo_tiles(const uint4* __restrict__ flags, unsigned* dist, int w, int h,
                                                              int tiles_x, int tiles_y, const int* __restrict__ list,
                                                              const int* __restrict__ list_count, int* __restrict__ next_list,
-                                                             int* __restrict__ next_count, unsigned* __restrict__ stamp,
-                                                             int pass_no) {
+                                                             int* __restrict__ next_count, int* __restrict__ clear_count,
+                                                             unsigned* __restrict__ stamp, int pass_no) {
     __shared__ unsigned tr[TH][TW + 1];
     __shared__ unsigned init_l[TH], init_r[TH];
     __shared__ unsigned chg_bits;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *clear_count = 0;  // the counter of pass + 2 (ring)
     int tile;
     if (pass_no == 0) tile = blockIdx.x;
     else {
@@ -1190,118 +1357,182 @@ struct GeoRegion {
     unsigned chg_bits;
 };
 
+// one region of G x G windows: load, solve to stability with halo exchanges through LDS, store,
+// mark the neighbouring regions whose halo changed (in the pass lists, or the drain's bitmap)
+template <int G>
+__device__ __forceinline__ void geo_region_solve(const uint4* __restrict__ flags, unsigned* dist, int w, int h, int tiles_x,
+                                                 int tiles_y, int rgx, int rgy, int region, GeoRegion<G>& sh,
+                                                 int* __restrict__ next_list, int* __restrict__ next_count,
+                                                 unsigned* __restrict__ stamp, int pass_no, unsigned long long* dirty) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wx = wv % G, wy = wv / G;
+    auto& tr = sh.tr[wv];
+    const int rx = region % rgx, ry = region / rgx;
+    const int tx = rx * G + wx, ty = ry * G + wy;
+    const bool valid = tx < tiles_x && ty < tiles_y;
+    const bool has_l = valid && wx > 0, has_u = valid && wy > 0;
+    const bool has_r = valid && wx < G - 1 && tx + 1 < tiles_x, has_d = valid && wy < G - 1 && ty + 1 < tiles_y;
+    if (threadIdx.x == 0) sh.chg_bits = 0;
+    unsigned v[TH];
+    GeoFlags F;
+    unsigned init_top = 0, init_bot = 0;
+    if (valid) {
+        geo_load(flags, dist, w, h, tx, ty, tiles_x, lane, v, F);
+        if (lane == 1) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) { sh.init_l[wv][y] = v[y]; sh.xleft[wv][y] = v[y]; }
+        }
+        if (lane == TW - 2) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) { sh.init_r[wv][y] = v[y]; sh.xright[wv][y] = v[y]; }
+        }
+        init_top = v[1];
+        init_bot = v[TH - 2];
+        sh.xtop[wv][lane] = v[1];
+        sh.xbot[wv][lane] = v[TH - 2];
+    }
+    __syncthreads();
+    // owned pixels: rows 1..62 of lanes 1..62; the halo rows and lanes change only through the
+    // exchanges below, which the full-column sums also see
+    const uint64_t own0 = geo_key_sum(v, 1, TH - 1);
+    uint64_t full = own0 + (uint64_t)(~v[0]) + (uint64_t)(~v[TH - 1]), own = own0;
+    for (int round = 0; round < (1 << 20); ++round) {
+        bool changed = false;
+        if (valid) {
+            if (has_u) {
+                const unsigned nb = sh.xbot[wv - G][lane];
+                if (nb < v[0]) v[0] = nb;
+            }
+            if (has_d) {
+                const unsigned nb = sh.xtop[wv + G][lane];
+                if (nb < v[TH - 1]) v[TH - 1] = nb;
+            }
+            geo_col_sweeps(v, F);
+#pragma unroll
+            for (int y = 0; y < TH; ++y) tr[y][lane] = v[y];
+        }
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int x = 0; x < TW; ++x) v[x] = tr[lane][x];
+            if (has_l) {
+                const unsigned nb = sh.xright[wv - 1][lane];
+                if (nb < v[0]) v[0] = nb;
+            }
+            if (has_r) {
+                const unsigned nb = sh.xleft[wv + 1][lane];
+                if (nb < v[TW - 1]) v[TW - 1] = nb;
+            }
+            geo_row_sweeps(v, F);
+            sh.xleft[wv][lane] = v[1];
+            sh.xright[wv][lane] = v[TW - 2];
+#pragma unroll
+            for (int x = 0; x < TW; ++x) tr[lane][x] = v[x];
+        }
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) v[y] = tr[y][lane];
+            sh.xtop[wv][lane] = v[1];
+            sh.xbot[wv][lane] = v[TH - 2];
+            own = geo_key_sum(v, 1, TH - 1);
+            const uint64_t f1 = own + (uint64_t)(~v[0]) + (uint64_t)(~v[TH - 1]);
+            changed = f1 != full;
+            full = f1;
+        }
+        if (!__syncthreads_or(changed ? 1 : 0)) break;
+    }
+    const bool lane_owned = lane >= 1 && lane <= TW - 2;
+    if (valid && __any(lane_owned && own != own0)) geo_store(dist, w, h, tx, ty, lane, v);
+    unsigned bits = 0;
+    if (valid) {
+        const bool edge_l = wx == 0, edge_r = wx == G - 1 || tx == tiles_x - 1;
+        const bool edge_u = wy == 0, edge_d = wy == G - 1 || ty == tiles_y - 1;
+        if (lane >= 1 && lane <= TW - 2) {
+            if (edge_u && v[1] != init_top) bits |= CH_TOP;
+            if (edge_d && v[TH - 2] != init_bot) bits |= CH_BOT;
+        }
+        if (lane == 1 && edge_l) {
+            for (int y = 1; y < TH - 1; ++y)
+                if (v[y] != sh.init_l[wv][y]) bits |= CH_LEFT;
+        }
+        if (lane == TW - 2 && edge_r) {
+            for (int y = 1; y < TH - 1; ++y)
+                if (v[y] != sh.init_r[wv][y]) bits |= CH_RIGHT;
+        }
+    }
+    if (dirty) __threadfence();  // the region's pixels are visible before any neighbour is marked
+    if (bits) atomicOr(&sh.chg_bits, bits);
+    __syncthreads();
+    // 4-connected: edge neighbours only
+    if (dirty)
+        mark_neighbours_bits(threadIdx.x, sh.chg_bits, rx, ry, rgx, rgy, 0, true, dirty);
+    else
+        mark_neighbours(threadIdx.x, sh.chg_bits, rx, ry, rgx, rgy, 0, true, (unsigned)pass_no + 1, stamp, next_list,
+                        next_count);
+    __syncthreads();
+}
+
 template <int G>
 __global__ __launch_bounds__(64 * G * G) void geo_regions(const uint4* __restrict__ flags, unsigned* dist, int w, int h,
                                                           int tiles_x, int tiles_y, int rgx, int rgy,
                                                           const int* __restrict__ list, const int* __restrict__ list_count,
                                                           int* __restrict__ next_list, int* __restrict__ next_count,
-                                                          unsigned* __restrict__ stamp, int pass_no) {
+                                                          int* __restrict__ clear_count, unsigned* __restrict__ stamp,
+                                                          int pass_no) {
     __shared__ GeoRegion<G> sh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *clear_count = 0;  // the counter of pass + 2 (ring)
     const int n = pass_no == 0 ? rgx * rgy : *list_count;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         const int region = pass_no == 0 ? k : list[k];
-        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int wx = wv % G, wy = wv / G;
-        auto& tr = sh.tr[wv];
-        const int rx = region % rgx, ry = region / rgx;
-        const int tx = rx * G + wx, ty = ry * G + wy;
-        const bool valid = tx < tiles_x && ty < tiles_y;
-        const bool has_l = valid && wx > 0, has_u = valid && wy > 0;
-        const bool has_r = valid && wx < G - 1 && tx + 1 < tiles_x, has_d = valid && wy < G - 1 && ty + 1 < tiles_y;
-        if (threadIdx.x == 0) sh.chg_bits = 0;
-        unsigned v[TH];
-        GeoFlags F;
-        unsigned init_top = 0, init_bot = 0;
-        if (valid) {
-            geo_load(flags, dist, w, h, tx, ty, tiles_x, lane, v, F);
-            if (lane == 1) {
-#pragma unroll
-                for (int y = 0; y < TH; ++y) { sh.init_l[wv][y] = v[y]; sh.xleft[wv][y] = v[y]; }
+        geo_region_solve<G>(flags, dist, w, h, tiles_x, tiles_y, rgx, rgy, region, sh, next_list, next_count, stamp,
+                            pass_no, nullptr);
+    }
+}
+
+// The geodesic distance as a device-side work queue (as recon_drain): every region starts dirty;
+// the first wave of the workgroup claims a region and the workgroup solves it.
+template <int G>
+__global__ __launch_bounds__(64 * G * G) void geo_drain(const uint4* __restrict__ flags, unsigned* dist, int w, int h,
+                                                        int tiles_x, int tiles_y, int rgx, int rgy, int nwords,
+                                                        unsigned long long* dirty, unsigned long long* busy, int* running,
+                                                        int spin_limit) {
+    __shared__ GeoRegion<G> sh;
+    __shared__ int s_got;
+    int idle = 0;
+    const int start = (int)((blockIdx.x * 97u) % (unsigned)nwords);
+    for (;;) {
+        if (threadIdx.x < 64) {
+            bool seen = false;
+            int got = drain_claim(threadIdx.x, nwords, start, dirty, busy, running, seen);
+            if (got < 0) {
+                const int r = __hip_atomic_load(running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                got = ((!seen && r == 0) || ++idle > spin_limit) ? -2 : -1;
+            } else {
+                idle = 0;
             }
-            if (lane == TW - 2) {
-#pragma unroll
-                for (int y = 0; y < TH; ++y) { sh.init_r[wv][y] = v[y]; sh.xright[wv][y] = v[y]; }
-            }
-            init_top = v[1];
-            init_bot = v[TH - 2];
-            sh.xtop[wv][lane] = v[1];
-            sh.xbot[wv][lane] = v[TH - 2];
+            if (threadIdx.x == 0) s_got = got;
         }
         __syncthreads();
-        // owned pixels: rows 1..62 of lanes 1..62; the halo rows and lanes change only through the
-        // exchanges below, which the full-column sums also see
-        const uint64_t own0 = geo_key_sum(v, 1, TH - 1);
-        uint64_t full = own0 + (uint64_t)(~v[0]) + (uint64_t)(~v[TH - 1]), own = own0;
-        for (int round = 0; round < (1 << 20); ++round) {
-            bool changed = false;
-            if (valid) {
-                if (has_u) {
-                    const unsigned nb = sh.xbot[wv - G][lane];
-                    if (nb < v[0]) v[0] = nb;
-                }
-                if (has_d) {
-                    const unsigned nb = sh.xtop[wv + G][lane];
-                    if (nb < v[TH - 1]) v[TH - 1] = nb;
-                }
-                geo_col_sweeps(v, F);
-#pragma unroll
-                for (int y = 0; y < TH; ++y) tr[y][lane] = v[y];
-            }
-            __syncthreads();
-            if (valid) {
-#pragma unroll
-                for (int x = 0; x < TW; ++x) v[x] = tr[lane][x];
-                if (has_l) {
-                    const unsigned nb = sh.xright[wv - 1][lane];
-                    if (nb < v[0]) v[0] = nb;
-                }
-                if (has_r) {
-                    const unsigned nb = sh.xleft[wv + 1][lane];
-                    if (nb < v[TW - 1]) v[TW - 1] = nb;
-                }
-                geo_row_sweeps(v, F);
-                sh.xleft[wv][lane] = v[1];
-                sh.xright[wv][lane] = v[TW - 2];
-#pragma unroll
-                for (int x = 0; x < TW; ++x) tr[lane][x] = v[x];
-            }
-            __syncthreads();
-            if (valid) {
-#pragma unroll
-                for (int y = 0; y < TH; ++y) v[y] = tr[y][lane];
-                sh.xtop[wv][lane] = v[1];
-                sh.xbot[wv][lane] = v[TH - 2];
-                own = geo_key_sum(v, 1, TH - 1);
-                const uint64_t f1 = own + (uint64_t)(~v[0]) + (uint64_t)(~v[TH - 1]);
-                changed = f1 != full;
-                full = f1;
-            }
-            if (!__syncthreads_or(changed ? 1 : 0)) break;
+        const int got = s_got;
+        __syncthreads();  // s_got is rewritten only after every thread has read it
+        if (got == -2) return;
+        if (got == -1) {
+            __builtin_amdgcn_s_sleep(4);
+            continue;
         }
-        const bool lane_owned = lane >= 1 && lane <= TW - 2;
-        if (valid && __any(lane_owned && own != own0)) geo_store(dist, w, h, tx, ty, lane, v);
-        unsigned bits = 0;
-        if (valid) {
-            const bool edge_l = wx == 0, edge_r = wx == G - 1 || tx == tiles_x - 1;
-            const bool edge_u = wy == 0, edge_d = wy == G - 1 || ty == tiles_y - 1;
-            if (lane >= 1 && lane <= TW - 2) {
-                if (edge_u && v[1] != init_top) bits |= CH_TOP;
-                if (edge_d && v[TH - 2] != init_bot) bits |= CH_BOT;
-            }
-            if (lane == 1 && edge_l) {
-                for (int y = 1; y < TH - 1; ++y)
-                    if (v[y] != sh.init_l[wv][y]) bits |= CH_LEFT;
-            }
-            if (lane == TW - 2 && edge_r) {
-                for (int y = 1; y < TH - 1; ++y)
-                    if (v[y] != sh.init_r[wv][y]) bits |= CH_RIGHT;
-            }
-        }
-        if (bits) atomicOr(&sh.chg_bits, bits);
-        __syncthreads();
-        mark_neighbours(threadIdx.x, sh.chg_bits, rx, ry, rgx, rgy, 0, true, (unsigned)pass_no + 1, stamp, next_list,
-                        next_count);  // 4-connected: edge neighbours only
-        __syncthreads();
+        __threadfence();  // acquire: the neighbours' latest distances, not stale L1 lines
+        geo_region_solve<G>(flags, dist, w, h, tiles_x, tiles_y, rgx, rgy, got, sh, nullptr, nullptr, nullptr, 1, dirty);
+        if (threadIdx.x == 0) drain_release(got, busy, running);
+    }
+}
+
+// sets the first n bits of a bitmap (every region dirty)
+__global__ void k_set_bits(unsigned long long* bits, int n) {
+    const int nw = (n + 63) / 64;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
+        const int left = n - 64 * i;
+        bits[i] = left >= 64 ? ~0ull : ((1ull << left) - 1);
     }
 }
 
@@ -1315,30 +1546,55 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
     const int tiles_x = (int)cdiv(w, OW), tiles_y = (int)cdiv(h, OH);
     const int rgx = (int)cdiv(tiles_x, G), rgy = (int)cdiv(tiles_y, G);
     const int nt = rgx * rgy;
-    const int max_pass = 4 * nt + 8;
+    const long long max_pass = pass_limit((long long)w * h);
     const int nwin = tiles_x * tiles_y;
     Scratch sc;
-    NSC_CHECK(sc.alloc(sizeof(uint4) * TW * (size_t)nwin + sizeof(unsigned) * nt + 2 * sizeof(int) * nt +
-                           sizeof(int) * (max_pass + 2),
-                       s));
+    NSC_CHECK(sc.alloc(sizeof(uint4) * TW * (size_t)nwin + recon_scratch_bytes(nt), s));
     uint4* flags = sc.as<uint4>();
-    unsigned* stamp = reinterpret_cast<unsigned*>(flags + TW * (size_t)nwin);
-    int* counts = reinterpret_cast<int*>(stamp + nt);  // next to the stamps: one fill clears both
-    int* listA = counts + max_pass + 2;
-    int* listB = listA + nt;
-    NSC_HIP(hipMemsetAsync(stamp, 0, sizeof(unsigned) * nt + sizeof(int) * (max_pass + 2), s));
+    const ReconBufs B = recon_carve(flags + TW * (size_t)nwin, nt);
+    recon_clear(B, s);
+    unsigned* stamp = B.stamp;
+    int* counts = B.counts;
+    int* listA = B.listA;
+    int* listB = B.listB;
     hipLaunchKernelGGL(geo_flags, dim3(nwin), dim3(TW), 0, s, pass, flags, w, h, tiles_x);
+    if (G == 2 && drain_env()) {
+        // every region dirty, one drain launch, then the bitmap must be empty
+        const int nw = (int)recon_words(nt);
+        hipLaunchKernelGGL(k_set_bits, dim3(std::min<unsigned>(cdiv(nw, 256), 256u)), dim3(256), 0, s, B.dirty, nt);
+        static int grid_d = [] {
+            int dev = 0, per_cu = 0, cus = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, geo_drain<2>, 256, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                return 512;
+            }
+            return std::max(1, per_cu) * std::max(1, cus);
+        }();
+        hipLaunchKernelGGL(geo_drain<2>, dim3(std::min(nt, grid_d)), dim3(256), 0, s, flags, dist, w, h, tiles_x, tiles_y,
+                           rgx, rgy, nw, B.dirty, B.busy, &B.flags[0], 4096);
+        hipLaunchKernelGGL(k_any_bits, dim3(std::min<unsigned>(cdiv(nw, 256), 256u)), dim3(256), 0, s, B.dirty, nw,
+                           &B.flags[1]);
+        NSC_HIP(hipGetLastError());
+        int left = 0;
+        NSC_HIP(hipMemcpyAsync(&left, &B.flags[1], sizeof(int), hipMemcpyDeviceToHost, s));
+        NSC_HIP(stream_sync(s));
+        if (!left) return NSC_OK;
+        // waves gave up with regions still marked: the passes finish from the current distances
+        ++g_drain_redo;
+    }
     const unsigned grid = (unsigned)std::min(nt, 1 << 16);
-    auto launch = [&](int p, const int* lst, const int* cnt, int* nl, int* nc) {
+    auto launch = [&](int p, const int* lst, const int* cnt, int* nl, int* nc, int* clr) {
         if (G == 2)
             hipLaunchKernelGGL(geo_regions<2>, dim3(grid), dim3(256), 0, s, flags, dist, w, h, tiles_x, tiles_y, rgx, rgy, lst,
-                               cnt, nl, nc, stamp, p);
+                               cnt, nl, nc, clr, stamp, p);
         else
             hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, flags, dist, w, h, tiles_x, tiles_y, lst, cnt, nl, nc,
-                               stamp, p);
+                               clr, stamp, p);
     };
-    launch(0, nullptr, nullptr, listA, &counts[0]);
-    int p = 1;
+    launch(0, nullptr, nullptr, listA, &counts[0], &counts[1]);
+    long long p = 1;
     // host checks after 8, 24 and 56 passes, then every 8: the front needs tens of passes to cross
     // a tile (about 60 on a 4096^2 nuclei tile), an empty pass costs a few microseconds and a host
     // round trip a few tens
@@ -1346,15 +1602,16 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
         for (int b = 0; b < batch && p <= max_pass; ++b, ++p) {
             int* cur = (p & 1) ? listA : listB;
             int* nxt = (p & 1) ? listB : listA;
-            launch(p, cur, &counts[p - 1], nxt, &counts[p]);
+            launch((int)std::min<long long>(p, INT_MAX - 1), cur, &counts[(p - 1) % kRing], nxt, &counts[p % kRing],
+                   &counts[(p + 1) % kRing]);
         }
         NSC_HIP(hipGetLastError());
         int c = 0;
-        NSC_HIP(hipMemcpyAsync(&c, &counts[p - 1], sizeof(int), hipMemcpyDeviceToHost, s));
+        NSC_HIP(hipMemcpyAsync(&c, &counts[(p - 1) % kRing], sizeof(int), hipMemcpyDeviceToHost, s));
         NSC_HIP(stream_sync(s));
         if (c == 0) break;
         if (p > max_pass) {
-            set_error("geodesic distance: no convergence after %d passes", max_pass);
+            set_error("geodesic distance: no convergence after %lld passes", max_pass);
             return NSC_ERR_LIMIT;
         }
     }

@@ -124,7 +124,6 @@ struct nsc_server {
     void worker(int k) {
         Slot& S = slots[k];
         (void)hipSetDevice(device);
-        nsc::set_thread_recon_schedule(0);
         nsc::set_thread_blocking_sync(true);
         for (;;) {
             Req* q = nullptr;

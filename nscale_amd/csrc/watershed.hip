@@ -168,6 +168,11 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
 static std::atomic<int> g_ws_fast{0}, g_ws_fallback{0}, g_ws_last_flags{0};
 void ws_stats_record(int ok) { if (ok) ++g_ws_fast; else ++g_ws_fallback; }
 void ws_last_flags_record(int flags) { g_ws_last_flags = flags; }
+// phase-1 runs per LDS tier, summed over the decomposed floods of the process (reporting only)
+static std::atomic<long long> g_ws_tier_runs[8];
+void ws_tier_record(const int* runs, int ntiers) {
+    for (int t = 0; t < ntiers && t < 8; ++t) g_ws_tier_runs[t] += runs[t];
+}
 
 size_t cvws_scratch(int W, int H) {
     const size_t n = (size_t)W * H;
@@ -255,8 +260,16 @@ void nsc_watershed_stats(int* fast, int* fallback, int* last_reason) {
     if (last_reason) *last_reason = g_ws_last_flags;
 }
 
+int nsc_watershed_tier_runs(long long* runs, int n) {
+    if (!runs || n <= 0) return 0;
+    const int k = n < 6 ? n : 6;
+    for (int t = 0; t < k; ++t) runs[t] = g_ws_tier_runs[t];
+    return 6;
+}
+
 int nsc_cv_watershed(const uint8_t* bgr, int32_t* markers, int w, int h, void* stream) {
     NSC_ARG(bgr && markers && w > 0 && h > 0, "cv_watershed");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     Scratch sc;
@@ -266,6 +279,7 @@ int nsc_cv_watershed(const uint8_t* bgr, int32_t* markers, int w, int h, void* s
 
 int nsc_watershed2(const uint8_t* bgr, const float* img, int32_t* out, int w, int h, int conn, void* stream) {
     NSC_ARG(bgr && img && out && w > 0 && h > 0 && valid_conn(conn), "watershed2");
+    NSC_LABEL_SIZE(w, h);
     NSC_CHECK(ensure_init());
     hipStream_t s = as_stream(stream);
     const size_t n = (size_t)w * h;

@@ -230,8 +230,8 @@ __global__ void k_phase1(Dev D, const int32_t* __restrict__ box5, int nrun, cons
     const int r = list[idx];
     const int W = D.W;
     const int id = r + 1;
-    int* head = qh + (size_t)idx * NQ;
-    int* tail = qt + (size_t)idx * NQ;
+    int* head = qh + (size_t)r * NQ;  // per run (run ids are unique across the tiers)
+    int* tail = qt + (size_t)r * NQ;
     for (int l = 0; l < NQ; ++l) { head[l] = -1; tail[l] = -1; }
     int active = NQ, seq = 0, popc = 0;
     auto enq = [&](int l, int x) {
@@ -651,6 +651,7 @@ __global__ void k_collect_b0(Dev D, int* __restrict__ blist, int* __restrict__ b
 // runs by the size of their LDS frame (box + 1-pixel margin): bucket b holds frames of at most
 // kRunTier[b] pixels; the last bucket floods from global memory
 constexpr int kRunTiers = 6;
+
 __constant__ int kRunTier[kRunTiers - 1] = {512, 1024, 2048, 4096, 6400};
 // (one append per wave and bucket: an atomic per run on the few bucket counters serialises)
 __global__ void k_bucket_runs(const int32_t* __restrict__ box5, int nrun, int* __restrict__ lists, int* __restrict__ counts) {
@@ -1455,6 +1456,7 @@ inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u
 // Returns NSC_OK with *ok = 1 when the decomposition produced the exact result in m, *ok = 0
 // when the caller must run the sequential flood instead (m is then undefined).
 void ws_last_flags_record(int flags);
+void ws_tier_record(const int* runs, int ntiers);
 
 int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t s);
 
@@ -1612,6 +1614,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         hipLaunchKernelGGL(k_geo_init, grid1(N), dim3(256), 0, s, D, &small[8], fmask, dist);
         const int grc = geodist_impl(fmask, dist, W, H, s);
         if (grc == NSC_ERR_LIMIT) {  // outside the tile search's range: the sequential flood decides
+            clear_error();  // the call still succeeds: no error text left behind
             ws_last_flags_record(FB_CMP);
             return NSC_OK;
         }
@@ -1623,9 +1626,11 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     // phase 2
     NSC_CHECK(ccl_neq(fmask, 0, lab2, W, H, 4, s));
     NSC_CHECK(relabel_impl(lab2, W, H, -1, &small[16], head /* rank scratch: head[] is dead after k_bresolve */, s));
-    int ncomp = 0;
-    NSC_HIP(hipMemcpyAsync(&ncomp, &small[16], sizeof(int), hipMemcpyDeviceToHost, s));
+    int sm16[16];  // small[16..31]: ncomp and, at 26..31, the phase-1 run counts per tier
+    NSC_HIP(hipMemcpyAsync(sm16, &small[16], sizeof(sm16), hipMemcpyDeviceToHost, s));
     NSC_HIP(stream_sync(s));
+    const int ncomp = sm16[0];
+    ws_tier_record(sm16 + 10, kRunTiers);
     if (ncomp > 0) {
         Scratch sc3;
         Carver m3(nullptr);

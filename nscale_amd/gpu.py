@@ -7,7 +7,8 @@ import ctypes
 
 import torch
 
-from nscale_amd._lib import ReconDesc, Request, Result, SegParams, ServerConfig, call, check, lib
+from nscale_amd._lib import (BBOX_ALL, BBOX_NONE, BBOX_REFERENCE_CPU, ObjFeaturesOut, ReconDesc, Request, Result,
+                             SegIO, SegParams, ServerConfig, call, check, lib)
 
 _P = ctypes.c_void_p
 
@@ -15,6 +16,18 @@ _P = ctypes.c_void_p
 def _ptr(t):
     assert t.is_cuda and t.is_contiguous(), "expects a contiguous device tensor"
     return _P(t.data_ptr())
+
+
+def _rows(t):
+    """Device pointer and row step in bytes of a 2-D (h, w) or BGR (h, w, 3) tensor whose rows are
+    contiguous but may be spaced further apart (a pitched view, like a GpuMat's step)."""
+    assert t.is_cuda
+    es = t.element_size()
+    if t.dim() == 3:
+        assert t.stride(2) == 1 and t.stride(1) == t.shape[2], "BGR pixels must be interleaved"
+    else:
+        assert t.stride(1) == 1, "rows must be contiguous"
+    return _P(t.data_ptr()), int(t.stride(0) * es)
 
 
 def _stream():
@@ -54,7 +67,8 @@ def imreconstruct_batched(pairs, conns):
 def autotune_recon(pairs, conns):
     """Pick the reconstruction schedule (windows per workgroup side, passes per host check) by timing
     each candidate on this batch; later calls of the same element type use the choice.
-    Returns {"region": G, "passes": P, "timings_ms": [...]} (candidates (1,4), (1,8), (2,4), (2,8))."""
+    Returns {"region": G, "passes": P, "timings_ms": [...]} (candidates (1,4), (1,8), (2,4), (2,8) and the
+    device-side work queue, reported as passes 0)."""
     n = len(pairs)
     arr = (ReconDesc * n)()
     outs = [torch.empty_like(mask) for _, mask in pairs]
@@ -62,10 +76,15 @@ def autotune_recon(pairs, conns):
         h, w = _hw(mask)
         arr[i] = ReconDesc(_ptr(marker), _ptr(mask), _ptr(o), w, h, int(conns[i]))
     g, p = ctypes.c_int(0), ctypes.c_int(0)
-    t = (ctypes.c_float * 4)()
+    t = (ctypes.c_float * 5)()
     call("nsc_recon_autotune", arr, n, 1 if pairs[0][1].dtype == torch.float32 else 0, _stream(), ctypes.byref(g),
          ctypes.byref(p), t)
     return {"region": g.value, "passes": p.value, "timings_ms": [round(x, 3) for x in t]}
+
+
+def set_recon_schedule(is_float=False, region=0, passes=0, queue=-1):
+    """Sets the calling thread's reconstruction schedule (see nsc_recon_set_schedule)."""
+    call("nsc_recon_set_schedule", 1 if is_float else 0, int(region), int(passes), int(queue))
 
 
 def imhmin_batched(imgs, hval, conns):
@@ -298,6 +317,110 @@ def segment_nuclei(bgr, params=None, ws=None, stages=(), timing=False):
         from nscale_amd._lib import check
         check(st, "nsc_segment_nuclei")
     return st, labels, cc.value, bufs, {i: ms[i] for i in range(32) if ms[i] > 0}
+
+
+def segment_nuclei_io(bgr, labels=None, params=None, ws=None, canonical=False, bbox_mode=BBOX_NONE, max_boxes=0,
+                      with_area=False, stages=()):
+    """HistologicalEntities::segmentNuclei with pitched images and the reference's compcount/bbox.
+
+    bgr and labels may be pitched views (rows contiguous, spaced by their row stride).  bbox_mode:
+    BBOX_REFERENCE_CPU = ConnComponents::boundingBox(labels, 0) as the reference CPU chain calls it,
+    BBOX_ALL = one box per nucleus (nscale::gpu::boundingBox2 on the returned labels).
+    Returns (status, labels, compcount, nuclei, bbox (5 x compcount) or None, area or None, stages)."""
+    h, w = int(bgr.shape[0]), int(bgr.shape[1])
+    p = params or default_params()
+    ws = ws or Workspace(w, h)
+    if labels is None:
+        labels = torch.empty(h, w, dtype=torch.int32, device=bgr.device)
+    bp, bstep = _rows(bgr)
+    lp, lstep = _rows(labels)
+    box = area = None
+    if bbox_mode != BBOX_NONE:
+        box = torch.empty(5 * max(max_boxes, 1), dtype=torch.int32, device=bgr.device)
+        if with_area:
+            area = torch.empty(max(max_boxes, 1), dtype=torch.int32, device=bgr.device)
+    io = SegIO(bstep, lstep, 1 if canonical else 0, bbox_mode, _P(box.data_ptr()) if box is not None else None,
+               _P(area.data_ptr()) if area is not None else None, max_boxes, 0, 0)
+    arr = (_P * 32)()
+    bufs = {}
+    for s_ in stages:
+        if s_ in (0, 19):
+            bufs[s_] = torch.zeros(h, w, 3, dtype=torch.uint8, device=bgr.device)
+        else:
+            bufs[s_] = torch.zeros(h, w, dtype=STAGE_TYPES.get(s_, torch.uint8), device=bgr.device)
+        arr[s_] = bufs[s_].data_ptr()
+    st = lib().nsc_segment_nuclei_io(bp, w, h, ctypes.byref(p), lp, ctypes.byref(io), ws.handle,
+                                     arr if stages else None, None, _stream())
+    if st < -2:
+        check(st, "nsc_segment_nuclei_io")
+    n = io.compcount
+    if box is not None:
+        box = box[:5 * n].view(5, n)
+        if area is not None:
+            area = area[:n]
+    return st, labels, n, io.nuclei, box, area, bufs
+
+
+def pl_find_nucleus_candidates(bgr, params=None, ws=None, out=None):
+    """HistologicalEntities::plFindNucleusCandidates (S1): (status, seg_norbc).  status 4 (CONTINUE)
+    means the chain goes on; pitched views are accepted."""
+    h, w = int(bgr.shape[0]), int(bgr.shape[1])
+    p = params or default_params()
+    ws = ws or Workspace(w, h)
+    out = out if out is not None else torch.empty(h, w, dtype=torch.uint8, device=bgr.device)
+    bp, bstep = _rows(bgr)
+    op, ostep = _rows(out)
+    st = lib().nsc_pl_find_nucleus_candidates(bp, bstep, w, h, ctypes.byref(p), op, ostep, ws.handle, _stream())
+    if st < -2:
+        check(st, "nsc_pl_find_nucleus_candidates")
+    return st, out
+
+
+def pl_separate_nuclei(bgr, seg_open, min_size_pl=30, conn=8, ws=None, out=None):
+    """HistologicalEntities::plSeparateNuclei (A4): (status, seg_nonoverlap)."""
+    h, w = int(bgr.shape[0]), int(bgr.shape[1])
+    ws = ws or Workspace(w, h)
+    out = out if out is not None else torch.empty(h, w, dtype=torch.uint8, device=bgr.device)
+    bp, bstep = _rows(bgr)
+    sp, sstep = _rows(seg_open)
+    op, ostep = _rows(out)
+    st = lib().nsc_pl_separate_nuclei(bp, bstep, sp, sstep, w, h, int(min_size_pl), int(conn), op, ostep, ws.handle,
+                                      _stream())
+    if st < -2:
+        check(st, "nsc_pl_separate_nuclei")
+    return st, out
+
+
+def object_features(labels, bbox, img=None, channel=0, intensity=True, gradient=True):
+    """nscale::ObjFeatures on the device: labels (h, w) int32, bbox (5, n) int32 [label | minx | maxx |
+    miny | maxy], img an interleaved (h, w, c) or (h, w) uint8 image whose plane `channel` is measured.
+    Returns a dict of device tensors: area, perimeter, major_axis, minor_axis, eccentricity,
+    extent_ratio, circularity, and intensity (n, 8) / gradient (n, 6) when an image is given."""
+    h, w = _hw(labels)
+    n = int(bbox.shape[1])
+    box = bbox.contiguous()
+    dev = labels.device
+    out = {"area": torch.empty(max(n, 1), dtype=torch.int32, device=dev)}
+    for k in ("perimeter", "major_axis", "minor_axis", "eccentricity", "extent_ratio", "circularity"):
+        out[k] = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
+    chans = 1
+    if img is not None:
+        chans = 1 if img.dim() == 2 else int(img.shape[2])
+        if intensity:
+            out["intensity"] = torch.empty(max(n, 1), 8, dtype=torch.float32, device=dev)
+        if gradient:
+            out["gradient"] = torch.empty(max(n, 1), 6, dtype=torch.float32, device=dev)
+    F = ObjFeaturesOut(*[_P(out[k].data_ptr()) if k in out else None for k, _ in ObjFeaturesOut._fields_])
+    call("nsc_object_features", _ptr(labels), w, h, _ptr(box), n, _ptr(img) if img is not None else None, chans,
+         int(channel), ctypes.byref(F), _stream())
+    return {k: v[:n] for k, v in out.items()}
+
+
+def watershed_tier_runs():
+    """Phase-1 runs per LDS tier (<=512, 1024, 2048, 4096, 6400 px, global bucket), process-wide sums."""
+    arr = (ctypes.c_longlong * 6)()
+    lib().nsc_watershed_tier_runs(arr, 6)
+    return list(arr)
 
 
 def watershed_stats(with_reason=False):

@@ -25,3 +25,15 @@ def synth_he(h, w, seed):
     out = np.empty((h, w, 3), np.uint8)
     call("nsc_synth_he", out.ctypes.data_as(ctypes.c_void_p), w, h, seed)
     return out
+
+
+def serve_sizes(n, seed):
+    """Tile sizes (h, w) of the batched / served workloads: short side log-uniform in [512, 4096],
+    long side = short * U[1, 2], random orientation."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        a = int(round(float(np.exp(rng.uniform(np.log(512), np.log(4096))))))
+        b = int(round(a * float(rng.uniform(1.0, 2.0))))
+        out.append((a, b) if rng.random() < 0.5 else (b, a))
+    return out

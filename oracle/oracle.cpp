@@ -857,4 +857,275 @@ int orc_segment_nuclei(const uint8_t* img, int w, int h, const orc_seg_params* p
     return 0;  // SUCCESS
 }
 
+// The chain's compcount and bbox outputs: ConnComponents::boundingBox(w, h, labels, 0, compcount)
+// on the final labels, as HistologicalEntities.cpp:589-591 calls it (bgval 0 while the labels use
+// -1 for background: the scan stops at the first -1 pixel, ConnComponents.cpp:250-253).  bbox5
+// receives the reference's block, 5 x compcount ints with column stride compcount, when
+// compcount <= cap.
+int orc_segment_nuclei_bbox(const uint8_t* bgr, int w, int h, const orc_seg_params* p, int32_t* labels, int* compcount,
+                            int32_t* bbox5, int cap, void** stage, double* stage_ms) {
+    const int st = orc_segment_nuclei(bgr, w, h, p, labels, stage, stage_ms);
+    if (compcount) *compcount = 0;
+    if (st != 0) return st;
+    const int n = orc_cc_bounding_box_literal(w, h, labels, 0, nullptr, 0);
+    if (compcount) *compcount = n;
+    if (bbox5 && n > 0 && n <= cap) {
+        std::vector<int32_t> tmp(5 * (size_t)n);
+        orc_cc_bounding_box_literal(w, h, labels, 0, tmp.data(), n);
+        std::memcpy(bbox5, tmp.data(), sizeof(int32_t) * 5 * n);
+    }
+    return st;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Per-nucleus features (src/features/ObjFeatures.cpp, Operators.cpp) on a label image and the
+// reference's box block (5 x n, column stride n: [label | minx | maxx | miny | maxy]).
+extern "C" {
+
+// ObjFeatures::area (ObjFeatures.cpp:15-45): pixels of the label inside its box
+void orc_feat_area(const int32_t* bbox5, int n, const int32_t* labels, int w, int h, int32_t* area) {
+    (void)h;
+    for (int i = 0; i < n; ++i) {
+        const int label = bbox5[i], minx = bbox5[n + i], maxx = bbox5[2 * n + i], miny = bbox5[3 * n + i],
+                  maxy = bbox5[4 * n + i];
+        int a = 0;
+        for (int y = miny; y <= maxy; ++y)
+            for (int x = minx; x <= maxx; ++x) a += labels[(size_t)y * w + x] == label;
+        area[i] = a;
+    }
+}
+
+// ObjFeatures::ellipse (ObjFeatures.cpp:48-116): moments about the box centre (float offsets,
+// double sums), major / minor axis and eccentricity
+void orc_feat_ellipse(const int32_t* bbox5, const int32_t* area, int n, const int32_t* labels, int w, int h,
+                      double* major, double* minor, double* ecc) {
+    (void)h;
+    const double frac = 1.0 / 12.0, root = std::sqrt(8.0);
+    for (int i = 0; i < n; ++i) {
+        const int label = bbox5[i];
+        const int minX = bbox5[n + i], maxX = bbox5[2 * n + i], minY = bbox5[3 * n + i], maxY = bbox5[4 * n + i];
+        const float midX = (float)(minX + maxX) / 2.0f, midY = (float)(minY + maxY) / 2.0f;
+        double sx = 0, sy = 0, sxy = 0, ssqx = 0, ssqy = 0;
+        for (int y = minY; y <= maxY; ++y) {
+            const float cy = (float)y - midY;
+            for (int x = minX; x <= maxX; ++x) {
+                const float cx = (float)x - midX;
+                if (labels[(size_t)y * w + x] == label) {
+                    sx += cx;
+                    sy += cy;
+                    sxy += cx * cy;
+                    ssqx += cx * cx;
+                    ssqy += cy * cy;
+                }
+            }
+        }
+        const double a = area[i];
+        const double xbar = sx / a, ybar = sy / a;
+        const double mxx = ((ssqx + a * xbar * xbar - 2.0 * xbar * sx) / a) + frac;
+        const double myy = ((ssqy + a * ybar * ybar - 2.0 * ybar * sy) / a) + frac;
+        const double mxy = (sxy - ybar * sx - xbar * sy + xbar * ybar * a) / a;
+        const double delta = std::sqrt((mxx - myy) * (mxx - myy) + 4.0 * mxy * mxy);
+        major[i] = root * std::sqrt(mxx + myy + delta);
+        minor[i] = root * std::sqrt(mxx + myy - delta);
+        ecc[i] = (2.0 * std::sqrt(std::fabs(major[i] * major[i] - minor[i] * minor[i]) / 4.0)) / major[i];
+    }
+}
+
+// ObjFeatures::extent_ratio (ObjFeatures.cpp:118-141) and ::circularity (:281-293)
+void orc_feat_extent_circ(const int32_t* bbox5, const int32_t* area, const double* perimeter, int n, double* extent,
+                          double* circ) {
+    for (int i = 0; i < n; ++i) {
+        const int width = bbox5[2 * n + i] - bbox5[n + i] + 1, height = bbox5[4 * n + i] - bbox5[3 * n + i] + 1;
+        extent[i] = (double)area[i] / (double)(width * height);
+        circ[i] = (4.0 * 3.14159265359 * (double)area[i]) / (perimeter[i] * perimeter[i]);
+    }
+}
+
+// ObjFeatures::perimeter (ObjFeatures.cpp:144-278): marching squares over the box (cell code from
+// lower-left, lower-right, upper-right, upper-left as bits 3..0, looked up in a table), plus the
+// box's edge and corner half-cells, summed in the reference's order
+void orc_feat_perimeter(const int32_t* bbox5, int n, const int32_t* labels, int w, int h, double* perimeter) {
+    (void)h;
+    double lut[16] = {0};
+    const double hs = 0.70710678118, s2 = 1.41421356237;
+    lut[1] = lut[2] = lut[4] = lut[8] = hs;
+    lut[3] = lut[6] = lut[9] = lut[12] = 1.0;
+    lut[7] = lut[11] = lut[13] = lut[14] = hs;
+    lut[5] = lut[10] = s2;
+    for (int i = 0; i < n; ++i) {
+        const int label = bbox5[i];
+        const int xmin = bbox5[n + i], xmax = bbox5[2 * n + i], ymin = bbox5[3 * n + i], ymax = bbox5[4 * n + i];
+        auto L = [&](int x, int y) { return labels[(size_t)y * w + x] == label ? 1 : 0; };
+        double p = 0.0;
+        for (int y = ymin; y < ymax; ++y)
+            for (int x = xmin; x < xmax; ++x) {
+                int m = L(x, y);
+                m = (m << 1) | L(x + 1, y);
+                m = (m << 1) | L(x + 1, y + 1);
+                m = (m << 1) | L(x, y + 1);
+                p += lut[m];
+            }
+        for (int x = xmin; x < xmax; ++x) {
+            int m = L(x, ymax);
+            m = (m << 1) | L(x + 1, ymax);
+            m <<= 2;
+            p += lut[m];
+            m = L(x + 1, ymin);
+            m = (m << 1) | L(x, ymin);
+            p += lut[m];
+        }
+        for (int y = ymin; y < ymax; ++y) {
+            int m = L(xmin, y);
+            m = (m << 1) | L(xmin, y + 1);
+            m <<= 1;
+            p += lut[m];
+            m = L(xmax, y);
+            m <<= 2;
+            m = (m << 1) | L(xmax, y + 1);
+            p += lut[m];
+        }
+        p += lut[L(xmin, ymin) << 1];
+        p += lut[L(xmax, ymin)];
+        p += lut[L(xmax, ymax) << 3];
+        p += lut[L(xmin, ymax) << 2];
+        perimeter[i] = p;
+    }
+}
+
+// The histogram statistics of Operators.cpp:281-505 as ObjFeatures::intensityFeatures
+// (ObjFeatures.cpp:342-375: mean, max, min, std, entropy, energy, skewness, kurtosis) and
+// ::gradientFeatures (:377-418: the same without max and min) use them
+static void hist_stats(const int* hist, float* f, bool with_minmax) {
+    double hm = 0.0;  // calcMeanFromHistogram
+    int ne = 0;
+    for (int i = 0; i < 256; ++i) {
+        hm += hist[i] * i;
+        ne += hist[i];
+    }
+    const double mean = ne == 0 ? 0.0 : hm / ne;
+    unsigned mx = 0, mn = 256;  // calcMaxFromHistogram / calcMinFromHistogram
+    for (int i = 255; i >= 0; --i)
+        if (hist[i] > 0) { mx = i; break; }
+    for (int i = 0; i < 256; ++i)
+        if (hist[i] > 0) { mn = i; break; }
+    double sum = 0.0, nd = 0;  // calcStdFromHistogram
+    for (int i = 0; i < 256; ++i) {
+        sum += i * hist[i];
+        nd += hist[i];
+    }
+    const double m2 = sum / nd;
+    double sq = 0.0;
+    for (int i = 0; i < 256; ++i) {
+        const double d = i - m2;
+        sq += d * d * hist[i];
+    }
+    const double sd = std::sqrt(sq / nd);
+    const float nf = (float)ne;  // calcEntropyFromHistogram / calcEnergyFromHistogram (float)
+    float ent = 0.0f, energy = 0.0f;
+    for (int i = 0; i < 256; ++i) {
+        const float p = (float)hist[i] / nf;
+        if (hist[i] != 0) ent += p * std::log2(p);
+    }
+    for (int i = 0; i < 256; ++i) energy += std::pow((float)hist[i] / nf, 2);
+    float n2 = 0.f, avg = 0.f;  // calcSkewnessFromHistogram / calcKurtosisFromHistogram (float)
+    for (int i = 0; i < 256; ++i) {
+        n2 += (float)hist[i];
+        avg += i * (float)hist[i];
+    }
+    avg /= n2;
+    float d3 = 0.f, d2 = 0.f, d4 = 0.f;
+    for (int i = 0; i < 256; ++i) {
+        const float e = (float)hist[i];
+        d3 += e * std::pow(i - avg, 3);
+        d2 += e * std::pow(i - avg, 2);
+        d4 += e * std::pow(i - avg, 4);
+    }
+    float skew = 0.f, kurt = 0.f;
+    if (d3 != 0 && d2 != 0 && n2 != 0) {
+        float a = d3 / n2, b = d2 / n2;
+        b = std::sqrt(b);
+        b = std::pow(b, 3);
+        skew = a / b;
+    }
+    if (d4 != 0 && d2 != 0 && n2 != 0) {
+        float a = d4 / n2, b = d2 / n2;
+        b = std::pow(b, 2);
+        kurt = a / b;
+    }
+    int k = 0;
+    f[k++] = (float)mean;
+    if (with_minmax) {
+        f[k++] = (float)mx;
+        f[k++] = (float)mn;
+    }
+    f[k++] = (float)sd;
+    f[k++] = -1.0f * ent;
+    f[k++] = energy;
+    f[k++] = skew;
+    f[k++] = kurt;
+}
+
+// Operators::buildHistogram256CPU (Operators.cpp:515-531) of each nucleus, then its statistics:
+// out8[8 i + k] = mean, max, min, std, entropy, energy, skewness, kurtosis
+void orc_feat_intensity(const int32_t* bbox5, int n, const int32_t* labels, const uint8_t* gray, int w, int h,
+                        float* out8) {
+    (void)h;
+    for (int i = 0; i < n; ++i) {
+        int hist[256] = {0};
+        const int label = bbox5[i];
+        for (int y = bbox5[3 * n + i]; y <= bbox5[4 * n + i]; ++y)
+            for (int x = bbox5[n + i]; x <= bbox5[2 * n + i]; ++x)
+                if (labels[(size_t)y * w + x] == label) hist[gray[(size_t)y * w + x]]++;
+        hist_stats(hist, out8 + 8 * (size_t)i, true);
+    }
+}
+
+// Operators::gradient (Operators.cpp:568-640): central differences [-0.5 0 0.5] along x and y
+// with replicated borders (cvFilter2D), the first and last column (row) doubled into one-sided
+// differences, magnitude in float, rounded half up and clamped to uint8
+void orc_gradient_u8(const uint8_t* gray, uint8_t* out, int w, int h) {
+    auto G = [&](int xx, int yy) {
+        xx = std::min(std::max(xx, 0), w - 1);
+        yy = std::min(std::max(yy, 0), h - 1);
+        return (float)gray[(size_t)yy * w + xx];
+    };
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            float dx = 0.5f * G(x + 1, y) - 0.5f * G(x - 1, y);
+            if (x == 0 || x == w - 1) dx *= 2;
+            float dy = 0.5f * G(x, y + 1) - 0.5f * G(x, y - 1);
+            if (y == 0 || y == h - 1) dy *= 2;
+            float mag = 0.f;
+            mag += dx * dx;
+            mag += dy * dy;
+            mag = std::sqrt(mag);
+            int v;
+            if (mag < 0.f) v = 0;
+            else if (mag > 255.f) v = 255;
+            else {
+                v = (int)std::floor(mag);
+                if (mag - std::floor(mag) >= 0.5f) v += 1;
+            }
+            out[(size_t)y * w + x] = (uint8_t)v;
+        }
+}
+
+// ObjFeatures::gradientFeatures (ObjFeatures.cpp:377-418) on the gradient image:
+// out6[6 i + k] = mean, std, entropy, energy, skewness, kurtosis
+void orc_feat_gradient(const int32_t* bbox5, int n, const int32_t* labels, const uint8_t* gray, int w, int h,
+                       float* out6) {
+    std::vector<uint8_t> g((size_t)w * h);
+    orc_gradient_u8(gray, g.data(), w, h);
+    for (int i = 0; i < n; ++i) {
+        int hist[256] = {0};
+        const int label = bbox5[i];
+        for (int y = bbox5[3 * n + i]; y <= bbox5[4 * n + i]; ++y)
+            for (int x = bbox5[n + i]; x <= bbox5[2 * n + i]; ++x)
+                if (labels[(size_t)y * w + x] == label) hist[g[(size_t)y * w + x]]++;
+        hist_stats(hist, out6 + 6 * (size_t)i, false);
+    }
+}
+
 }  // extern "C"

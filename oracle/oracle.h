@@ -89,6 +89,11 @@ void orc_seg_params_default(orc_seg_params* p);
    per-stage wall times in milliseconds, indexed by the same ids.  Returns the status code. */
 int orc_segment_nuclei(const uint8_t* bgr, int w, int h, const orc_seg_params* p, int32_t* labels,
                        void** stage, double* stage_ms);
+/* segmentNuclei's compcount and bbox outputs (HistologicalEntities.cpp:589-591): the literal
+   ConnComponents::boundingBox(w, h, labels, 0, compcount) on the final labels.  bbox5 (may be
+   NULL) receives 5 x compcount ints, column stride compcount, when compcount <= cap. */
+int orc_segment_nuclei_bbox(const uint8_t* bgr, int w, int h, const orc_seg_params* p, int32_t* labels, int* compcount,
+                            int32_t* bbox5, int cap, void** stage, double* stage_ms);
 
 #ifdef __cplusplus
 }

@@ -34,6 +34,14 @@ def oracle():
         _oracle.orc_cc_area_threshold_labeled.restype = ctypes.c_int
         _oracle.orc_cc_bounding_box_literal.restype = ctypes.c_int
         _oracle.orc_segment_nuclei.restype = ctypes.c_int
+        _oracle.orc_segment_nuclei_bbox.restype = ctypes.c_int
+        _oracle.orc_feat_area.argtypes = [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P]
+        _oracle.orc_feat_ellipse.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
+        _oracle.orc_feat_perimeter.argtypes = [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P]
+        _oracle.orc_feat_extent_circ.argtypes = [_P, _P, _P, ctypes.c_int, _P, _P]
+        _oracle.orc_feat_intensity.argtypes = [_P, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int, _P]
+        _oracle.orc_feat_gradient.argtypes = [_P, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int, _P]
+        _oracle.orc_gradient_u8.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int]
         _oracle.orc_imhmin_f32.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int]
         _oracle.orc_get_background.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint8, ctypes.c_uint8, ctypes.c_uint8]
         _oracle.orc_get_rbc.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
@@ -178,6 +186,48 @@ def o_segment(bgr, params=None, stages=()):
     return st, lab, bufs
 
 
+def o_segment_bbox(bgr, params=None, cap=1 << 16):
+    """Oracle chain plus its compcount and bbox outputs exactly as HistologicalEntities.cpp:589-591
+    computes them (ConnComponents::boundingBox(labels, bgval 0)): (status, labels, compcount, bbox
+    as a 5 x compcount array)."""
+    h, w = bgr.shape[:2]
+    p = params or default_params()
+    lab = np.full((h, w), -7, np.int32)
+    cc = ctypes.c_int(0)
+    box = np.zeros(5 * cap, np.int32)
+    st = oracle().orc_segment_nuclei_bbox(ptr(np.ascontiguousarray(bgr)), w, h, ctypes.byref(p), ptr(lab),
+                                          ctypes.byref(cc), ptr(box), cap, None, None)
+    n = cc.value
+    return st, lab, n, box[:5 * n].reshape(5, n)
+
+
+def ref_test_params():
+    """The parameter set of the reference's own chain test, nu-segmentTest.cpp:207:
+    segmentNuclei(img, mask, 220, 220, 220, 4.0, 3.5, 44, 5, 9000, 31, 20, 5, 1150, 4, 4, 4)."""
+    p = SegParams()
+    p.blue, p.green, p.red = 220, 220, 220
+    p.T1, p.T2 = 4.0, 3.5
+    p.G1, p.minSize, p.maxSize = 44, 5, 9000
+    p.G2, p.minSizePl, p.minSizeSeg, p.maxSizeSeg = 31, 20, 5, 1150
+    p.fillHolesConnectivity, p.reconConnectivity, p.watershedConnectivity = 4, 4, 4
+    return p
+
+
+def boxes_numpy(lab, bg=-1):
+    """[label | minx | maxx | miny | maxy] (5 x n) and areas of every label != bg, ascending label order."""
+    ys, xs = np.nonzero(lab != bg)
+    v = lab[ys, xs]
+    labels, inv, area = np.unique(v, return_inverse=True, return_counts=True)
+    n = len(labels)
+    out = np.zeros((5, n), np.int64)
+    out[0] = labels
+    out[1] = np.full(n, 1 << 40); np.minimum.at(out[1], inv, xs)
+    out[2] = np.full(n, -1); np.maximum.at(out[2], inv, xs)
+    out[3] = np.full(n, 1 << 40); np.minimum.at(out[3], inv, ys)
+    out[4] = np.full(n, -1); np.maximum.at(out[4], inv, ys)
+    return out.astype(np.int32), area.astype(np.int32)
+
+
 # ---------------------------------------------------------------- structuring elements
 DISK19 = np.array([[0, 0, 0, 0] + [1] * 11 + [0, 0, 0, 0],
                    [0, 0, 0] + [1] * 13 + [0, 0, 0],
@@ -193,3 +243,33 @@ assert DISK19.shape == (19, 19)
 
 def structure(conn):
     return np.ones((3, 3), bool) if conn == 8 else CROSS3.astype(bool)
+
+
+# ---------------------------------------------------------------- long propagation paths
+def serpentine(h=124, w=62, top=50, bottom=70, value=200):
+    """A one-pixel path of vertical segments at the odd columns, rows top..bottom, joined alternately
+    at the top and bottom rows: with 62x62 tiles every segment crosses the seam at row 62, so the
+    front needs about one pass per segment.  Returns (marker, mask, expected reconstruction): the
+    path is one component holding the marker value, so the whole path takes it."""
+    mask = np.zeros((h, w), np.uint8)
+    xs = list(range(1, w, 2))
+    for k, x in enumerate(xs):
+        mask[top:bottom + 1, x] = value
+        if k + 1 < len(xs):
+            mask[top if k % 2 == 0 else bottom, x + 1] = value
+    marker = np.zeros_like(mask)
+    marker[bottom, xs[0]] = value
+    return marker, mask, mask.copy()
+
+
+def boustrophedon(n=1024, value=200):
+    """A one-pixel maze: corridors along the even rows joined alternately at the right and left ends,
+    one path of about n*n/2 pixels that crosses every vertical tile seam once per corridor.
+    Returns (marker, mask, expected reconstruction)."""
+    mask = np.zeros((n, n), np.uint8)
+    mask[0::2, :] = value
+    for k, y in enumerate(range(1, n - 1, 2)):
+        mask[y, n - 1 if k % 2 == 0 else 0] = value
+    marker = np.zeros_like(mask)
+    marker[0, 0] = value
+    return marker, mask, mask.copy()

@@ -67,6 +67,20 @@ def test_cpp_operator_surface_compiles_and_links(tmp_path):
         "  nscale::hip::Image<uint8_t> o(nullptr, 0, 0);\n"
         "  try { nscale::hip::imreconstruct<uint8_t>(a, a, o, 8, nullptr); return 2; }\n"
         "  catch (const nscale::hip::Error& e) { std::printf(\"%s|%d|%d\\n\", nsc_version(), e.code, p.minSize); }\n"
+        "  if (p.minSize < 0) {  // never true: instantiates and links the whole surface, runs none of it\n"
+        "    nscale::hip::Image<const float> f(nullptr, 0, 0, 64);\n"
+        "    nscale::hip::Image<float> fo(nullptr, 0, 0);\n"
+        "    nscale::hip::Image<int32_t> l(nullptr, 0, 0, 256);\n"
+        "    int n = 0; int* bb = nullptr;\n"
+        "    nscale::hip::imreconstruct<float>(f, f, fo, 8, nullptr);\n"
+        "    nscale::hip::imfillHoles(a, o, 4, nullptr); nscale::hip::bwlabel2(a, l, 8, nullptr);\n"
+        "    nscale::hip::bwareaopen2(a, o, 1, 2, 8, n, nullptr); nscale::hip::distanceTransform(a, fo, nullptr);\n"
+        "    nscale::hip::watershed2(a, f, l, 8, nullptr); nscale::hip::imhmin(f, fo, 1.f, 8, nullptr);\n"
+        "    nscale::hip::boundingBox(l, 0, n, nullptr); nscale::hip::freeData(nscale::hip::boundingBox2(l, n, nullptr));\n"
+        "    nscale::hip::HistologicalEntities::segmentNuclei(a, l, n, bb, NSC_BBOX_ALL, nullptr, nullptr);\n"
+        "    nscale::hip::HistologicalEntities::plFindNucleusCandidates(a, o, nullptr, nullptr);\n"
+        "    nscale::hip::HistologicalEntities::plSeparateNuclei(a, a, o, nullptr, nullptr);\n"
+        "  }\n"
         "  return 0;\n}\n")
     exe = tmp_path / "shim"
     libdir = os.path.join(ROOT, "nscale_amd")

@@ -66,3 +66,64 @@ def test_recon_u8_full_tile_4096():
     print("recon u8 4096^2", flush=True)
     got = gpu_recon(marker, mask, 8)
     assert np.array_equal(got, o_recon(marker, mask, 8))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_i32_vs_oracle(conn):
+    rng = np.random.default_rng(90 + conn)
+    mask = rng.integers(-50000, 50000, (333, 517)).astype(np.int32)
+    mask[rng.random(mask.shape) < 0.4] = 2 ** 31 - 1
+    marker = (mask.astype(np.int64) - rng.integers(0, 100000, mask.shape)).clip(-2 ** 31, 2 ** 31 - 1).astype(np.int32)
+    print("recon i32", conn, flush=True)
+    assert np.array_equal(gpu_recon(marker, mask, conn), o_recon(marker, mask, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_serpentine_crosses_a_seam_per_segment(conn):
+    """31 segments each crossing the seam between two tile rows: about 31 passes on 2 tiles."""
+    from tests._lib import serpentine
+    marker, mask, exp = serpentine()
+    print("recon serpentine", conn, flush=True)
+    got = gpu_recon(marker, mask, conn)
+    assert np.array_equal(got, exp)
+    assert np.array_equal(got, o_recon(marker, mask, conn))
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_recon_maze_1024(conn):
+    """A 1024^2 one-pixel maze: thousands of passes, far above a per-tile pass budget."""
+    from tests._lib import boustrophedon
+    marker, mask, exp = boustrophedon(1024)
+    print("recon maze 1024", conn, flush=True)
+    got = gpu_recon(marker, mask, conn)
+    assert np.array_equal(got, exp)
+    assert np.array_equal(got, o_recon(marker, mask, conn))
+
+
+@pytest.fixture
+def work_queue():
+    """The reconstruction's device-side work queue for the calling thread, reset afterwards."""
+    from nscale_amd import gpu
+    gpu.set_recon_schedule(False, 1, 0, 1)
+    gpu.set_recon_schedule(True, 1, 0, 1)
+    yield
+    gpu.set_recon_schedule(False, 0, 0, -1)
+    gpu.set_recon_schedule(True, 0, 0, -1)
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_work_queue_schedule_matches_oracle(work_queue, conn):
+    """The device-side work queue (one launch drains the marked tiles) gives the oracle's bytes on
+    the long-path fixtures, a 4096^2 ellipse tile and float data."""
+    from nscale_amd.synth import synth_mask, synth_marker
+    from tests._lib import boustrophedon, serpentine
+    print("work queue", conn, flush=True)
+    for marker, mask, exp in (serpentine(), boustrophedon(512)):
+        assert np.array_equal(gpu_recon(marker, mask, conn), exp)
+    mask = synth_mask(4096, 4096, 2026)
+    marker = synth_marker(mask)
+    assert np.array_equal(gpu_recon(marker, mask, conn), o_recon(marker, mask, conn))
+    rng = np.random.default_rng(70 + conn)
+    fm = rng.uniform(-4, 4, (300, 277)).astype(np.float32)
+    fk = (fm - rng.uniform(0, 3, fm.shape)).astype(np.float32)
+    assert np.array_equal(gpu_recon(fk, fm, conn), o_recon(fk, fm, conn))

@@ -480,3 +480,31 @@ def test_chain_synthetic_structure():
     assert ((s[20] >= 0) | (s[15] == 0)).sum() < lab.size  # some dams inside seg_big
     st2, lab2, _ = o_segment(bgr)
     assert np.array_equal(lab, lab2)  # deterministic
+
+
+def test_chain_literal_compcount_matches_reference_code():
+    """segmentNuclei's compcount/bbox (HistologicalEntities.cpp:589-591) on the oracle chain's labels,
+    against the reference's own ConnComponents::boundingBox on the same labels.  With bwlabel2's
+    labels (smallest raster index, bg -1) the scan meets label 0 or stops at the first -1 before any
+    other label, so the reference CPU chain reports compcount 0."""
+    from nscale_amd.synth import synth_he
+    from tests._lib import o_segment_bbox
+    ref = ref_cc()
+    for k, (h, w) in enumerate([(300, 300), (256, 384)]):
+        bgr = synth_he(h, w, 60 + k).copy()
+        if k == 1:
+            bgr[:40, :40] = (140, 60, 90)
+        st, lab, n, box = o_segment_bbox(bgr)
+        assert st == 0 and n == 0 and box.shape == (5, 0)
+        if ref is not None:
+            out = np.zeros(5 * h * w, np.int32)
+            assert ref.ref_cc_bounding_box(w, h, ptr(lab), 0, ptr(out), h * w) == 0
+
+
+def test_long_path_fixtures_reconstruct_whole_path():
+    """The serpentine and maze fixtures of the GPU pass-limit tests: the oracle reconstructs the whole
+    one-pixel path (it is one component holding the seed value) under both connectivities."""
+    from tests._lib import boustrophedon, serpentine
+    for mk, m, e in (serpentine(), boustrophedon(256)):
+        for conn in (4, 8):
+            assert np.array_equal(o_recon(mk, m, conn), e)

@@ -68,6 +68,16 @@ int nsc_imreconstruct_f32(const float* marker, const float* mask, float* out, in
 /* nscale::imreconstruct<int>(seeds, image, conn)  MorphologicOperations.h:34, .cpp:1730-1903 */
 int nsc_imreconstruct_i32(const int32_t* marker, const int32_t* mask, int32_t* out, int w, int h, int conn,
                           void* stream);
+/* nscale::imreconstruct<uchar> without any allocation.  No counterpart in the reference: for
+   callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  The pass driver's scratch is a fixed device array of the
+   library (about 0.8 MB per GPU, held until the process ends), so images are limited to 65536
+   tiles of 62 x 62 pixels (e.g. 15872 x 15872); larger ones return NSC_ERR_LIMIT before anything
+   is launched.  Every call on one GPU shares that array: calls that overlap on one GPU (different
+   streams or threads) race and must be serialised by the caller.  Same results as
+   nsc_imreconstruct_u8. */
+int nsc_imreconstruct_u8_noalloc(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int w, int h, int conn,
+                                 void* stream);
 /* Batched reconstruction: n independent (marker, mask) pairs of any sizes and connectivity in
    one launch per pass (the reference calls imreconstruct once per tile, nu-segment.cpp:175).
    Results are identical to n single calls.  marker/mask/out are device pointers of the element

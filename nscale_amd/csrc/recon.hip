@@ -1096,6 +1096,66 @@ template int recon_batched_impl<float>(const nsc_recon_desc*, int, hipStream_t);
 
 }  // namespace nsc
 
+// ---- reconstruction without allocation ---------------------------------------------------
+// The pass driver's scratch (tile stamps, the counter ring, two tile lists: 12 B per tile plus the
+// ring) for callers that cannot allocate lives in one file-scope device array, `scratch`, which
+// every GPU holds at its own address for the life of the process.  Host code never passes the
+// array's name to a kernel (in host code it is a stand-in, not a device address): the device
+// address comes from hipGetSymbolAddress, once per device, with that device current.
+namespace nsc {
+namespace {
+
+constexpr int kNoAllocMaxTiles = 1 << 16;  // 62 x 62 tiles: e.g. 15872 x 15872 pixels
+constexpr size_t kNoAllocBytes = 2 * sizeof(unsigned long long) * (kNoAllocMaxTiles / 64) + 4 * sizeof(int) +
+                                sizeof(unsigned) * kNoAllocMaxTiles + sizeof(int) * (kRing + 2 * (size_t)kNoAllocMaxTiles);
+static_assert(kNoAllocMaxTiles % 64 == 0, "whole bitmap words");
+
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// the calling device's copy of `scratch` (looked up once per device)
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    NSC_HIP(hipGetDevice(&dev));
+    NSC_ARG(dev >= 0 && dev < kMaxDevices, "imreconstruct_u8_noalloc: device index");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        NSC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return NSC_OK;
+}
+
+}  // namespace
+}  // namespace nsc
+
+extern "C" int nsc_imreconstruct_u8_noalloc(const uint8_t* marker, const uint8_t* mask, uint8_t* out, int w, int h,
+                                            int conn, void* stream) {
+    using namespace nsc;
+    NSC_ARG(marker && mask && out && w > 0 && h > 0 && valid_conn(conn), "imreconstruct_u8_noalloc");
+    NSC_CHECK(ensure_init());
+    RImg<uint8_t> one{marker, mask, out, w, h, (int)cdiv(w, OW), (int)cdiv(h, OH), conn, 0, 0, 0};
+    const int G = region_side<uint8_t>();
+    set_regions(one, G);
+    const long long nt = (long long)one.rgx * one.rgy;
+    if (nt > kNoAllocMaxTiles) {
+        set_error("imreconstruct_u8_noalloc: %lld tiles of 62 x 62 pixels exceed the fixed scratch of %d tiles", nt,
+                  kNoAllocMaxTiles);
+        return NSC_ERR_LIMIT;
+    }
+    void* base = nullptr;
+    NSC_CHECK(noalloc_scratch(&base));
+    hipStream_t s = as_stream(stream);
+    const ReconBufs B = recon_carve(base, (int)nt);  // fits: nt <= kNoAllocMaxTiles
+    recon_clear(B, s);
+    return recon_run<uint8_t>(nullptr, one, nullptr, (int)nt, (long long)w * h, conn, G, B, s);
+}
+
 extern "C" int nsc_recon_set_schedule(int is_float, int region, int passes, int queue) {
     NSC_ARG(region >= 0 && region <= 4 && passes >= 0 && queue >= -1 && queue <= 1, "recon_set_schedule");
     nsc::ReconTune& t = is_float ? nsc::tune<float>() : nsc::tune<uint8_t>();

@@ -373,6 +373,43 @@ def bench_slide(args):
         sys.exit(1)
 
 
+def bench_normalize(args):
+    """Normalization::normalization (Reinhard colour transfer) of the 4096x4096 synthetic H&E tile
+    to the target statistics of segment/test/src/nu-segmentTest.cpp:185-187, input resident in HBM,
+    output to a second device buffer.  Algorithmic bytes: 3 B/px read + 3 B/px written."""
+    import torch
+    from nscale_amd import gpu
+    from nscale_amd.synth import synth_he
+
+    S = args.size
+    mean_t, std_t = [-0.632356, -0.0516004, 0.0376543], [0.26235, 0.0514831, 0.0114217]
+    bgr = torch.from_numpy(synth_he(S, S, 2026)).cuda()
+    out = torch.empty_like(bgr)
+    for _ in range(args.warmup):
+        gpu.normalization(bgr, mean_t, std_t, out=out)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        gpu.normalization(bgr, mean_t, std_t, out=out)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = sorted(a.elapsed_time(b) for a, b in evs)
+    avg = sum(ms) / len(ms)
+    achieved = 6 * S * S / (avg * 1e-3) / 1e9
+    print(json.dumps({"metric": "normalization Mpix/s", "value": round(args.steps * S * S / wall / 1e6, 3),
+                      "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+                      "config": {"workload": "Normalization::normalization %dx%d H&E tile" % (S, S)},
+                      "p50_ms": round(statistics.median(ms), 4),
+                      "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                                   "avg_ms": round(avg, 4)}}), flush=True)
+
+
 def bench_copy(args):
     """Calibration leg for HBM byte counters: a plain device copy of a known size (64 MiB)."""
     import torch
@@ -434,7 +471,8 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--cpu-size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("chain", "recon", "copy", "serve", "batched", "slide"), default="chain")
+    ap.add_argument("--workload", choices=("chain", "recon", "copy", "serve", "batched", "slide", "normalize"),
+                    default="chain")
     ap.add_argument("--slide-tiles", type=int, default=0, help="slide: process only the first N tiles (0 = all)")
     ap.add_argument("--pairs", type=int, default=256, help="batched: mask/marker pairs")
     ap.add_argument("--batch", type=int, default=16, help="tiles per served step")
@@ -442,7 +480,7 @@ def main():
     ap.add_argument("--slots", type=int, default=8, help="server slots (concurrent chains)")
     ap.add_argument("--budget-gb", type=int, default=200, help="server HBM budget")
     ap.add_argument("--rates", default="10,50,100,200,500,1000,2000", help="serve: arrival rates, req/s")
-    ap.add_argument("--requests", type=int, default=48, help="serve: requests per rate")
+    ap.add_argument("--requests", type=int, default=500, help="serve: requests per rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the segmentation path has no multi-GPU component in this build "
@@ -463,6 +501,8 @@ def main():
         return bench_batched(args)
     if args.workload == "slide":
         return bench_slide(args)
+    if args.workload == "normalize":
+        return bench_normalize(args)
     S = args.size
     bgr = torch.from_numpy(synth_he(S, S, 2026)).cuda()
     ws = gpu.Workspace(S, S)

@@ -329,6 +329,29 @@ public:
     }
 };
 
+// nscale::Normalization (normalization/Normalization.h:30-47): Reinhard colour transfer of the
+// foreground of a BGR image to target l-alpha-beta statistics, and those statistics of an image.
+class Normalization {
+   public:
+    // writes the normalized image to `out` (may be `originalI` itself); asynchronous on s
+    static void normalization(Image<const uint8_t> originalI, const float targetMean[3], const float targetStd[3],
+                              Image<uint8_t> out, hipStream_t s) {
+        const size_t ld = originalI.step ? originalI.step : (size_t)3 * originalI.width;
+        const size_t old = out.step ? out.step : (size_t)3 * out.width;
+        check(nsc_normalization(originalI.data, originalI.width, originalI.height, ld, targetMean, targetStd, out.data,
+                                old, s),
+              "normalization");
+    }
+    // synchronizes s
+    static void targetParameters(Image<const uint8_t> originalI, float (&targetMean)[3], float (&targetStd)[3],
+                                 hipStream_t s) {
+        const size_t ld = originalI.step ? originalI.step : (size_t)3 * originalI.width;
+        check(nsc_normalization_target_parameters(originalI.data, originalI.width, originalI.height, ld, targetMean,
+                                                  targetStd, nullptr, s),
+              "targetParameters");
+    }
+};
+
 }  // namespace hip
 }  // namespace nscale
 

@@ -287,39 +287,68 @@ typedef struct nsc_object_features_out {
 int nsc_object_features(const int32_t* labels, int w, int h, const int32_t* bbox5, int n, const uint8_t* img,
                         int channels, int channel, nsc_object_features_out* out, void* stream);
 
+/* ---- colour normalization -------------------------------------------------------------- */
+/* Normalization::targetParameters(originalI, targetMean, targetStd)
+   (normalization/Normalization.h:44, .cpp:502-582): l-alpha-beta mean and standard deviation of
+   the foreground pixels of a DEVICE BGR image (row stride ld >= 3 w bytes) that the fixed Fisher
+   discriminant selects (segFG, .cpp:19-56) and whose L*a*b product is positive (.cpp:549-553).
+   mean3 / std3 are host float[3]; count (host, may be NULL) receives the number of pixels used.
+   Synchronizes the stream.  No foreground gives NaN statistics, as in the reference. */
+int nsc_normalization_target_parameters(const uint8_t* bgr, int w, int h, size_t ld, float* mean3, float* std3,
+                                        int64_t* count, void* stream);
+/* Normalization::normalization(originalI, targetMean, targetStd) (Normalization.h:43,
+   .cpp:436-499): out (DEVICE BGR, row stride out_ld) = the image with its foreground's LAB
+   statistics moved to the target's (host float[3] each), background pixels converted to LAB and
+   back.  Asynchronous on the stream; out may be the input itself (same stride), any other overlap
+   is rejected.  The transcendental steps are correctly rounded (up to double-precision ties), so a
+   C library's log10f / powf may give results one level apart on a few channel values. */
+int nsc_normalization(const uint8_t* bgr, int w, int h, size_t ld, const float* target_mean3, const float* target_std3,
+                      uint8_t* out, size_t out_ld, void* stream);
+
 /* ---- continuous-batching tile server --------------------------------------------------- */
 /* Replaces the reference's one-tile-per-MPI-process scale-out of segmentNuclei
-   (nu-segment.cpp:274-433, nu-segmentTest.cpp:207): requests of any tile size are admitted when a
-   slot (worker + stream + device workspace for max_w x max_h pixels, either orientation) is free
-   and their device-memory footprint (nsc_request_bytes) fits the HBM budget left after the slot
-   workspaces; admitted requests run concurrently on their own streams.  Outputs go to the
-   caller's device buffers; the caller keeps inputs and outputs alive until the request is
+   (nu-segment.cpp:274-433, nu-segmentTest.cpp:207) and its task split into stages S1 / A4
+   (tasks/seg-tasks/S1Task.cpp, A4Task.cpp): requests of any tile size up to max_w x max_h pixels
+   queue FIFO and are admitted when a slot (worker + stream + device memory) is free and the device
+   memory the slot must add for them fits the HBM budget.  A slot's memory is a chain workspace for
+   the largest tile it has served (grown on demand) plus a staging buffer for host inputs; the
+   budget bounds the sum over slots, and an idle slot gives its memory back when a queued request
+   could not otherwise be admitted.  Admitted requests run concurrently on their own streams;
+   outputs go to the caller's device buffers, which the caller keeps alive until the request is
    waited on or polled complete. */
 typedef struct nsc_server nsc_server;
 typedef struct nsc_server_config {
-    int slots;            /* concurrent chains, 1..64 */
-    size_t hbm_budget;    /* device bytes for slot workspaces plus in-flight requests */
-    int max_w, max_h;     /* largest tile (w*h pixels; either orientation) */
+    int slots;            /* concurrent requests, 1..64 */
+    size_t hbm_budget;    /* device bytes the slots may hold together */
+    int max_w, max_h;     /* largest tile: requests may have any shape of at most max_w*max_h pixels */
 } nsc_server_config;
+/* nsc_request.stage */
+#define NSC_STAGE_CHAIN 0             /* HistologicalEntities::segmentNuclei */
+#define NSC_STAGE_FIND_CANDIDATES 1   /* plFindNucleusCandidates (S1): bgr -> mask_out = seg_norbc */
+#define NSC_STAGE_SEPARATE_NUCLEI 2   /* plSeparateNuclei (A4): bgr, mask_in = seg_open -> mask_out = seg_nonoverlap */
 typedef struct nsc_request {
     const uint8_t* bgr;   /* BGR tile: device pointer, or host pointer when host_input != 0 */
     int w, h;
     int host_input;
-    int32_t* labels;      /* device int32 w*h: the chain's labels (smallest raster index, bg -1) */
-    int canonical;        /* != 0: relabel to 1..N in raster order of first pixel, bg 0 */
-    int32_t* bbox5;       /* optional device n x 5 [label|minx|maxx|miny|maxy] (implies canonical) */
-    int32_t* area;        /* optional device per-label pixel count (with bbox5) */
+    int32_t* labels;      /* chain: device int32 w*h, the chain's labels (smallest raster index, bg -1) */
+    int canonical;        /* chain: != 0 relabels to 1..N in raster order of first pixel, bg 0 */
+    int32_t* bbox5;       /* chain, optional: device n x 5 [label|minx|maxx|miny|maxy] (implies canonical) */
+    int32_t* area;        /* chain, optional: device per-label pixel count (with bbox5) */
     int max_labels;       /* capacity of bbox5 / area in labels */
     const nsc_seg_params* params; /* NULL = the reference's defaults */
+    int stage;            /* NSC_STAGE_*: the whole chain or one of its two task stages */
+    uint8_t* mask_out;    /* stages S1 / A4: device u8 w*h output mask */
+    const uint8_t* mask_in; /* stage A4: device u8 w*h seg_open */
 } nsc_request;
 typedef struct nsc_result {
-    int status;           /* chain status code */
-    int compcount;        /* nuclei found */
+    int status;           /* chain status code (stages: NSC_SEG_CONTINUE when the chain goes on) */
+    int compcount;        /* nuclei found (chain) */
     int error;            /* NSC_OK or NSC_ERR_* */
     int slot;             /* the slot that ran it */
     double queue_ms, run_ms, total_ms;  /* submit -> start -> end, host clock */
     double start_ms, end_ms;            /* start and end, ms since the server was created */
 } nsc_result;
+/* device bytes a slot holds to serve a w x h tile from device memory (host inputs add 3*w*h) */
 size_t nsc_request_bytes(int w, int h);
 int nsc_server_create(const nsc_server_config* cfg, nsc_server** server);
 int nsc_server_submit(nsc_server* server, const nsc_request* req, int64_t* ticket);
@@ -327,8 +356,8 @@ int nsc_server_submit(nsc_server* server, const nsc_request* req, int64_t* ticke
 int nsc_server_wait(nsc_server* server, int64_t ticket, nsc_result* res);
 /* 1 while pending; otherwise like nsc_server_wait */
 int nsc_server_poll(nsc_server* server, int64_t ticket, nsc_result* res);
-int nsc_server_stats(nsc_server* server, int* queued, int* running, size_t* inflight_bytes,
-                     size_t* request_budget);
+/* queued and running requests, device bytes the slots hold, the budget */
+int nsc_server_stats(nsc_server* server, int* queued, int* running, size_t* used_bytes, size_t* budget);
 /* drains every submitted request, then frees the slots */
 int nsc_server_destroy(nsc_server* server);
 

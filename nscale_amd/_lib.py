@@ -33,7 +33,11 @@ class ServerConfig(ctypes.Structure):
 
 class Request(ctypes.Structure):
     _fields_ = [("bgr", _P), ("w", _I), ("h", _I), ("host_input", _I), ("labels", _P), ("canonical", _I),
-                ("bbox5", _P), ("area", _P), ("max_labels", _I), ("params", _P)]
+                ("bbox5", _P), ("area", _P), ("max_labels", _I), ("params", _P), ("stage", _I), ("mask_out", _P),
+                ("mask_in", _P)]
+
+
+STAGE_CHAIN, STAGE_FIND_CANDIDATES, STAGE_SEPARATE_NUCLEI = 0, 1, 2
 
 
 class SegIO(ctypes.Structure):
@@ -91,6 +95,8 @@ def lib():
         L.nsc_pl_separate_nuclei.argtypes = [_P, ctypes.c_size_t, _P, ctypes.c_size_t, _I, _I, _I, _I, _P,
                                              ctypes.c_size_t, _P, _P]
         L.nsc_object_features.argtypes = [_P, _I, _I, _P, _I, _P, _I, _I, ctypes.POINTER(ObjFeaturesOut), _P]
+        L.nsc_normalization_target_parameters.argtypes = [_P, _I, _I, ctypes.c_size_t, _P, _P, _P, _P]
+        L.nsc_normalization.argtypes = [_P, _I, _I, ctypes.c_size_t, _P, _P, _P, ctypes.c_size_t, _P]
         L.nsc_watershed_tier_runs.argtypes = [ctypes.POINTER(ctypes.c_longlong), _I]
         L.nsc_recon_autotune.argtypes = [ctypes.POINTER(ReconDesc), _I, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                          ctypes.POINTER(ctypes.c_float)]

@@ -5,6 +5,8 @@ torch is used for device memory and streams only; every operator runs the librar
 """
 import ctypes
 
+import numpy as np
+
 import torch
 
 from nscale_amd._lib import (BBOX_ALL, BBOX_NONE, BBOX_REFERENCE_CPU, ObjFeaturesOut, ReconDesc, Request, Result,
@@ -416,6 +418,31 @@ def object_features(labels, bbox, img=None, channel=0, intensity=True, gradient=
     return {k: v[:n] for k, v in out.items()}
 
 
+def normalization_target_parameters(bgr):
+    """nscale::Normalization::targetParameters on a device BGR (h, w, 3) uint8 image (rows may be
+    pitched): (mean[3], std[3], pixel count) of the foreground's l-alpha-beta values."""
+    h, w = int(bgr.shape[0]), int(bgr.shape[1])
+    bp, ld = _rows(bgr)
+    m = (ctypes.c_float * 3)()
+    s = (ctypes.c_float * 3)()
+    n = ctypes.c_int64()
+    call("nsc_normalization_target_parameters", bp, w, h, ld, m, s, ctypes.byref(n), _stream())
+    return np.array(m, np.float32), np.array(s, np.float32), int(n.value)
+
+
+def normalization(bgr, target_mean, target_std, out=None):
+    """nscale::Normalization::normalization on the device: the BGR image with its foreground moved to
+    the target l-alpha-beta statistics.  out may be bgr itself (in place) or None (a new tensor)."""
+    h, w = int(bgr.shape[0]), int(bgr.shape[1])
+    out = out if out is not None else torch.empty(h, w, 3, dtype=torch.uint8, device=bgr.device)
+    bp, ld = _rows(bgr)
+    op, old = _rows(out)
+    m = (ctypes.c_float * 3)(*[float(v) for v in target_mean])
+    s = (ctypes.c_float * 3)(*[float(v) for v in target_std])
+    call("nsc_normalization", bp, w, h, ld, m, s, op, old, _stream())
+    return out
+
+
 def watershed_tier_runs():
     """Phase-1 runs per LDS tier (<=512, 1024, 2048, 4096, 6400 px, global bucket), process-wide sums."""
     arr = (ctypes.c_longlong * 6)()
@@ -443,33 +470,47 @@ class Server:
         call("nsc_server_create", ctypes.byref(cfg), ctypes.byref(self.handle))
         self._live = {}
 
-    def submit(self, bgr, canonical=False, max_labels=0, params=None):
+    def submit(self, bgr, canonical=False, max_labels=0, params=None, stage=0, seg_open=None):
+        """stage 0: the whole chain (labels out); 1: plFindNucleusCandidates (S1, seg_norbc out);
+        2: plSeparateNuclei (A4, seg_open in, seg_nonoverlap out)."""
         h, w = int(bgr.shape[0]), int(bgr.shape[1])
         assert bgr.is_contiguous() and bgr.dtype == torch.uint8 and bgr.dim() == 3 and bgr.shape[2] == 3
         dev = bgr.device if bgr.is_cuda else torch.device("cuda", torch.cuda.current_device())
         # outputs always live on the device, also for host inputs (the worker copies those in)
-        labels = torch.empty(h, w, dtype=torch.int32, device=dev)
-        bbox = area = None
-        if max_labels:
-            bbox = torch.empty(5 * max_labels, dtype=torch.int32, device=dev)
-            area = torch.empty(max_labels, dtype=torch.int32, device=dev)
-        req = Request(_P(bgr.data_ptr()), w, h, 0 if bgr.is_cuda else 1, _P(labels.data_ptr()), 1 if canonical else 0,
+        labels = bbox = area = mask = None
+        if stage == 0:
+            labels = torch.empty(h, w, dtype=torch.int32, device=dev)
+            if max_labels:
+                bbox = torch.empty(5 * max_labels, dtype=torch.int32, device=dev)
+                area = torch.empty(max_labels, dtype=torch.int32, device=dev)
+        else:
+            mask = torch.empty(h, w, dtype=torch.uint8, device=dev)
+        if stage == 2:
+            assert seg_open is not None and seg_open.is_cuda and seg_open.is_contiguous()
+        req = Request(_P(bgr.data_ptr()), w, h, 0 if bgr.is_cuda else 1,
+                      _P(labels.data_ptr()) if labels is not None else None, 1 if canonical else 0,
                       _P(bbox.data_ptr()) if bbox is not None else None,
                       _P(area.data_ptr()) if area is not None else None, max_labels,
-                      ctypes.cast(ctypes.byref(params), _P) if params is not None else None)
+                      ctypes.cast(ctypes.byref(params), _P) if params is not None else None, int(stage),
+                      _P(mask.data_ptr()) if mask is not None else None,
+                      _P(seg_open.data_ptr()) if seg_open is not None else None)
         # inputs must be ready on the device before another stream reads them
         if bgr.is_cuda:
             torch.cuda.current_stream().synchronize()
         t = ctypes.c_int64(0)
         call("nsc_server_submit", self.handle, ctypes.byref(req), ctypes.byref(t))
-        self._live[t.value] = (bgr, labels, bbox, area)
+        self._live[t.value] = (bgr, labels, bbox, area, mask, seg_open)
         return t.value
 
     def wait(self, ticket):
+        """(status, compcount, labels, bbox5, area, result) for the chain; (status, 0, mask, None,
+        None, result) for a stage request."""
         res = Result()
         rc = lib().nsc_server_wait(self.handle, ticket, ctypes.byref(res))
-        bgr, labels, bbox, area = self._live.pop(ticket)
+        bgr, labels, bbox, area, mask, _ = self._live.pop(ticket)
         check(rc, "nsc_server_wait")
+        if mask is not None:
+            return res.status, 0, mask, None, None, res
         n = res.compcount
         if bbox is not None:
             bbox = bbox[:5 * n].view(5, n) if n else bbox[:0].view(5, 0)
@@ -477,6 +518,7 @@ class Server:
         return res.status, n, labels, bbox, area, res
 
     def stats(self):
+        """(queued, running, device bytes the slots hold, budget)"""
         q, r = ctypes.c_int(0), ctypes.c_int(0)
         b, bud = ctypes.c_size_t(0), ctypes.c_size_t(0)
         call("nsc_server_stats", self.handle, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b), ctypes.byref(bud))

@@ -1128,4 +1128,173 @@ void orc_feat_gradient(const int32_t* bbox5, int n, const int32_t* labels, const
     }
 }
 
+
+}  // extern "C"
+
+// ---- colour normalization (src/normalization/Normalization.cpp) ----------------------------
+// Paths in this section are relative to the reference's src/normalization/.  The OpenCV 2.4
+// matrix expressions are restated with the arithmetic OpenCV 2.4.9 runs for them:
+// addWeighted on float images works in double (arithm.cpp addWeighted32f), convertTo
+// float -> float scales in float, Mat -= / += scalar and Mat.mul are float operations.
+
+namespace {
+
+// segFG (Normalization.cpp:19-56) for one pixel: the expression
+// M00*rd + M01*gd + M02*bd + M03 evaluates as addWeighted(addWeighted(rd, M00, gd, M01, 0),
+// 1, bd, M02, M03), each addWeighted rounding its double result to float.
+bool norm_fg(uint8_t b, uint8_t g, uint8_t r) {
+    static const float M[8] = {-0.154f, 0.035f, 0.549f, -45.718f, -0.057f, -0.817f, 1.170f, -49.887f};
+    float d[2];
+    for (int k = 0; k < 2; ++k) {
+        const float* m = M + 4 * k;
+        const float t = (float)((double)r * (double)m[0] + (double)g * (double)m[1] + 0.0);
+        d[k] = (float)((double)t * 1.0 + (double)b * (double)m[2] + (double)m[3]);
+    }
+    return d[0] < d[1];
+}
+
+struct NormConsts {
+    float k3, k6, k2;  // 1/sqrtf(3), 1/sqrtf(6), 1/sqrtf(2) (Normalization.cpp:112-114, 172-174)
+    float s255;        // the float scale convertTo applies for "/= 255" (Normalization.cpp:331)
+    NormConsts() {
+        k3 = 1 / sqrtf(3.0f);
+        k6 = 1 / sqrtf(6.0f);
+        k2 = 1 / sqrtf(2.0f);
+        s255 = (float)(1. / 255);
+    }
+};
+const NormConsts& nc() {
+    static const NormConsts c;
+    return c;
+}
+
+// bgr2Lab (Normalization.cpp:64-146) of one pixel after PixelClass's float conversion and
+// scaling (Normalization.cpp:328-333)
+void norm_lab(uint8_t b8, uint8_t g8, uint8_t r8, float lab[3]) {
+    const NormConsts& c = nc();
+    const float b = (float)b8 * c.s255, g = (float)g8 * c.s255, r = (float)r8 * c.s255;
+    const float L = (float)(0.3811 * r + 0.5783 * g + 0.0402 * b);
+    const float M = (float)(0.1967 * r + 0.7244 * g + 0.0782 * b);
+    const float S = (float)(0.0241 * r + 0.1288 * g + 0.8444 * b);
+    const float l = log10f(L), m = log10f(M), s = log10f(S);
+    // Matrix2 = diag(k3, k6, k2) * [1 1 1; 1 1 -2; 1 -1 0] (exact products)
+    lab[0] = l * c.k3 + m * c.k3 + s * c.k3;
+    lab[1] = l * c.k6 + m * c.k6 + s * (-2 * c.k6);
+    lab[2] = l * c.k2 + m * (-c.k2) + s * 0.0f;
+}
+
+// lab2BGR (Normalization.cpp:162-306) of one pixel
+void norm_bgr(const float lab[3], uint8_t out[3]) {
+    const NormConsts& c = nc();
+    // Matrix1 = [1 1 1; 1 1 -1; 1 -2 0] * diag(k3, k6, k2) (exact products)
+    const float ll = lab[0] * c.k3 + lab[1] * c.k6 + lab[2] * c.k2;
+    const float lm = lab[0] * c.k3 + lab[1] * c.k6 + lab[2] * (-c.k2);
+    const float ls = lab[0] * c.k3 + lab[1] * (-2 * c.k6) + lab[2] * (0 * c.k2);
+    float lms[3] = {powf(10.0f, ll), powf(10.0f, lm), powf(10.0f, ls)};
+    for (float& v : lms)
+        if (!(v <= DBL_MAX && v >= -DBL_MAX)) v = 0.0f;  // NaN and +-inf
+    const float L = lms[0], M = lms[1], S = lms[2];
+    const float bgrf[3] = {L * 0.0585f + M * -0.2611f + S * 1.2057f,
+                           L * -1.2197f + M * 2.3831f + S * -0.1626f,
+                           L * 4.4687f + M * -3.5887f + S * 0.1196f};
+    for (int k = 0; k < 3; ++k) {
+        float v = (float)((double)bgrf[k] * 255.0);
+        if (v < 0.0) v = 0.0f;
+        if (v > 255.0) v = 255.0f;
+        // rndint (Normalization.cpp:149-158): floor, plus one from a fraction of 0.5 up.  A NaN
+        // converts to INT_MIN on x86-64, whose low byte is 0.
+        if (v != v) {
+            out[k] = 0;
+            continue;
+        }
+        const float f = floorf(v);
+        int q = (int)f;
+        if (v - f >= 0.5f) q += 1;
+        out[k] = (uint8_t)q;
+    }
+}
+
+// The foreground statistics of TransferI / targetParameters (Normalization.cpp:365-394,
+// 540-569): pixels that are foreground and whose float product L*a*b is positive; per channel a
+// double sum of the values and of their float squares, in raster order.
+void norm_stats(const uint8_t* bgr, int w, int h, double mean[3], double stdv[3], long long* count) {
+    double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+    long long n = 0;
+    for (size_t i = 0; i < (size_t)w * h; ++i) {
+        const uint8_t* p = bgr + 3 * i;
+        if (!norm_fg(p[0], p[1], p[2])) continue;
+        float lab[3];
+        norm_lab(p[0], p[1], p[2], lab);
+        if (!((lab[0] * lab[1]) * lab[2] > 0)) continue;
+        for (int k = 0; k < 3; ++k) {
+            sum[k] += lab[k];
+            sq[k] += lab[k] * lab[k];
+        }
+        ++n;
+    }
+    for (int k = 0; k < 3; ++k) {
+        // int count in the reference: images below 2^31 pixels
+        mean[k] = sum[k] / (int)n;
+        const double var = sq[k] / (int)n - mean[k] * mean[k];
+        stdv[k] = sqrt(var);
+    }
+    if (count) *count = n;
+}
+
+}  // namespace
+
+extern "C" {
+
+// segFG (Normalization.cpp:19-56): 255 where the foreground discriminant wins, else 0
+void orc_norm_seg_fg(const uint8_t* bgr, uint8_t* out, int w, int h) {
+    for (size_t i = 0; i < (size_t)w * h; ++i)
+        out[i] = norm_fg(bgr[3 * i], bgr[3 * i + 1], bgr[3 * i + 2]) ? 255 : 0;
+}
+
+// bgr2Lab of every pixel (Normalization.cpp:64-146, 328-333): lab[3 i + k]
+void orc_norm_bgr2lab(const uint8_t* bgr, float* lab, int w, int h) {
+    for (size_t i = 0; i < (size_t)w * h; ++i) norm_lab(bgr[3 * i], bgr[3 * i + 1], bgr[3 * i + 2], lab + 3 * i);
+}
+
+// lab2BGR of every pixel (Normalization.cpp:162-306)
+void orc_norm_lab2bgr(const float* lab, uint8_t* bgr, int w, int h) {
+    for (size_t i = 0; i < (size_t)w * h; ++i) norm_bgr(lab + 3 * i, bgr + 3 * i);
+}
+
+// Normalization::targetParameters (Normalization.cpp:502-582); returns the pixel count used
+long long orc_norm_target_parameters(const uint8_t* bgr, int w, int h, float* mean3, float* std3) {
+    double m[3], s[3];
+    long long n;
+    norm_stats(bgr, w, h, m, s, &n);
+    for (int k = 0; k < 3; ++k) {
+        mean3[k] = (float)m[k];
+        std3[k] = (float)s[k];
+    }
+    return n;
+}
+
+// Normalization::normalization (Normalization.cpp:436-499): foreground LAB values move to the
+// target statistics (TransferI, Normalization.cpp:396-404), background ones are kept, and every
+// pixel converts back (lab2BGR).  Background-labelled pixels of the transferred image are exactly
+// those copied from o_bg_lab (Normalization.cpp:487), so each pixel depends on itself and on the
+// six statistics only.
+void orc_norm_normalization(const uint8_t* bgr, int w, int h, const float* target_mean, const float* target_std,
+                            uint8_t* out) {
+    double m[3], s[3];
+    norm_stats(bgr, w, h, m, s, nullptr);
+    float meanI[3], ratio[3];
+    for (int k = 0; k < 3; ++k) {
+        meanI[k] = (float)m[k];
+        ratio[k] = target_std[k] / (float)s[k];
+    }
+    for (size_t i = 0; i < (size_t)w * h; ++i) {
+        const uint8_t* p = bgr + 3 * i;
+        float lab[3];
+        norm_lab(p[0], p[1], p[2], lab);
+        if (norm_fg(p[0], p[1], p[2]))
+            for (int k = 0; k < 3; ++k) lab[k] = (lab[k] - meanI[k]) * ratio[k] + target_mean[k];
+        norm_bgr(lab, out + 3 * i);
+    }
+}
+
 }  // extern "C"

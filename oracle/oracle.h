@@ -95,6 +95,15 @@ int orc_segment_nuclei(const uint8_t* bgr, int w, int h, const orc_seg_params* p
 int orc_segment_nuclei_bbox(const uint8_t* bgr, int w, int h, const orc_seg_params* p, int32_t* labels, int* compcount,
                             int32_t* bbox5, int cap, void** stage, double* stage_ms);
 
+
+// colour normalization (src/normalization/Normalization.cpp)
+void orc_norm_seg_fg(const uint8_t* bgr, uint8_t* out, int w, int h);
+void orc_norm_bgr2lab(const uint8_t* bgr, float* lab, int w, int h);
+void orc_norm_lab2bgr(const float* lab, uint8_t* bgr, int w, int h);
+long long orc_norm_target_parameters(const uint8_t* bgr, int w, int h, float* mean3, float* std3);
+void orc_norm_normalization(const uint8_t* bgr, int w, int h, const float* target_mean, const float* target_std,
+                            uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,7 @@ def oracle():
         _oracle.orc_cc_bounding_box_literal.restype = ctypes.c_int
         _oracle.orc_segment_nuclei.restype = ctypes.c_int
         _oracle.orc_segment_nuclei_bbox.restype = ctypes.c_int
+        _oracle.orc_norm_target_parameters.restype = ctypes.c_longlong
         _oracle.orc_feat_area.argtypes = [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P]
         _oracle.orc_feat_ellipse.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         _oracle.orc_feat_perimeter.argtypes = [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P]
@@ -156,6 +157,48 @@ def o_watershed2(bgr, img, conn):
     out = np.empty((h, w), np.int32)
     oracle().orc_watershed2(ptr(np.ascontiguousarray(bgr)), ptr(np.ascontiguousarray(img)), ptr(out), w, h, conn)
     return out
+
+
+def o_norm_fg(bgr):
+    h, w = bgr.shape[:2]
+    out = np.zeros((h, w), np.uint8)
+    oracle().orc_norm_seg_fg(ptr(np.ascontiguousarray(bgr)), ptr(out), w, h)
+    return out
+
+
+def o_norm_lab(bgr):
+    h, w = bgr.shape[:2]
+    lab = np.zeros((h, w, 3), np.float32)
+    oracle().orc_norm_bgr2lab(ptr(np.ascontiguousarray(bgr)), ptr(lab), w, h)
+    return lab
+
+
+def o_norm_bgr(lab):
+    h, w = lab.shape[:2]
+    out = np.zeros((h, w, 3), np.uint8)
+    oracle().orc_norm_lab2bgr(ptr(np.ascontiguousarray(lab, np.float32)), ptr(out), w, h)
+    return out
+
+
+def o_norm_target(bgr):
+    h, w = bgr.shape[:2]
+    m, s = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    n = oracle().orc_norm_target_parameters(ptr(np.ascontiguousarray(bgr)), w, h, ptr(m), ptr(s))
+    return m, s, n
+
+
+def o_normalize(bgr, mean, std):
+    h, w = bgr.shape[:2]
+    out = np.zeros((h, w, 3), np.uint8)
+    oracle().orc_norm_normalization(ptr(np.ascontiguousarray(bgr)), w, h, ptr(np.asarray(mean, np.float32)),
+                                    ptr(np.asarray(std, np.float32)), ptr(out))
+    return out
+
+
+def all_colours():
+    """4096 x 4096 BGR image holding every 24-bit colour once (b = low byte of the pixel index)."""
+    i = np.arange(1 << 24, dtype=np.uint32)
+    return np.stack([i & 255, (i >> 8) & 255, i >> 16], -1).astype(np.uint8).reshape(4096, 4096, 3)
 
 
 def default_params():

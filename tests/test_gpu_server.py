@@ -53,26 +53,45 @@ def test_small_tiles_match_the_oracle_through_the_server():
     srv.close()
 
 
-def test_budget_admits_one_request_at_a_time():
+def test_budget_bounds_the_slots_memory():
+    """A budget with room for one 1024^2 workspace: requests run one at a time in whichever slot
+    holds that memory, the slots never hold more than the budget, results stay exact."""
     from nscale_amd import gpu
     tiles = [torch.from_numpy(t).cuda() for t in _tiles()[:6]]
-    # workspaces of 3 slots + room for exactly one largest request
+    alone = [_plain(t) for t in tiles]
     one = gpu.request_bytes(1024, 1024)
-    probe = gpu.Server(slots=3, hbm_budget=200 << 30, max_w=1024, max_h=1024)
-    budget_left = probe.stats()[3]
-    probe.close()
-    fixed = (200 << 30) - budget_left
-    srv = gpu.Server(slots=3, hbm_budget=fixed + one + one // 4, max_w=1024, max_h=1024)
+    srv = gpu.Server(slots=3, hbm_budget=one + one // 4, max_w=1024, max_h=1024)
     tks = [srv.submit(t) for t in tiles]
     spans = []
-    for tk in tks:
+    for k, tk in enumerate(tks):
         st, n, lab, _, _, res = srv.wait(tk)
+        assert res.error == 0 and st == alone[k][0] and np.array_equal(lab.cpu().numpy(), alone[k][1])
         spans.append((res.start_ms, res.end_ms, gpu.request_bytes(int(lab.shape[1]), int(lab.shape[0]))))
+        assert srv.stats()[2] <= one + one // 4
     srv.close()
-    # at any instant the running requests' footprints fit in what the budget leaves
+    # at any instant the running requests' workspaces fit in the budget
     for a in spans:
         live = sum(b[2] for b in spans if b[0] <= a[0] < b[1])
         assert live <= one + one // 4
+
+
+def test_stage_requests_pipeline_s1_and_a4():
+    """S1 (plFindNucleusCandidates) and A4 (plSeparateNuclei) requests of different tiles in one
+    server: each equals the oracle's stage 11 / stage 21 of the same tile."""
+    from nscale_amd import gpu
+    from tests._lib import o_segment
+    tiles = _tiles()[:4]
+    exp = [o_segment(t, stages=(11, 13, 21))[2] for t in tiles]
+    dev = [torch.from_numpy(t).cuda() for t in tiles]
+    opened = [torch.from_numpy(e[13]).cuda() for e in exp]
+    srv = gpu.Server(slots=4, hbm_budget=32 << 30, max_w=1024, max_h=1024)
+    tks = [(k, 1, srv.submit(d, stage=1)) for k, d in enumerate(dev)]
+    tks += [(k, 2, srv.submit(d, stage=2, seg_open=o)) for k, (d, o) in enumerate(zip(dev, opened))]
+    for k, stage, tk in tks:
+        st, _, mask, _, _, res = srv.wait(tk)
+        assert res.error == 0 and st == 4
+        assert np.array_equal(mask.cpu().numpy(), exp[k][11 if stage == 1 else 21]), (k, stage)
+    srv.close()
 
 
 def test_canonical_labels_boxes_and_areas():

@@ -484,9 +484,10 @@ def test_chain_synthetic_structure():
 
 def test_chain_literal_compcount_matches_reference_code():
     """segmentNuclei's compcount/bbox (HistologicalEntities.cpp:589-591) on the oracle chain's labels,
-    against the reference's own ConnComponents::boundingBox on the same labels.  With bwlabel2's
-    labels (smallest raster index, bg -1) the scan meets label 0 or stops at the first -1 before any
-    other label, so the reference CPU chain reports compcount 0."""
+    against the reference's own ConnComponents::boundingBox on the same labels.  Every pixel before
+    the first -1 is foreground and connected to pixel 0, so with bwlabel2's labels (smallest raster
+    index, bg -1) it carries label 0, which the scan skips: the reference CPU chain reports
+    compcount 0."""
     from nscale_amd.synth import synth_he
     from tests._lib import o_segment_bbox
     ref = ref_cc()

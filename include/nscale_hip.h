@@ -300,8 +300,9 @@ int nsc_normalization_target_parameters(const uint8_t* bgr, int w, int h, size_t
    .cpp:436-499): out (DEVICE BGR, row stride out_ld) = the image with its foreground's LAB
    statistics moved to the target's (host float[3] each), background pixels converted to LAB and
    back.  Asynchronous on the stream; out may be the input itself (same stride), any other overlap
-   is rejected.  The transcendental steps are correctly rounded (up to double-precision ties), so a
-   C library's log10f / powf may give results one level apart on a few channel values. */
+   is rejected.  log10f follows the fdlibm formula around a correctly rounded logf and powf(10, x)
+   is correctly rounded, so results match a C library except where its own logf / powf is not
+   correctly rounded (one level apart on a few channel values in a million). */
 int nsc_normalization(const uint8_t* bgr, int w, int h, size_t ld, const float* target_mean3, const float* target_std3,
                       uint8_t* out, size_t out_ld, void* stream);
 

@@ -12,10 +12,11 @@
 //   3. k_norm_apply: LAB again, the transfer for foreground pixels, back to BGR.
 // Arithmetic follows the reference operation by operation: the discriminant is the double
 // arithmetic of OpenCV 2.4's float addWeighted rounded to float twice, the RGB->LMS products are
-// double (the constants are double literals) rounded to float, everything else is float.  The
-// two transcendental steps (log10f, powf(10, x)) are evaluated in double and rounded once, so
-// they are correctly rounded except where the double result lies within about 2^-50 of a float
-// rounding boundary; a C library's float versions may differ from them by an ulp.
+// double (the constants are double literals) rounded to float, everything else is float.  log10f
+// follows the fdlibm formula C libraries use (float operations around a correctly rounded logf);
+// powf(10, x) is correctly rounded.  Both evaluate their core in double with relative error near
+// 2^-48, so they differ from a C library only where that library's own logf / powf is not
+// correctly rounded (under 1% / 0.1% of arguments for glibc 2.35).
 #include <cfloat>
 #include <cmath>
 
@@ -43,8 +44,13 @@ struct NormParams {
 };
 
 // segFG (Normalization.cpp:19-56): addWeighted(addWeighted(R, m0, G, m1, 0), 1, B, m2, m3) for
-// each discriminant; each addWeighted evaluates in double and stores float.
+// each discriminant; each addWeighted evaluates in double and stores float.  With the decimal
+// coefficients, d_fg - d_bg = (97 R - 852 G + 621 B - 4169) / 1000 exactly; the float coefficients
+// and the float roundings of values below 300 move it by less than 1e-4, so wherever that integer
+// is not zero its sign decides, and only exact ties (75 of the 2^24 colours) take the float path.
 __device__ __forceinline__ bool norm_fg(const NormDisc& D, uint32_t b, uint32_t g, uint32_t r) {
+    const int num = 97 * (int)r - 852 * (int)g + 621 * (int)b - 4169;
+    if (num != 0) return num > 0;
     float d[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -55,8 +61,90 @@ __device__ __forceinline__ bool norm_fg(const NormDisc& D, uint32_t b, uint32_t 
     return d[0] < d[1];
 }
 
-__device__ __forceinline__ float log10_rn(float x) { return (float)log10((double)x); }
-__device__ __forceinline__ float exp10_rn(float x) { return (float)exp10((double)x); }
+// Natural logarithm of m in [0.5, 2) in double: m = 2^e u with u in [sqrt(1/2), sqrt(2)),
+// log u = 2 atanh(t), t = (u - 1) / (u + 1), |t| <= 0.1716, series to t^17 (the first omitted
+// term is below 2^-50 of the result).  Relative error about 2^-50, so the float rounding of the
+// result is the correct rounding except within that distance of a rounding boundary.  The
+// reciprocal is the hardware estimate refined by two Newton steps; polynomials use fused
+// multiply-adds (this is not an operation of the reference, only its accuracy matters).
+__device__ __forceinline__ double log_unit(double m) {
+    double e = 0.0;
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m *= 0.5;
+        e = 1.0;
+    } else if (m < 0x1.6a09e667f3bcdp-1) {
+        m *= 2.0;
+        e = -1.0;
+    }
+    const double d = m + 1.0;  // m - 1 and m + 1 exact (m has 24 significant bits)
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    const double t = (m - 1.0) * r;
+    const double t2 = t * t;
+    double p = 1.0 / 17;
+    p = fma(p, t2, 1.0 / 15);
+    p = fma(p, t2, 1.0 / 13);
+    p = fma(p, t2, 1.0 / 11);
+    p = fma(p, t2, 1.0 / 9);
+    p = fma(p, t2, 1.0 / 7);
+    p = fma(p, t2, 1.0 / 5);
+    p = fma(p, t2, 1.0 / 3);
+    p = fma(p, t2, 1.0);
+    return fma(e, 0x1.62e42fefa39efp-1, 2.0 * t * p);
+}
+
+// log10f as the fdlibm-derived C libraries compute it (glibc's e_log10f.c, the function the
+// reference calls at Normalization.cpp:92-94): x = 2^y m with m in [1, 2) (in [0.5, 1) when the
+// exponent is negative), log10 x = y lo + ivln10 logf(m) + y hi in float operations with fdlibm's
+// constants; logf(m) is the correctly rounded natural logarithm.
+__device__ __forceinline__ float log10f_lib(float x) {
+    constexpr float kIvln10 = 0x1.bcb7b2p-2f;  // 4.3429449201e-01
+    constexpr float kHi = 0x1.344100p-2f;      // 3.0102920532e-01
+    constexpr float kLo = 0x1.a84fb6p-21f;     // 7.9034151668e-07
+    if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : __builtin_nanf("");
+    if (x == INFINITY) return x;
+    uint32_t hx = __float_as_uint(x);
+    int k = 0;
+    if (hx < 0x00800000u) {  // subnormal: scale by 2^25
+        x *= 33554432.0f;
+        hx = __float_as_uint(x);
+        k = -25;
+    }
+    k += (int)(hx >> 23) - 127;
+    const int i = k < 0 ? 1 : 0;
+    const float m = __uint_as_float((hx & 0x007fffffu) | ((uint32_t)(0x7f - i) << 23));
+    const float y = (float)(k + i);
+    const float lnm = (float)log_unit((double)m);
+    const float z = y * kLo + kIvln10 * lnm;
+    return z + y * kHi;
+}
+
+// 10^y rounded to float, evaluated in double: y log2(10) = k + f with the constant split so that
+// y * hi is exact, 2^f = e^(f ln2) by its Taylor series to degree 11 (|f ln2| <= 0.347, truncation
+// below 2^-48, fused multiply-adds); relative error about 2^-48 before the final rounding.
+__device__ __forceinline__ float exp10f_rn(float y) {
+    if (y != y) return y;
+    if (y > 40.0f) return INFINITY;
+    if (y < -46.0f) return 0.0f;
+    const double z = (double)y * 0x1.a934f09p+1;  // log2(10) to 29 bits: exact product
+    const double k = rint(z);
+    const double f = (z - k) + (double)y * 0x1.e68dc57f24960p-29;
+    const double g = f * 0x1.62e42fefa39efp-1;
+    double p = 1.0 / 39916800;
+    p = fma(p, g, 1.0 / 3628800);
+    p = fma(p, g, 1.0 / 362880);
+    p = fma(p, g, 1.0 / 40320);
+    p = fma(p, g, 1.0 / 5040);
+    p = fma(p, g, 1.0 / 720);
+    p = fma(p, g, 1.0 / 120);
+    p = fma(p, g, 1.0 / 24);
+    p = fma(p, g, 1.0 / 6);
+    p = fma(p, g, 0.5);
+    p = fma(p, g, 1.0);
+    p = fma(p, g, 1.0);
+    return (float)ldexp(p, (int)k);
+}
 
 // bgr2Lab of one pixel (Normalization.cpp:64-146) after PixelClass's scaling (:328-333)
 __device__ __forceinline__ void norm_lab(const NormConsts& C, uint32_t b8, uint32_t g8, uint32_t r8, float lab[3]) {
@@ -64,7 +152,7 @@ __device__ __forceinline__ void norm_lab(const NormConsts& C, uint32_t b8, uint3
     const float L = (float)(0.3811 * r + 0.5783 * g + 0.0402 * b);
     const float M = (float)(0.1967 * r + 0.7244 * g + 0.0782 * b);
     const float S = (float)(0.0241 * r + 0.1288 * g + 0.8444 * b);
-    const float l = log10_rn(L), m = log10_rn(M), s = log10_rn(S);
+    const float l = log10f_lib(L), m = log10f_lib(M), s = log10f_lib(S);
     lab[0] = l * C.k3 + m * C.k3 + s * C.k3;
     lab[1] = l * C.k6 + m * C.k6 + s * (-2.0f * C.k6);
     lab[2] = l * C.k2 + m * (-C.k2) + s * 0.0f;  // s * 0 keeps the NaN of a black pixel
@@ -76,7 +164,7 @@ __device__ __forceinline__ void norm_bgr(const NormConsts& C, const float lab[3]
     const float ll = lab[0] * C.k3 + lab[1] * C.k6 + lab[2] * C.k2;
     const float lm = lab[0] * C.k3 + lab[1] * C.k6 + lab[2] * (-C.k2);
     const float ls = lab[0] * C.k3 + lab[1] * (-2.0f * C.k6) + lab[2] * (0.0f * C.k2);
-    float L = exp10_rn(ll), M = exp10_rn(lm), S = exp10_rn(ls);
+    float L = exp10f_rn(ll), M = exp10f_rn(lm), S = exp10f_rn(ls);
     if (!(fabsf(L) <= FLT_MAX)) L = 0.0f;
     if (!(fabsf(M) <= FLT_MAX)) M = 0.0f;
     if (!(fabsf(S) <= FLT_MAX)) S = 0.0f;

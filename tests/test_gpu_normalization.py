@@ -13,8 +13,9 @@ MEAN_T = [-0.632356, -0.0516004, 0.0376543]  # segment/test/src/nu-segmentTest.c
 STD_T = [0.26235, 0.0514831, 0.0114217]
 
 
-def _close(got, exp, frac=2e-4):
+def _close(got, exp, frac=1e-6):
     d = np.abs(got.astype(np.int32) - exp.astype(np.int32))
+    print("channel values one level apart: %d of %d" % (int((d > 0).sum()), d.size), flush=True)
     assert d.max() <= 1, d.max()
     assert (d > 0).mean() <= frac, (d > 0).mean()
 

@@ -287,6 +287,48 @@ typedef struct nsc_object_features_out {
 int nsc_object_features(const int32_t* labels, int w, int h, const int32_t* bbox5, int n, const uint8_t* img,
                         int channels, int channel, nsc_object_features_out* out, void* stream);
 
+/* ---- tile files ------------------------------------------------------------------------ */
+/* cv::imread(path) as the reference calls it (HistologicalEntities.cpp:456, IMREAD_COLOR) for PNG
+   files: non-interlaced grey (1-16 bits), grey+alpha, RGB, RGBA (8/16 bits) or palette (1-8 bits)
+   images become 8-bit BGR (low-depth grey expands to 8 bits, 16-bit samples keep their high byte,
+   alpha is dropped).  *bgr receives a HOST buffer of
+   3*w*h bytes allocated with malloc (release with nsc_free_host).  Other formats return an error. */
+int nsc_png_read(const char* path, uint8_t** bgr, int* w, int* h);
+/* cv::imwrite(path, img) for an 8-bit grey (channels 1) or BGR (channels 3) HOST image, row stride
+   ld bytes (0 = dense): a PNG that decodes to the same pixels as OpenCV's. */
+int nsc_png_write(const char* path, const uint8_t* img, int w, int h, int channels, size_t ld);
+void nsc_free_host(void* p);
+/* The 8-bit image cv::imwrite stores for an int32 label image (HistologicalEntities.cpp:466):
+   saturate_cast<uchar> of every label (background -1 -> 0, labels above 255 -> 255).  DEVICE
+   planes with row strides in bytes (0 = dense).  Asynchronous. */
+int nsc_labels_to_u8(const int32_t* labels, size_t labels_ld, int w, int h, uint8_t* out, size_t out_ld,
+                     void* stream);
+/* HistologicalEntities::segmentNuclei(const std::string& in, const std::string& out, int& compcount,
+   int*& bbox, ...) (HistologicalEntities.h:46-50, .cpp:453-470): reads the PNG tile `in`, runs the
+   chain on the device with params (NULL = defaults) and, on NSC_SEG_SUCCESS, writes the label image
+   to the PNG file `out` as imwrite stores it (nsc_labels_to_u8).  compcount and *bbox (HOST,
+   malloc'd 5 x compcount block, NULL when compcount is 0; may be passed as NULL) are the
+   reference's ConnComponents::boundingBox outputs.  Returns the chain status (NSC_SEG_INVALID_IMAGE
+   when the file cannot be read) or a negative error.  Synchronous. */
+int nsc_segment_nuclei_file(const char* in, const char* out, const nsc_seg_params* params, int* compcount,
+                            int32_t** bbox, void* stream);
+
+/* ---- whole-slide stitching -------------------------------------------------------------- */
+/* Not in the reference, which segments the tiles of a slide independently (nu-segment.cpp:274-433):
+   the step that removes the duplicates a tiling with overlap creates.  Tile (x0, y0, w, h) of a
+   slide_w x slide_h slide cut on a regular grid of stride tile - overlap owns a core (the tile minus
+   overlap/2 on every side with a neighbour; the cores partition the slide).  Each nucleus of the
+   tile -- box i of the DEVICE bbox5 block (n x 5 [label | minx | maxx | miny | maxy], tile
+   coordinates, canonical labels 1..n) -- is kept iff its box centre lies in the core; kept nuclei
+   get ids first_id, first_id + 1, ... in label order, written to ids[i] (DEVICE, n ints; 0 =
+   dropped).  owned (DEVICE w x h int32, row stride owned_ld bytes, may be NULL) receives the tile's
+   labels (DEVICE, canonical, row stride labels_ld) mapped to those ids, 0 elsewhere.  *kept (host):
+   number kept; *truncated (host, may be NULL): kept nuclei whose box touches a tile edge that has
+   a neighbour (possibly cut: the overlap is smaller than twice their size).  Synchronous. */
+int nsc_stitch_tile(const int32_t* labels, size_t labels_ld, const int32_t* bbox5, int n, int x0, int y0, int w, int h,
+                    int slide_w, int slide_h, int overlap, int32_t first_id, int32_t* ids, int32_t* owned,
+                    size_t owned_ld, int* kept, int* truncated, void* stream);
+
 /* ---- colour normalization -------------------------------------------------------------- */
 /* Normalization::targetParameters(originalI, targetMean, targetStd)
    (normalization/Normalization.h:44, .cpp:502-582): l-alpha-beta mean and standard deviation of

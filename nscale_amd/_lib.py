@@ -97,6 +97,15 @@ def lib():
         L.nsc_object_features.argtypes = [_P, _I, _I, _P, _I, _P, _I, _I, ctypes.POINTER(ObjFeaturesOut), _P]
         L.nsc_normalization_target_parameters.argtypes = [_P, _I, _I, ctypes.c_size_t, _P, _P, _P, _P]
         L.nsc_normalization.argtypes = [_P, _I, _I, ctypes.c_size_t, _P, _P, _P, ctypes.c_size_t, _P]
+        L.nsc_png_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_I), ctypes.POINTER(_I)]
+        L.nsc_png_write.argtypes = [ctypes.c_char_p, _P, _I, _I, _I, ctypes.c_size_t]
+        L.nsc_free_host.argtypes = [_P]
+        L.nsc_free_host.restype = None
+        L.nsc_labels_to_u8.argtypes = [_P, ctypes.c_size_t, _I, _I, _P, ctypes.c_size_t, _P]
+        L.nsc_segment_nuclei_file.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _P, ctypes.POINTER(_I),
+                                              ctypes.POINTER(_P), _P]
+        L.nsc_stitch_tile.argtypes = [_P, ctypes.c_size_t, _P, _I, _I, _I, _I, _I, _I, _I, _I, ctypes.c_int32, _P, _P,
+                                      ctypes.c_size_t, ctypes.POINTER(_I), ctypes.POINTER(_I), _P]
         L.nsc_watershed_tier_runs.argtypes = [ctypes.POINTER(ctypes.c_longlong), _I]
         L.nsc_recon_autotune.argtypes = [ctypes.POINTER(ReconDesc), _I, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                          ctypes.POINTER(ctypes.c_float)]

@@ -103,8 +103,13 @@ struct RImg {
 template <typename T>
 struct RShared {
     using R = typename RegOf<T>::type;
-    T lm[TH][TW + 4];  // mask per window pixel (frozen pixels: mask = value)
-    T tr[TH][TW + 1];  // transpose buffer (values fit the storage type)
+    // the window's mask (frozen pixels: mask = value) is staged here only to be read back
+    // transposed into registers; the sweeps' transposes then reuse the same LDS (one wave per
+    // workgroup, so the staging reads precede the first transpose write in program order)
+    union {
+        T lm[TH][TW + 4];
+        T tr[TH][TW + 1];
+    };
     R init_l[TH], init_r[TH];
     unsigned chg_bits;
 };
@@ -142,22 +147,25 @@ __device__ __forceinline__ int32_t nbr3(int32_t pc) {
     return max(max(l, pc), r);
 }
 
-// Mask of a window in both orientations.  uint8: packed four to a register (column state: rows
-// 4j..4j+3 of the lane's column; row state: columns 4j..4j+3 of the lane's row), so the sweeps
-// never touch LDS for it.  Four-byte types read it from the LDS copy, prefetching at most eight
-// rows ahead (a scheduling fence every eight steps keeps the register count at four waves/SIMD).
+// Mask of a window in both orientations, in registers, so the sweeps never touch LDS for it:
+// column state (mc: the lane's column, filled while the window loads) and row state (mr: the
+// lane's row, read back transposed from the LDS staging copy).  uint8: packed four to a register
+// (rows / columns 4j..4j+3).  Four-byte types: one value per register (192 registers with the
+// window itself, which keeps two waves per SIMD with the staging buffer shared by the transposes).
 template <typename T>
 struct WinMask {
     using R = typename RegOf<T>::type;
     const T (*lm)[TW + 4];
     int lane;
-    __device__ void build() {}
-    __device__ void opaque() {}
-    __device__ __forceinline__ R clamp_col(R v, R n, int y) const { return clamp_up(v, n, (R)lm[y][lane]); }
-    __device__ __forceinline__ R clamp_row(R v, R n, int x) const { return clamp_up(v, n, (R)lm[lane][x]); }
-    static __device__ __forceinline__ void fence(int k) {
-        if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    R mc[TH], mr[TW];
+    __device__ __forceinline__ void build() {
+#pragma unroll
+        for (int x = 0; x < TW; ++x) mr[x] = (R)lm[lane][x];
     }
+    __device__ void opaque() {}
+    __device__ __forceinline__ R clamp_col(R v, R n, int y) const { return clamp_up(v, n, mc[y]); }
+    __device__ __forceinline__ R clamp_row(R v, R n, int x) const { return clamp_up(v, n, mr[x]); }
+    static __device__ __forceinline__ void fence(int) {}
 };
 template <>
 struct WinMask<uint8_t> {
@@ -325,6 +333,8 @@ __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass
             if constexpr (U8) {
                 if ((y & 3) == 0) M.mc[y >> 2] = 0;
                 M.mc[y >> 2] |= (uint32_t)mw << (8 * (y & 3));
+            } else {
+                M.mc[y] = mw;
             }
         }
         __builtin_amdgcn_sched_barrier(0);

@@ -353,7 +353,8 @@ __device__ __forceinline__ int cdiff_packed(uint32_t a, uint32_t b) {
 
 template <int LCAP>
 __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restrict__ box5, int nrun,
-                                                   const int* __restrict__ list, const int* __restrict__ count) {
+                                                   const int* __restrict__ list, const int* __restrict__ count,
+                                                   int* __restrict__ next_run) {
     __shared__ uint8_t code[LCAP];
     __shared__ int32_t lk[LCAP];     // known label: seed label, label of a popped F pixel, else 0
     __shared__ uint8_t lv0[LCAP];    // initial push level (valid with K_LV)
@@ -363,13 +364,19 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     __shared__ uint16_t qn[2 * LCAP];  // FIFO links: F item k, leaf of B pixel k at LCAP + k
     __shared__ uint16_t lfj[LCAP];
     __shared__ uint16_t qhead[NQ], qtail[NQ];
-    __shared__ int sh_base;
+    __shared__ int sh_base, sh_it;
 
-    // a fixed grid strides over the tier's runs: launching one workgroup per run of the whole
+    // a fixed grid takes the tier's runs from a work counter, one at a time, so a workgroup that
+    // drew short runs takes more of them (a fixed stride over the list left the tier's time set by
+    // the workgroup that drew several long floods); launching one workgroup per run of the whole
     // image (most of them idle for this tier) would hold the large LDS allocation of every idle
     // workgroup across the device
     const int nr = *count;
-    for (int it = blockIdx.x; it < nr; it += gridDim.x) {
+    for (;;) {
+    if (threadIdx.x == 0) sh_it = atomicAdd(next_run, 1);
+    __syncthreads();
+    const int it = sh_it;
+    if (it >= nr) break;
     [&]() {  // one run; `return` ends the run
     const int r = list[it];
     const int lane = threadIdx.x;
@@ -1513,7 +1520,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     carve(cut, cls, lvl0, fmask, run, st, ilvl, pseq, parent, bnode, popidx, lab, head, real, lab2, kind, small);
     // small[]: 0 nF, 1 nrun, 2 nleaf, 3 flags, 4 ncand, 8..10 info, 16 ncomp, 20 deferred tie components,
     // 24..25 tie pool bump pointer (long long), 26..31 run bucket sizes, 32..34 LDS flood queue counts,
-    // 35 B pixels with queue entries
+    // 35 B pixels with queue entries, 40..44 phase-1 tier work counters
     NSC_HIP(hipMemsetAsync(small, 0, 64 * sizeof(int), s));
 
     Dev D{};
@@ -1573,15 +1580,22 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         }
         auto L = [&](int b) { return lists + (size_t)b * nrun; };
         // the few large runs set the latency: the largest tiers start first, each on a side stream,
-        // beside the many small runs on the caller's stream
+        // beside the many small runs on the caller's stream; the 512 tier follows the 2048 tier
+        // (together shorter than the 1024 tier) instead of the 1024 tier
         SideStream& ss = side_stream();
         NSC_CHECK(ss.fork(s));
-        hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(std::min(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(4), bcount + 4);
+        int* next_run = &small[40];  // per-tier work counters (zeroed with small[])
+        hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(std::min(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(4), bcount + 4,
+                           next_run + 4);
         hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(5), bcount + 5, qh2, qt);
-        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(std::min(nrun, 256)), dim3(64), 0, ss.side[1], D, box5, nrun, L(3), bcount + 3);
-        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(std::min(nrun, 768)), dim3(64), 0, ss.side[2], D, box5, nrun, L(2), bcount + 2);
-        hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(std::min(nrun, 1536)), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1);
-        hipLaunchKernelGGL(k_phase1_lds<512>, dim3(std::min(nrun, 3072)), dim3(64), 0, s, D, box5, nrun, L(0), bcount);
+        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(std::min(nrun, 256)), dim3(64), 0, ss.side[1], D, box5, nrun, L(3), bcount + 3,
+                           next_run + 3);
+        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(std::min(nrun, 768)), dim3(64), 0, ss.side[2], D, box5, nrun, L(2), bcount + 2,
+                           next_run + 2);
+        hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(std::min(nrun, 1536)), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1,
+                           next_run + 1);
+        hipLaunchKernelGGL(k_phase1_lds<512>, dim3(std::min(nrun, 3072)), dim3(64), 0, ss.side[2], D, box5, nrun, L(0), bcount,
+                           next_run);
         NSC_CHECK(ss.join(s));
     }
     // B pixels with a queue entry: initially queued ones, then those with leaves (the phase-1

@@ -283,7 +283,12 @@ def bench_batched(args):
     t_plain, outs_p = timed(lambda: [gpu.imreconstruct(mk, m, c) for (mk, m), c in zip(pairs, conns)], reps)
     t_batch, outs_b = timed(lambda: gpu.imreconstruct_batched(pairs, conns), reps)
     exact_u8 = all(torch.equal(a, b) for a, b in zip(outs_p, outs_b))
-    del outs_p, outs_b
+    # the same batch as a replayed hipGraph (16 window passes per replay)
+    graph = gpu.ReconGraph(pairs, conns, passes=16)
+    t_graph, g_passes = timed(graph.run, reps)
+    exact_u8 = exact_u8 and all(torch.equal(a, b) for a, b in zip(graph.outs, outs_b))
+    graph.close()
+    del outs_p, outs_b, graph
     # float32 h-minima on -EDT(mask < 150)
     imgs = []
     for _, m in pairs:
@@ -300,6 +305,7 @@ def bench_batched(args):
                       "megapixels": round(px / 1e6, 1)},
            "plain_mpix_s": round(px / t_plain / 1e6, 1), "batched_vs_plain": round(t_plain / t_batch, 2),
            "autotuned": tuned,
+           "graph": {"mpix_s": round(px / t_graph / 1e6, 1), "passes": g_passes, "passes_per_replay": 16},
            "bit_exact": exact_u8 and exact_f32,
            "hminima_f32": {"batched_mpix_s": round(px / t_hbatch / 1e6, 1), "plain_mpix_s": round(px / t_hplain / 1e6, 1),
                            "batched_vs_plain": round(t_hplain / t_hbatch, 2)},

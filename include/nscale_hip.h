@@ -90,6 +90,19 @@ typedef struct nsc_recon_desc {
 } nsc_recon_desc;
 int nsc_imreconstruct_u8_batched(const nsc_recon_desc* descs, int n, void* stream);
 int nsc_imreconstruct_f32_batched(const nsc_recon_desc* descs, int n, void* stream);
+/* A batched reconstruction as a replayable hipGraph (no reference counterpart: for callers that
+   replay one batch layout many times or embed it in their own graphs).  Create captures a graph of
+   `passes_per_replay` window passes (uint8, or float when is_float) over the images of descs --
+   the same buffers every run, the contents may change between runs; the descriptors' buffers must
+   stay allocated until destroy.  Run resets the pass state with kernels (no runtime fills are
+   recorded), replays the graph until no tile is left marked (one host check per replay) and
+   synchronizes the stream; *passes (may be NULL) receives the passes launched.  Results equal
+   nsc_imreconstruct_u8_batched / _f32_batched on the same inputs. */
+typedef struct nsc_recon_graph nsc_recon_graph;
+int nsc_recon_graph_create(const nsc_recon_desc* descs, int n, int is_float, int passes_per_replay,
+                           nsc_recon_graph** graph);
+int nsc_recon_graph_run(nsc_recon_graph* graph, int* passes, void* stream);
+int nsc_recon_graph_destroy(nsc_recon_graph* graph);
 /* Autotuner of the reconstruction schedule (no reference counterpart; the reference has fixed
    shapes): times each candidate -- windows per workgroup side 1 or 2 with 4 or 8 passes between
    host checks, and the device-side work queue for the residual wavefront -- on this batch (outputs

@@ -106,6 +106,9 @@ def lib():
                                               ctypes.POINTER(_P), _P]
         L.nsc_stitch_tile.argtypes = [_P, ctypes.c_size_t, _P, _I, _I, _I, _I, _I, _I, _I, _I, ctypes.c_int32, _P, _P,
                                       ctypes.c_size_t, ctypes.POINTER(_I), ctypes.POINTER(_I), _P]
+        L.nsc_recon_graph_create.argtypes = [ctypes.POINTER(ReconDesc), _I, _I, _I, ctypes.POINTER(_P)]
+        L.nsc_recon_graph_run.argtypes = [_P, ctypes.POINTER(_I), _P]
+        L.nsc_recon_graph_destroy.argtypes = [_P]
         L.nsc_watershed_tier_runs.argtypes = [ctypes.POINTER(ctypes.c_longlong), _I]
         L.nsc_recon_autotune.argtypes = [ctypes.POINTER(ReconDesc), _I, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                          ctypes.POINTER(ctypes.c_float)]

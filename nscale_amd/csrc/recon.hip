@@ -408,12 +408,12 @@ __device__ __forceinline__ void recon_tile(const RImg<T>& im, int tile, int pass
 // stride over their work; a batch's images are found through tile_img (global tile -> image).
 // CONN 0 = per-image connectivity.
 template <typename T, int CONN>
-__global__ __launch_bounds__(64, RECON_WAVES) void recon_tiles(const RImg<T>* __restrict__ imgs, RImg<T> one, const int* __restrict__ tile_img,
-                                                  int ntiles, const int* __restrict__ list, const int* __restrict__ list_count,
-                                                  int* __restrict__ next_list, int* __restrict__ next_count,
-                                                  int* __restrict__ clear_count, unsigned* __restrict__ stamp, int pass,
-                                                  unsigned long long* __restrict__ dirty) {
-    __shared__ RShared<T> sh;
+__device__ __forceinline__ void recon_tiles_body(RShared<T>& sh, const RImg<T>* __restrict__ imgs, const RImg<T>& one,
+                                                 const int* __restrict__ tile_img, int ntiles, const int* __restrict__ list,
+                                                 const int* __restrict__ list_count, int* __restrict__ next_list,
+                                                 int* __restrict__ next_count, int* __restrict__ clear_count,
+                                                 unsigned* __restrict__ stamp, int pass,
+                                                 unsigned long long* __restrict__ dirty) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && clear_count) *clear_count = 0;  // the counter of pass + 2 (ring)
     const int n = (pass == 0) ? ntiles : *list_count;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
@@ -426,6 +426,17 @@ __global__ __launch_bounds__(64, RECON_WAVES) void recon_tiles(const RImg<T>* __
             recon_tile<T, 4>(im, local, pass, sh, next_list, next_count, stamp, dirty);
         __syncthreads();  // the shared window is reused by the next tile
     }
+}
+
+template <typename T, int CONN>
+__global__ __launch_bounds__(64, RECON_WAVES) void recon_tiles(const RImg<T>* __restrict__ imgs, RImg<T> one, const int* __restrict__ tile_img,
+                                                  int ntiles, const int* __restrict__ list, const int* __restrict__ list_count,
+                                                  int* __restrict__ next_list, int* __restrict__ next_count,
+                                                  int* __restrict__ clear_count, unsigned* __restrict__ stamp, int pass,
+                                                  unsigned long long* __restrict__ dirty) {
+    __shared__ RShared<T> sh;
+    recon_tiles_body<T, CONN>(sh, imgs, one, tile_img, ntiles, list, list_count, next_list, next_count, clear_count, stamp,
+                              pass, dirty);
 }
 
 // ---- the residual wavefront as a device-side work queue ----------------------------------------
@@ -1104,7 +1115,176 @@ static int recon_autotune(const nsc_recon_desc* d, int n, hipStream_t s, int* re
 template int recon_batched_impl<uint8_t>(const nsc_recon_desc*, int, hipStream_t);
 template int recon_batched_impl<float>(const nsc_recon_desc*, int, hipStream_t);
 
+namespace {
+
+// ---- a batched reconstruction as a replayable hipGraph ----------------------------------------
+// The pass structure of recon_passes with the pass number read from a device counter: one captured
+// sequence of K window-pass launches and the counter's advance replays as passes base .. base+K-1,
+// base+K .. base+2K-1, and so on.  A pass whose list is empty only clears its ring counter, so
+// passes past convergence cost a near-empty launch; the host checks the last pass's count once per
+// replay.  Every clear is a kernel (nothing a graph would record as a runtime fill).
+template <typename T, int CONN>
+__global__ __launch_bounds__(64, RECON_WAVES) void recon_tiles_graph(const RImg<T>* __restrict__ imgs,
+                                                                     const int* __restrict__ tile_img, int ntiles,
+                                                                     int* __restrict__ listA, int* __restrict__ listB,
+                                                                     int* __restrict__ counts, unsigned* __restrict__ stamp,
+                                                                     const int* __restrict__ base, int i) {
+    __shared__ RShared<T> sh;
+    const int pass = *base + i;
+    const int* list = (pass & 1) ? listA : listB;
+    int* next_list = (pass & 1) ? listB : listA;
+    const int* list_count = pass ? &counts[(pass - 1) % kRing] : nullptr;
+    recon_tiles_body<T, CONN>(sh, imgs, imgs[0], tile_img, ntiles, list, list_count, next_list, &counts[pass % kRing],
+                              &counts[(pass + 1) % kRing], stamp, pass, nullptr);
+}
+
+__global__ void k_pass_advance(int* base, int k) {
+    if (threadIdx.x == 0) *base += k;
+}
+
+__global__ void k_pass_reset(int* base) {
+    if (threadIdx.x == 0) *base = 0;
+}
+
+}  // namespace
 }  // namespace nsc
+
+struct nsc_recon_graph {
+    int is_float = 0, K = 0, nt = 0;
+    long long pixels = 0;
+    void* mem = nullptr;  // image table, tile -> image map, pass scratch, pass counter
+    nsc::ReconBufs B{};
+    int* base = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace nsc {
+namespace {
+
+template <typename T>
+int recon_graph_build(const nsc_recon_desc* d, int n, int K, nsc_recon_graph* g) {
+    std::vector<RImg<T>> imgs(n);
+    long long nt = 0, pixels = 0;
+    int conn_all = d[0].conn;
+    for (int i = 0; i < n; ++i) {
+        NSC_ARG(d[i].marker && d[i].mask && d[i].out && d[i].w > 0 && d[i].h > 0 && (d[i].conn == 4 || d[i].conn == 8),
+                "recon_graph: descriptor");
+        RImg<T>& r = imgs[i];
+        r.marker = static_cast<const T*>(d[i].marker);
+        r.mask = static_cast<const T*>(d[i].mask);
+        r.out = static_cast<T*>(d[i].out);
+        r.w = d[i].w;
+        r.h = d[i].h;
+        r.tiles_x = (int)cdiv(r.w, OW);
+        r.tiles_y = (int)cdiv(r.h, OH);
+        r.conn = d[i].conn;
+        r.base = (int)nt;
+        set_regions(r, 1);
+        nt += (long long)r.rgx * r.rgy;
+        pixels += (long long)r.w * r.h;
+        if (d[i].conn != conn_all) conn_all = 0;
+    }
+    NSC_ARG(nt < (1ll << 30), "recon_graph: too many tiles");
+    std::vector<int> tile_img((size_t)nt);
+    for (int i = 0; i < n; ++i)
+        std::fill(tile_img.begin() + imgs[i].base, tile_img.begin() + imgs[i].base + imgs[i].rgx * imgs[i].rgy, i);
+    Carver m(nullptr);
+    m.take<RImg<T>>(n);
+    m.take<int>((size_t)nt);
+    m.take<uint8_t>(recon_scratch_bytes((int)nt));
+    m.take<int>(1);
+    NSC_HIP(hipMalloc(&g->mem, m.size()));
+    Carver c(g->mem);
+    RImg<T>* imgs_dev = c.take<RImg<T>>(n);
+    int* tile_dev = c.take<int>((size_t)nt);
+    g->B = recon_carve(c.take<uint8_t>(recon_scratch_bytes((int)nt)), (int)nt);
+    g->base = c.take<int>(1);
+    NSC_HIP(hipMemcpy(imgs_dev, imgs.data(), sizeof(RImg<T>) * n, hipMemcpyHostToDevice));
+    NSC_HIP(hipMemcpy(tile_dev, tile_img.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice));
+    g->nt = (int)nt;
+    g->pixels = pixels;
+    g->K = K;
+    NSC_HIP(hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking));
+    NSC_HIP(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
+    const int grid = (int)std::min<long long>(nt, 1 << 16);
+    for (int i = 0; i < K; ++i) {
+        if (conn_all == 8)
+            hipLaunchKernelGGL((recon_tiles_graph<T, 8>), dim3(grid), dim3(TW), 0, g->cap, imgs_dev, tile_dev, (int)nt,
+                               g->B.listA, g->B.listB, g->B.counts, g->B.stamp, g->base, i);
+        else if (conn_all == 4)
+            hipLaunchKernelGGL((recon_tiles_graph<T, 4>), dim3(grid), dim3(TW), 0, g->cap, imgs_dev, tile_dev, (int)nt,
+                               g->B.listA, g->B.listB, g->B.counts, g->B.stamp, g->base, i);
+        else
+            hipLaunchKernelGGL((recon_tiles_graph<T, 0>), dim3(grid), dim3(TW), 0, g->cap, imgs_dev, tile_dev, (int)nt,
+                               g->B.listA, g->B.listB, g->B.counts, g->B.stamp, g->base, i);
+    }
+    hipLaunchKernelGGL(k_pass_advance, dim3(1), dim3(64), 0, g->cap, g->base, K);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(g->cap, &g->graph);
+    NSC_HIP(le);
+    NSC_HIP(ce);
+    NSC_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    return NSC_OK;
+}
+
+}  // namespace
+}  // namespace nsc
+
+using nsc::set_error;
+
+extern "C" int nsc_recon_graph_destroy(nsc_recon_graph* g) {
+    if (!g) return NSC_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->cap) (void)hipStreamDestroy(g->cap);
+    if (g->mem) (void)hipFree(g->mem);
+    delete g;
+    return NSC_OK;
+}
+
+extern "C" int nsc_recon_graph_create(const nsc_recon_desc* descs, int n, int is_float, int passes_per_replay,
+                                      nsc_recon_graph** out) {
+    NSC_CHECK(nsc::ensure_init());
+    NSC_ARG(descs && n > 0 && out, "recon_graph_create");
+    NSC_ARG(passes_per_replay >= 1 && passes_per_replay <= 256, "recon_graph_create: passes per replay 1..256");
+    *out = nullptr;
+    auto* g = new nsc_recon_graph();
+    g->is_float = is_float ? 1 : 0;
+    const int rc = is_float ? nsc::recon_graph_build<float>(descs, n, passes_per_replay, g)
+                            : nsc::recon_graph_build<uint8_t>(descs, n, passes_per_replay, g);
+    if (rc != NSC_OK) {
+        nsc_recon_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return NSC_OK;
+}
+
+extern "C" int nsc_recon_graph_run(nsc_recon_graph* g, int* passes, void* stream) {
+    NSC_ARG(g && g->exec, "recon_graph_run");
+    const hipStream_t s = nsc::as_stream(stream);
+    nsc::recon_clear(g->B, s);
+    hipLaunchKernelGGL(nsc::k_pass_reset, dim3(1), dim3(64), 0, s, g->base);
+    NSC_HIP(hipGetLastError());
+    const long long max_pass = nsc::pass_limit(g->pixels);
+    for (long long done = 0;;) {
+        NSC_HIP(hipGraphLaunch(g->exec, s));
+        done += g->K;
+        int c = 0;
+        NSC_HIP(hipMemcpyAsync(&c, &g->B.counts[(done - 1) % nsc::kRing], sizeof(int), hipMemcpyDeviceToHost, s));
+        NSC_HIP(nsc::stream_sync(s));
+        if (c == 0) {
+            if (passes) *passes = (int)std::min<long long>(done, INT_MAX);
+            return NSC_OK;
+        }
+        if (done > max_pass) {
+            set_error("recon_graph_run: no convergence after %lld passes", max_pass);
+            return NSC_ERR_LIMIT;
+        }
+    }
+}
 
 // ---- reconstruction without allocation ---------------------------------------------------
 // The pass driver's scratch (tile stamps, the counter ring, two tile lists: 12 B per tile plus the
@@ -1183,10 +1363,6 @@ extern "C" int nsc_recon_autotune(const nsc_recon_desc* descs, int n, int is_flo
     return is_float ? nsc::recon_autotune<float>(descs, n, s, region, passes, timings)
                     : nsc::recon_autotune<uint8_t>(descs, n, s, region, passes, timings);
 }
-
-namespace nsc {
-
-}  // namespace nsc
 
 // ------------------------------------------------------------------------------------------
 // Geodesic (4-connected, unit step) distance inside a passable mask from source pixels with

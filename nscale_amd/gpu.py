@@ -66,6 +66,43 @@ def imreconstruct_batched(pairs, conns):
     return outs
 
 
+class ReconGraph:
+    """A batched reconstruction captured as a hipGraph of `passes` window passes per replay: the
+    (marker, mask) pairs and the outputs are fixed at creation (their contents may change), run()
+    recomputes every output (replaying the graph until no tile is left marked) and returns the
+    number of passes launched."""
+
+    def __init__(self, pairs, conns, passes=16):
+        n = len(pairs)
+        self.dtype = pairs[0][1].dtype
+        self.pairs = pairs
+        self.outs = [torch.empty_like(mask) for _, mask in pairs]
+        arr = (ReconDesc * n)()
+        for i, ((marker, mask), o) in enumerate(zip(pairs, self.outs)):
+            assert mask.dtype == self.dtype and marker.shape == mask.shape
+            h, w = _hw(mask)
+            arr[i] = ReconDesc(_ptr(marker), _ptr(mask), _ptr(o), w, h, int(conns[i]))
+        self.handle = _P()
+        call("nsc_recon_graph_create", arr, n, 1 if self.dtype == torch.float32 else 0, int(passes),
+             ctypes.byref(self.handle))
+
+    def run(self):
+        k = ctypes.c_int(0)
+        call("nsc_recon_graph_run", self.handle, ctypes.byref(k), _stream())
+        return k.value
+
+    def close(self):
+        if self.handle:
+            lib().nsc_recon_graph_destroy(self.handle)
+            self.handle = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def autotune_recon(pairs, conns):
     """Pick the reconstruction schedule (windows per workgroup side, passes per host check) by timing
     each candidate on this batch; later calls of the same element type use the choice.

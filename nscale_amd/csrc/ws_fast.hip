@@ -91,10 +91,6 @@ __device__ __forceinline__ int c_diff(const uint8_t* a, const uint8_t* b) {
 }
 
 // ---- item accessors: id < N -> F pixel or initially queued B pixel; id >= N -> leaf slot ----
-__device__ __forceinline__ int it_run(const Dev& D, int x) {
-    if (x >= D.N) return D.lf_run[x - D.N];
-    return D.cls[x] == C_F ? D.run[x] : -(x + 2);
-}
 __device__ __forceinline__ int it_lvl(const Dev& D, int x) {
     if (x >= D.N) return D.lf_lvl[x - D.N];
     return D.cls[x] == C_F ? D.ilvl[x] : D.lvl0[x];
@@ -102,14 +98,6 @@ __device__ __forceinline__ int it_lvl(const Dev& D, int x) {
 __device__ __forceinline__ int it_parent(const Dev& D, int x) {
     if (x >= D.N) return D.lf_parent[x - D.N];
     return D.cls[x] == C_F ? D.parent[x] : -1;
-}
-__device__ __forceinline__ int it_bnode(const Dev& D, int x) {
-    if (x >= D.N) return D.lf_bnode[x - D.N];
-    return D.cls[x] == C_F ? D.bnode[x] : x;
-}
-__device__ __forceinline__ int it_popidx(const Dev& D, int x) {
-    if (x >= D.N) return D.lf_popidx[x - D.N];
-    return D.cls[x] == C_F ? D.popidx[x] : 0;
 }
 __device__ __forceinline__ int it_pseq(const Dev& D, int x) {
     if (x >= D.N) return D.lf_pseq[x - D.N];

@@ -310,8 +310,12 @@ class Workspace:
     """Device workspace for segment_nuclei on tiles up to max_w x max_h."""
 
     def __init__(self, max_w, max_h):
+        self.max_w, self.max_h = int(max_w), int(max_h)
         self.handle = _P()
         call("nsc_workspace_create", max_w, max_h, ctypes.byref(self.handle))
+
+    def fits(self, w, h):
+        return (w <= self.max_w and h <= self.max_h) or (w <= self.max_h and h <= self.max_w)
 
     def __del__(self):
         try:

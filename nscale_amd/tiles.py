@@ -75,12 +75,65 @@ def segment_file(in_path, out_path, params=None, want_bbox=True):
     return st, cc.value, bbox
 
 
+def read_image(path):
+    """cv::imread(path) (IMREAD_COLOR) as (h, w, 3) uint8 BGR: PNG by the library's codec, other
+    formats (the reference's .tif / .tiff tiles) through PIL when it is installed; None when the
+    file cannot be decoded (imread's empty Mat)."""
+    if str(path).lower().endswith(".png"):
+        try:
+            return read_png(path)
+        except NscError:
+            return None
+    try:
+        from PIL import Image
+    except ImportError:
+        return None
+    try:
+        with Image.open(path) as im:
+            rgb = np.asarray(im.convert("RGB"))
+    except Exception:
+        return None
+    return np.ascontiguousarray(rgb[..., ::-1])
+
+
+def segment_image(bgr, out_path, params=None, ws=None):
+    """The chain on a host BGR tile (the file-level segmentNuclei after imread): on success writes
+    the label image to out_path as imwrite stores it.  Returns (status, compcount)."""
+    import torch
+    from nscale_amd import gpu
+    from nscale_amd._lib import BBOX_REFERENCE_CPU
+    dev = torch.from_numpy(np.ascontiguousarray(bgr)).cuda()
+    h, w = bgr.shape[:2]
+    st, lab, cc, _, _, _, _ = gpu.segment_nuclei_io(dev, params=params, ws=ws, bbox_mode=BBOX_REFERENCE_CPU,
+                                                   max_boxes=1 << 16)
+    if st != 0:
+        return st, 0
+    mask = torch.empty(h, w, dtype=torch.uint8, device=dev.device)
+    call("nsc_labels_to_u8", ctypes.c_void_p(lab.data_ptr()), 0, w, h, ctypes.c_void_p(mask.data_ptr()), 0,
+         ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    write_png(out_path, mask.cpu().numpy())
+    return st, cc
+
+
 def segment_directory(image_dir, out_dir, params=None):
-    """nu-segment's serial mode (nu-segment.cpp:505-528): every tile under image_dir through
-    segment_file, outputs named by output_names.  Returns [(image, status, compcount)]."""
+    """nu-segment's serial mode (nu-segment.cpp:505-528): every tile under image_dir segmented, the
+    label image written to the name output_names gives (PNG tiles through segment_file, other
+    formats decoded by read_image).  Returns [(image, status, compcount)]; an undecodable tile gives
+    the reference's INVALID_IMAGE (-1)."""
     res = []
+    ws = None
     for img in list_images(image_dir):
         mask, _ = output_names(img, image_dir, out_dir)
-        st, cc, _ = segment_file(img, mask, params, want_bbox=False)
+        if img.lower().endswith(".png"):
+            st, cc, _ = segment_file(img, mask, params, want_bbox=False)
+        else:
+            bgr = read_image(img)
+            if bgr is None:
+                st, cc = -1, 0
+            else:
+                if ws is None or not ws.fits(bgr.shape[1], bgr.shape[0]):
+                    from nscale_amd import gpu
+                    ws = gpu.Workspace(bgr.shape[1], bgr.shape[0])
+                st, cc = segment_image(bgr, mask, params, ws)
         res.append((img, st, cc))
     return res

@@ -150,6 +150,25 @@ def test_file_segmentation_writes_the_saturated_labels(tmp_path):
     assert np.array_equal(tiles.read_png(tmp_path / "out" / "t-0-0.mask.png")[..., 0], tiles.mask_of_labels(olab))
 
 
+def test_tiff_tiles_in_directory_mode(tmp_path):
+    """The reference's .tif tiles (decoded through PIL when installed) take the same chain."""
+    pil = pytest.importorskip("PIL.Image")
+    from nscale_amd import tiles
+    from nscale_amd.synth import synth_he
+    from tests._lib import o_segment
+    bgr = synth_he(300, 420, 43)
+    d = tmp_path / "in"
+    d.mkdir()
+    pil.fromarray(bgr[..., ::-1].copy()).save(d / "s-0-0.tif")
+    (d / "broken.tiff").write_bytes(b"II*\x00junk")
+    assert np.array_equal(tiles.read_image(d / "s-0-0.tif"), bgr)
+    res = tiles.segment_directory(str(d), str(tmp_path / "out"))
+    assert [(os.path.basename(r[0]), r[1]) for r in res] == [("broken.tiff", -1), ("s-0-0.tif", 0)]
+    ost, olab, _ = o_segment(bgr)
+    assert ost == 0
+    assert np.array_equal(tiles.read_png(tmp_path / "out" / "s-0-0.mask.png")[..., 0], tiles.mask_of_labels(olab))
+
+
 def test_slide_driver_outputs(tmp_path):
     from nscale_amd import gpu, tiles
     from nscale_amd.slide import segment_slide, tile_grid

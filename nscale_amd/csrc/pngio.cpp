@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -58,7 +59,7 @@ int paeth(int a, int b, int c) {
 
 using nsc::set_error;
 
-extern "C" int nsc_png_write(const char* path, const uint8_t* img, int w, int h, int channels, size_t ld) {
+static int png_write_impl(const char* path, const uint8_t* img, int w, int h, int channels, size_t ld) {
     NSC_ARG(path && img && w > 0 && h > 0 && (channels == 1 || channels == 3), "png_write");
     if (!ld) ld = (size_t)w * channels;
     NSC_ARG(ld >= (size_t)w * channels, "png_write: row stride");
@@ -105,7 +106,7 @@ extern "C" int nsc_png_write(const char* path, const uint8_t* img, int w, int h,
     return NSC_OK;
 }
 
-extern "C" int nsc_png_read(const char* path, uint8_t** bgr, int* w_out, int* h_out) {
+static int png_read_impl(const char* path, uint8_t** bgr, int* w_out, int* h_out) {
     NSC_ARG(path && bgr && w_out && h_out, "png_read");
     *bgr = nullptr;
     File in(fopen(path, "rb"));
@@ -272,15 +273,15 @@ extern "C" int nsc_labels_to_u8(const int32_t* labels, size_t lab_ld, int w, int
     return NSC_OK;
 }
 
-extern "C" int nsc_segment_nuclei_file(const char* in, const char* out, const nsc_seg_params* params, int* compcount,
-                                       int32_t** bbox, void* stream) {
+static int segment_file_impl(const char* in, const char* out, const nsc_seg_params* params, int* compcount,
+                             int32_t** bbox, void* stream) {
     NSC_ARG(in && out && compcount, "segment_nuclei_file");
     NSC_CHECK(nsc::ensure_init());
     if (bbox) *bbox = nullptr;
     *compcount = 0;
     uint8_t* host = nullptr;
     int w = 0, h = 0;
-    if (nsc_png_read(in, &host, &w, &h) != NSC_OK) return NSC_SEG_INVALID_IMAGE;  // imread failed
+    if (png_read_impl(in, &host, &w, &h) != NSC_OK) return NSC_SEG_INVALID_IMAGE;  // imread failed
     std::unique_ptr<uint8_t, void (*)(void*)> hold(host, free);
     const hipStream_t s = nsc::as_stream(stream);
     const size_t px = (size_t)w * h;
@@ -326,6 +327,36 @@ extern "C" int nsc_segment_nuclei_file(const char* in, const char* out, const ns
     std::vector<uint8_t> mask(px);
     NSC_HIP(hipMemcpyAsync(mask.data(), dmask.p, px, hipMemcpyDeviceToHost, s));
     NSC_HIP(hipStreamSynchronize(s));
-    NSC_CHECK(nsc_png_write(out, mask.data(), w, h, 1, (size_t)w));
+    NSC_CHECK(png_write_impl(out, mask.data(), w, h, 1, (size_t)w));
     return NSC_SEG_SUCCESS;
+}
+
+// host buffers of image size are std::vectors: an allocation failure becomes an error code, never
+// an exception across the C ABI
+extern "C" int nsc_png_write(const char* path, const uint8_t* img, int w, int h, int channels, size_t ld) {
+    try {
+        return png_write_impl(path, img, w, h, channels, ld);
+    } catch (const std::bad_alloc&) {
+        set_error("png_write: out of host memory");
+        return NSC_ERR_NOMEM;
+    }
+}
+
+extern "C" int nsc_png_read(const char* path, uint8_t** bgr, int* w, int* h) {
+    try {
+        return png_read_impl(path, bgr, w, h);
+    } catch (const std::bad_alloc&) {
+        set_error("png_read: out of host memory");
+        return NSC_ERR_NOMEM;
+    }
+}
+
+extern "C" int nsc_segment_nuclei_file(const char* in, const char* out, const nsc_seg_params* params, int* compcount,
+                                       int32_t** bbox, void* stream) {
+    try {
+        return segment_file_impl(in, out, params, compcount, bbox, stream);
+    } catch (const std::bad_alloc&) {
+        set_error("segment_nuclei_file: out of host memory");
+        return NSC_ERR_NOMEM;
+    }
 }

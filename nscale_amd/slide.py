@@ -96,9 +96,27 @@ def segment_slide(tile_source, width, height, tile=4096, overlap=64, name="slide
         os.makedirs(out_dir, exist_ok=True)
     inflight, report = [], []
 
+    # Writing masks needs the reference's labels (smallest raster index, background -1: the file
+    # is their 8-bit saturation); the stitcher then relabels them canonically here.  Without
+    # outputs the server relabels and boxes them itself.
+    write = bool(out_dir)
+
     def finish(entry):
         tk, (x0, y0, w, h) = entry
         status, n, lab, bbox, area, res = srv.wait(tk)
+        if res.error != 0:  # e.g. more nuclei than max_labels: reported, not stitched
+            status = res.error
+        if write:
+            mask = torch.empty(h, w, dtype=torch.uint8, device=lab.device)
+            if status == 0:  # the chain's labels, as imwrite stores them
+                call("nsc_labels_to_u8", _P(lab.data_ptr()), 0, w, h, _P(mask.data_ptr()), 0,
+                     _P(torch.cuda.current_stream().cuda_stream))
+            else:
+                mask.zero_()
+            tiles.write_png(os.path.join(out_dir, "%s-%d-%d.mask.png" % (name, x0, y0)), mask.cpu().numpy())
+            if status == 0:
+                n = gpu.cc_relabel(lab)
+                bbox, area = gpu.bounding_boxes(lab, n)
         report.append((x0, y0, status, n))
         if status == 0:
             _, own = st.add(x0, y0, lab, bbox, area, owned=raster is not None)
@@ -106,14 +124,6 @@ def segment_slide(tile_source, width, height, tile=4096, overlap=64, name="slide
                 reg = raster[y0:y0 + h, x0:x0 + w]
                 st.conflicts += int(((reg > 0) & (own > 0) & (reg != own)).sum())
                 reg.copy_(torch.where(own > 0, own, reg))
-        if out_dir:
-            mask = torch.empty(h, w, dtype=torch.uint8, device=lab.device)
-            if status == 0:
-                call("nsc_labels_to_u8", _P(lab.data_ptr()), 0, w, h, _P(mask.data_ptr()), 0,
-                     _P(torch.cuda.current_stream().cuda_stream))
-            else:
-                mask.zero_()
-            tiles.write_png(os.path.join(out_dir, "%s-%d-%d.mask.png" % (name, x0, y0)), mask.cpu().numpy())
 
     try:
         for i, (x0, y0, w, h) in enumerate(grid):
@@ -121,7 +131,8 @@ def segment_slide(tile_source, width, height, tile=4096, overlap=64, name="slide
                 finish(inflight.pop(0))
             buf = ring[i % len(ring)][:h * w * 3].view(h, w, 3)
             tile_source(x0, y0, w, h, buf)
-            inflight.append((srv.submit(buf, canonical=True, max_labels=max_labels, params=params), (x0, y0, w, h)))
+            inflight.append((srv.submit(buf, canonical=not write, max_labels=0 if write else max_labels, params=params),
+                             (x0, y0, w, h)))
         for e in inflight:
             finish(e)
     finally:

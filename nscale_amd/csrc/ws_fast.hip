@@ -320,14 +320,17 @@ __global__ void k_phase1(Dev D, const int32_t* __restrict__ box5, int nrun, cons
 
 // ---- phase 1 with the run's neighbourhood staged in LDS ----
 // Same flood as k_phase1, for runs whose bounding box plus a 2-pixel margin has at most LCAP
-// pixels.  The wave stages the frame (class, in-run flag, seed labels, push levels, colours) into
-// LDS and numbers the initial items in raster order with ballots; lane 0 then runs the flood with
-// every read in LDS, writing the per-item records to global memory as it goes.  Each queued F
-// pixel carries the key (level, push sequence) and global id of its chain's largest-key item, so a
-// push compares keys without following pointers.  The 2-pixel margin keeps every neighbour of a
-// leaf candidate inside the frame.  Leaf slots are reserved per run up front (one per B pixel of
-// the frame that could become a leaf), so no atomic sits on the flood's critical path; unused
-// slots keep lf_b = -1.
+// pixels.  The wave stages the frame into LDS (class, in-run flag, seed labels, and the push level
+// of every pixel towards its right and lower neighbour, from the colours), numbers the initial
+// items in raster order with ballots and links them into their level FIFOs chunk by chunk; lane 0
+// then runs the flood with every read in LDS, writing the per-item records to global memory as it
+// goes.  Each queued F pixel carries the key (level, push sequence) and frame index of its chain's
+// largest-key item, so a push compares keys without following pointers.  The 2-pixel margin keeps
+// every neighbour of a leaf candidate inside the frame.  Leaf slots are reserved per run up front
+// (one per B pixel of the frame that could become a leaf), so no atomic sits on the flood's
+// critical path; unused slots keep lf_b = -1.
+// LDS per frame pixel: 15 bytes (class 1, label 4, chain key 4, push levels 2, chain item or leaf
+// index 2, FIFO link 2), so a CU holds as many concurrent floods as its LDS allows for the tier.
 constexpr uint16_t NIL16 = 0xFFFF;
 enum { K_INRUN = 4, K_ST1 = 8, K_ST2 = 16, K_LV = 32 };
 constexpr int P1_MARGIN = 2;
@@ -345,12 +348,11 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                                                    int* __restrict__ next_run) {
     __shared__ uint8_t code[LCAP];
     __shared__ int32_t lk[LCAP];     // known label: seed label, label of a popped F pixel, else 0
-    __shared__ uint8_t lv0[LCAP];    // initial push level (valid with K_LV)
-    __shared__ uint32_t col[LCAP];
-    __shared__ uint32_t bkey[LCAP];  // (level << 16 | push seq) of the chain's largest-key item
-    __shared__ int32_t bglob[LCAP];  // its global item id
-    __shared__ uint16_t qn[2 * LCAP];  // FIFO links: F item k, leaf of B pixel k at LCAP + k
-    __shared__ uint16_t lfj[LCAP];
+    __shared__ uint32_t bkey[LCAP];  // queued F pixel: (level << 16 | push seq) of its chain's largest-key item
+                                     // (the pixel's colour while the frame is staged)
+    __shared__ uint16_t dif[LCAP];   // push level towards the right (low byte) and lower (high byte) neighbour
+    __shared__ uint16_t bloc[LCAP];  // queued F pixel: frame index of that item; B pixel: its leaf index
+    __shared__ uint16_t qn[LCAP];    // FIFO link of the item at this pixel (an F item, or a B pixel's leaf)
     __shared__ uint16_t qhead[NQ], qtail[NQ];
     __shared__ int sh_base, sh_it;
 
@@ -385,7 +387,6 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         const int gx = ox + kx, gy = oy + ky;
         uint8_t cd = C_FRAME;
         int32_t l0 = 0;
-        uint8_t lv = 0;
         uint32_t cc = 0;
         if (gx >= 0 && gx < W && gy >= 0 && gy < H) {  // the margin may step one pixel off the image
             const int g = gy * W + gx;
@@ -393,19 +394,28 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             cd = c;
             if (c == C_F && D.run[g] == id) cd |= K_INRUN;
             if (c == C_SEED) l0 = D.m0[g];
-            const uint16_t v0 = D.lvl0[g];
-            if (v0 != NOPUSH) { cd |= K_LV; lv = (uint8_t)v0; }
+            if (D.lvl0[g] != NOPUSH) cd |= K_LV;
             const uint8_t* q = D.img + 3 * (size_t)g;
             cc = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16);
         }
         code[k] = cd;
         lk[k] = l0;
-        lv0[k] = lv;
-        col[k] = cc;
+        bkey[k] = cc;
     }
     for (int l = lane; l < NQ; l += 64) { qhead[l] = NIL16; qtail[l] = NIL16; }
     __syncthreads();
-    // leaf candidates (B, never initially queued, 4-adjacent to the run) and initial items in raster order
+    // push levels between 4-neighbours (a pair that wraps around a frame row is never used: pushes
+    // start at run pixels, which lie the margin away from the frame's edge)
+    for (int k = lane; k < L; k += 64) {
+        const uint32_t c = bkey[k];
+        const int rt = k + 1 < L ? cdiff_packed(c, bkey[k + 1]) : 0;
+        const int dn = k + lw < L ? cdiff_packed(c, bkey[k + lw]) : 0;
+        dif[k] = (uint16_t)(rt | (dn << 8));
+    }
+    __syncthreads();
+    // leaf candidates (B, never initially queued, 4-adjacent to the run) and initial items in
+    // raster order: an initial item's records are written here, and the chunk's items of one level
+    // are linked behind that level's FIFO tail (the first of them reads the tail, the last moves it)
     int cnt = 0, ninit = 0;
     for (int k0 = 0; k0 < L; k0 += 64) {
         const int k = k0 + lane;
@@ -418,8 +428,40 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             init = (cd & K_INRUN) && (cd & K_LV);
         }
         const unsigned long long bc = __ballot(cand), bi = __ballot(init);
-        if (init) qn[LCAP + ninit + __popcll(bi & ((1ull << lane) - 1ull))] = (uint16_t)k;
         cnt += __popcll(bc);
+        if (!bi) continue;
+        int lvl = -1, g = 0;
+        if (init) {
+            g = gidx(k);
+            lvl = D.lvl0[g];
+        }
+        int prevk = -1, nextk = -1;  // this chunk's previous and next initial item of the same level
+        for (unsigned long long m = bi; m; m &= m - 1) {
+            const int j = __ffsll((long long)m) - 1;
+            const int lj = __builtin_amdgcn_readlane(lvl, j);
+            if (lj == lvl) {
+                if (j < lane) prevk = k0 + j;
+                else if (j > lane && nextk < 0) nextk = k0 + j;
+            }
+        }
+        if (init) {
+            const int q = ninit + __popcll(bi & ((1ull << lane) - 1ull));
+            bkey[k] = ((uint32_t)lvl << 16) | (uint32_t)q;
+            bloc[k] = (uint16_t)k;
+            code[k] |= K_ST1;
+            qn[k] = nextk < 0 ? NIL16 : (uint16_t)nextk;
+            D.ilvl[g] = (uint8_t)lvl;
+            D.pseq[g] = q;
+            D.parent[g] = -1;
+            D.bnode[g] = g;
+            D.st[g] = 1;
+            if (prevk < 0) {
+                const uint16_t t = qtail[lvl];
+                if (t == NIL16) qhead[lvl] = (uint16_t)k;
+                else qn[t] = (uint16_t)k;
+            }
+            if (nextk < 0) qtail[lvl] = (uint16_t)k;
+        }
         ninit += __popcll(bi);
     }
     if (lane == 0) sh_base = atomicAdd(D.nleaf, cnt);
@@ -431,23 +473,10 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         if (lane == 0) atomicOr(D.flags, FB_LEAF_CAP);
         return;
     }
-    // initial items: records in parallel
-    for (int q = lane; q < ninit; q += 64) {
-        const int k = qn[LCAP + q];
-        const int g = gidx(k);
-        bkey[k] = ((uint32_t)lv0[k] << 16) | (uint32_t)q;
-        bglob[k] = g;
-        code[k] |= K_ST1;
-        D.ilvl[g] = lv0[k];
-        D.pseq[g] = q;
-        D.parent[g] = -1;
-        D.bnode[g] = g;
-        D.st[g] = 1;
-    }
     __syncthreads();
     if (lane != 0) return;
 
-    int active = NQ;
+    int active = 0;  // no queued item below this level
     auto enq = [&](int l, int x) {  // FIFO of level l; the flood resumes at the lowest queued level
         qn[x] = NIL16;
         const uint16_t t = qtail[l];
@@ -456,15 +485,10 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         qtail[l] = (uint16_t)x;
         if (l < active) active = l;
     };
-    for (int q = 0; q < ninit; ++q) {
-        const int k = qn[LCAP + q];
-        enq(lv0[k], k);
-    }
     int seq = ninit, popc = 0, nlf = 0;
     const int ld[4] = {-1, 1, -lw, lw};
     const int gd[4] = {-1, 1, -W, W};
     for (;;) {
-        if (active >= NQ) break;
         int x = qhead[active];
         if (x == NIL16) {
             int l = active + 1;
@@ -474,27 +498,26 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             x = qhead[l];
         }
         const uint16_t nx = qn[x];
+        const uint8_t cpc = code[x];
         qhead[active] = nx;
         if (nx == NIL16) qtail[active] = NIL16;
-        if (x >= LCAP) {  // leaf: record when it pops
-            D.lf_popidx[base + lfj[x - LCAP]] = popc++;
+        if ((cpc & 3) == C_B) {  // a leaf: record when it pops
+            D.lf_popidx[base + bloc[x]] = popc++;
             continue;
         }
         const int p = x;
         // every LDS value this pop reads up front, so the reads overlap
         int nl[4];
         uint8_t nc[4];
-        uint32_t ncol[4];
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             nl[d] = lk[p + ld[d]];
             nc[d] = code[p + ld[d]];
-            ncol[d] = col[p + ld[d]];
         }
-        const uint8_t cpc = code[p];
-        const uint32_t cp = col[p];
+        const uint16_t dleft = dif[p - 1], dhere = dif[p], dup = dif[p - lw];
         const uint32_t bk = bkey[p];
-        const int bg = bglob[p];
+        const int bl = bloc[p];
+        const int lvd[4] = {dleft & 255, dhere & 255, dup >> 8, dhere >> 8};
         const int gp = gidx(p);
         D.popidx[gp] = popc++;
         int lab = 0;
@@ -516,17 +539,17 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             const uint8_t cs = nc[d];
             if ((cs & 3) == C_F) {
                 if (cs & (K_ST1 | K_ST2)) continue;
-                const int l = cdiff_packed(cp, ncol[d]);
+                const int l = lvd[d];
                 const int gs = gp + gd[d];
                 const uint32_t key = ((uint32_t)l << 16) | (uint32_t)seq;
                 const bool newer = bk < key;  // the new item has the larger key on its chain
                 bkey[sk] = newer ? key : bk;
-                bglob[sk] = newer ? gs : bg;
+                bloc[sk] = (uint16_t)(newer ? sk : bl);
                 code[sk] = cs | K_ST1;
                 D.ilvl[gs] = (uint8_t)l;
                 D.pseq[gs] = seq;
                 D.parent[gs] = gp;
-                D.bnode[gs] = newer ? gs : bg;
+                D.bnode[gs] = newer ? gs : gidx(bl);
                 D.st[gs] = 1;
                 ++seq;
                 enq(l, sk);
@@ -546,8 +569,8 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 if (nlf >= cnt) { atomicOr(D.flags, FB_LEAF_CAP); continue; }
                 const int j = nlf++;
                 const int slot = base + j;
-                lfj[sk] = (uint16_t)j;
-                const int l = cdiff_packed(cp, ncol[d]);
+                bloc[sk] = (uint16_t)j;
+                const int l = lvd[d];
                 const uint32_t key = ((uint32_t)l << 16) | (uint32_t)seq;
                 D.lf_b[slot] = gp + gd[d];
                 D.lf_run[slot] = id;
@@ -555,9 +578,9 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 D.lf_pseq[slot] = seq;
                 D.lf_parent[slot] = gp;
                 D.lf_popidx[slot] = -1;
-                D.lf_bnode[slot] = bk < key ? D.N + slot : bg;
+                D.lf_bnode[slot] = bk < key ? D.N + slot : gidx(bl);
                 ++seq;
-                enq(l, LCAP + sk);
+                enq(l, sk);
             }
         }
         if (seq >= 65535) { atomicOr(D.flags, FB_LEAF_CAP); return; }
@@ -565,6 +588,26 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     }();
     __syncthreads();  // the LDS arrays are reused by the next run
     }
+}
+
+// LDS bytes of one k_phase1_lds<LCAP> workgroup (the host sizes each tier's grid from it)
+constexpr int p1_lds_bytes(int lcap) { return 15 * lcap + 4 * NQ + 8; }
+
+// grid of a k_phase1_lds<LCAP> launch: the workgroups the device holds at once (LDS-bound), at most
+// one per run of the image
+template <int LCAP>
+int p1_grid(int nrun) {
+    static const int g = [] {
+        int dev = 0, per_cu = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_phase1_lds<LCAP>, 64, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            return 256 * std::max(1, 160 * 1024 / p1_lds_bytes(LCAP));
+        }
+        return std::max(1, per_cu) * std::max(1, cus);
+    }();
+    return std::max(1, std::min(nrun, g));
 }
 
 // wave-aggregated append of the lanes whose `take` is set
@@ -1576,13 +1619,13 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(std::min(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(4), bcount + 4,
                            next_run + 4);
         hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(5), bcount + 5, qh2, qt);
-        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(std::min(nrun, 256)), dim3(64), 0, ss.side[1], D, box5, nrun, L(3), bcount + 3,
-                           next_run + 3);
-        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(std::min(nrun, 768)), dim3(64), 0, ss.side[2], D, box5, nrun, L(2), bcount + 2,
-                           next_run + 2);
-        hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(std::min(nrun, 1536)), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1,
+        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(p1_grid<4096>(nrun)), dim3(64), 0, ss.side[1], D, box5, nrun, L(3),
+                           bcount + 3, next_run + 3);
+        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(p1_grid<2048>(nrun)), dim3(64), 0, ss.side[2], D, box5, nrun, L(2),
+                           bcount + 2, next_run + 2);
+        hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(p1_grid<1024>(nrun)), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1,
                            next_run + 1);
-        hipLaunchKernelGGL(k_phase1_lds<512>, dim3(std::min(nrun, 3072)), dim3(64), 0, ss.side[2], D, box5, nrun, L(0), bcount,
+        hipLaunchKernelGGL(k_phase1_lds<512>, dim3(p1_grid<512>(nrun)), dim3(64), 0, ss.side[2], D, box5, nrun, L(0), bcount,
                            next_run);
         NSC_CHECK(ss.join(s));
     }

@@ -553,23 +553,15 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 D.st[gs] = 1;
                 ++seq;
                 enq(l, sk);
-            } else if ((cs & 3) == C_B && !(cs & K_LV)) {
-                uint8_t fc[4];
-                int fl[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    fc[e] = code[sk + ld[e]];
-                    fl[e] = lk[sk + ld[e]];
-                }
-                bool done = false;  // queued already by an earlier labelled neighbour of this run
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (sk + ld[e] != p && (fc[e] & (K_INRUN | K_ST2)) == (K_INRUN | K_ST2) && fl[e] > 0) done = true;
-                if (done) continue;
+            } else if ((cs & 3) == C_B && !(cs & (K_LV | K_ST1))) {
+                // K_ST1 on a B pixel: queued already, by an earlier labelled neighbour of this run
+                // (every popped F pixel of the run is labelled or a dam, and a labelled one queues
+                // its B neighbours)
                 if (nlf >= cnt) { atomicOr(D.flags, FB_LEAF_CAP); continue; }
                 const int j = nlf++;
                 const int slot = base + j;
                 bloc[sk] = (uint16_t)j;
+                code[sk] = cs | K_ST1;
                 const int l = lvd[d];
                 const uint32_t key = ((uint32_t)l << 16) | (uint32_t)seq;
                 D.lf_b[slot] = gp + gd[d];

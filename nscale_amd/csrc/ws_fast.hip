@@ -477,19 +477,23 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     if (lane != 0) return;
 
     int active = 0;  // no queued item below this level
+    int hx = qhead[0];  // head of the active level's FIFO (a copy of qhead[active])
     auto enq = [&](int l, int x) {  // FIFO of level l; the flood resumes at the lowest queued level
         qn[x] = NIL16;
         const uint16_t t = qtail[l];
         if (t != NIL16) qn[t] = (uint16_t)x;
         else qhead[l] = (uint16_t)x;
         qtail[l] = (uint16_t)x;
-        if (l < active) active = l;
+        if (t == NIL16 && l <= active) {  // nothing is queued below `active`: l was empty
+            active = l;
+            hx = x;
+        }
     };
     int seq = ninit, popc = 0, nlf = 0;
     const int ld[4] = {-1, 1, -lw, lw};
     const int gd[4] = {-1, 1, -W, W};
     for (;;) {
-        int x = qhead[active];
+        int x = hx;
         if (x == NIL16) {
             int l = active + 1;
             while (l < NQ && qhead[l] == NIL16) ++l;
@@ -497,27 +501,32 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             active = l;
             x = qhead[l];
         }
-        const uint16_t nx = qn[x];
-        const uint8_t cpc = code[x];
-        qhead[active] = nx;
+        // the item's link, class and chain entry and its neighbours' values, in one round trip
+        // (the neighbours of a leaf's B pixel lie inside the frame too, so the reads are safe for
+        // either kind of item); the compiler would otherwise sink the pixel-side reads past the
+        // leaf and dam tests and fetch them in further round trips
+        uint32_t nx = qn[x], cpc = code[x];
+        int nl[4];
+        uint32_t nc[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            nl[d] = lk[x + ld[d]];
+            nc[d] = code[x + ld[d]];
+        }
+        uint32_t dleft = dif[x - 1], dhere = dif[x], dup = dif[x - lw];
+        uint32_t bk = bkey[x];
+        int bl = bloc[x];
+        asm volatile("" : "+v"(nx), "+v"(cpc), "+v"(nl[0]), "+v"(nl[1]), "+v"(nl[2]), "+v"(nl[3]), "+v"(nc[0]),
+                     "+v"(nc[1]), "+v"(nc[2]), "+v"(nc[3]), "+v"(dleft), "+v"(dhere), "+v"(dup), "+v"(bk), "+v"(bl));
+        qhead[active] = (uint16_t)nx;
+        hx = (int)nx;
         if (nx == NIL16) qtail[active] = NIL16;
         if ((cpc & 3) == C_B) {  // a leaf: record when it pops
-            D.lf_popidx[base + bloc[x]] = popc++;
+            D.lf_popidx[base + bl] = popc++;
             continue;
         }
         const int p = x;
-        // every LDS value this pop reads up front, so the reads overlap
-        int nl[4];
-        uint8_t nc[4];
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            nl[d] = lk[p + ld[d]];
-            nc[d] = code[p + ld[d]];
-        }
-        const uint16_t dleft = dif[p - 1], dhere = dif[p], dup = dif[p - lw];
-        const uint32_t bk = bkey[p];
-        const int bl = bloc[p];
-        const int lvd[4] = {dleft & 255, dhere & 255, dup >> 8, dhere >> 8};
+        const int lvd[4] = {(int)(dleft & 255), (int)(dhere & 255), (int)(dup >> 8), (int)(dhere >> 8)};
         const int gp = gidx(p);
         D.popidx[gp] = popc++;
         int lab = 0;

@@ -24,6 +24,7 @@
 #include <atomic>
 #include <climits>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -1592,32 +1593,31 @@ __global__ __launch_bounds__(64, 2) void geo_tiles(const uint4* __restrict__ fla
                     next_count);
 }
 
-// G x G windows per workgroup, shared halos exchanged through LDS after every round (as the
+// GX x GY windows per workgroup, shared halos exchanged through LDS after every round (as the
 // reconstruction regions above, in the (min, +1) algebra: halo refreshes take the min, since
 // distances only fall).
-template <int G>
+template <int GX, int GY>
 struct GeoRegion {
-    unsigned tr[G * G][TH][TW + 1];
-    unsigned init_l[G * G][TH], init_r[G * G][TH];
-    unsigned xtop[G * G][TW], xbot[G * G][TW], xleft[G * G][TH], xright[G * G][TH];
+    unsigned tr[GX * GY][TH][TW + 1];
+    unsigned xtop[GX * GY][TW], xbot[GX * GY][TW], xleft[GX * GY][TH], xright[GX * GY][TH];
     unsigned chg_bits;
 };
 
-// one region of G x G windows: load, solve to stability with halo exchanges through LDS, store,
+// one region of GX x GY windows: load, solve to stability with halo exchanges through LDS, store,
 // mark the neighbouring regions whose halo changed (in the pass lists, or the drain's bitmap)
-template <int G>
+template <int GX, int GY>
 __device__ __forceinline__ void geo_region_solve(const uint4* __restrict__ flags, unsigned* dist, int w, int h, int tiles_x,
-                                                 int tiles_y, int rgx, int rgy, int region, GeoRegion<G>& sh,
+                                                 int tiles_y, int rgx, int rgy, int region, GeoRegion<GX, GY>& sh,
                                                  int* __restrict__ next_list, int* __restrict__ next_count,
                                                  unsigned* __restrict__ stamp, int pass_no, unsigned long long* dirty) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wx = wv % G, wy = wv / G;
+    const int wx = wv % GX, wy = wv / GX;
     auto& tr = sh.tr[wv];
     const int rx = region % rgx, ry = region / rgx;
-    const int tx = rx * G + wx, ty = ry * G + wy;
+    const int tx = rx * GX + wx, ty = ry * GY + wy;
     const bool valid = tx < tiles_x && ty < tiles_y;
     const bool has_l = valid && wx > 0, has_u = valid && wy > 0;
-    const bool has_r = valid && wx < G - 1 && tx + 1 < tiles_x, has_d = valid && wy < G - 1 && ty + 1 < tiles_y;
+    const bool has_r = valid && wx < GX - 1 && tx + 1 < tiles_x, has_d = valid && wy < GY - 1 && ty + 1 < tiles_y;
     if (threadIdx.x == 0) sh.chg_bits = 0;
     unsigned v[TH];
     GeoFlags F;
@@ -1626,11 +1626,11 @@ __device__ __forceinline__ void geo_region_solve(const uint4* __restrict__ flags
         geo_load(flags, dist, w, h, tx, ty, tiles_x, lane, v, F);
         if (lane == 1) {
 #pragma unroll
-            for (int y = 0; y < TH; ++y) { sh.init_l[wv][y] = v[y]; sh.xleft[wv][y] = v[y]; }
+            for (int y = 0; y < TH; ++y) sh.xleft[wv][y] = v[y];
         }
         if (lane == TW - 2) {
 #pragma unroll
-            for (int y = 0; y < TH; ++y) { sh.init_r[wv][y] = v[y]; sh.xright[wv][y] = v[y]; }
+            for (int y = 0; y < TH; ++y) sh.xright[wv][y] = v[y];
         }
         init_top = v[1];
         init_bot = v[TH - 2];
@@ -1646,11 +1646,11 @@ __device__ __forceinline__ void geo_region_solve(const uint4* __restrict__ flags
         bool changed = false;
         if (valid) {
             if (has_u) {
-                const unsigned nb = sh.xbot[wv - G][lane];
+                const unsigned nb = sh.xbot[wv - GX][lane];
                 if (nb < v[0]) v[0] = nb;
             }
             if (has_d) {
-                const unsigned nb = sh.xtop[wv + G][lane];
+                const unsigned nb = sh.xtop[wv + GX][lane];
                 if (nb < v[TH - 1]) v[TH - 1] = nb;
             }
             geo_col_sweeps(v, F);
@@ -1692,20 +1692,15 @@ __device__ __forceinline__ void geo_region_solve(const uint4* __restrict__ flags
     if (valid && __any(lane_owned && own != own0)) geo_store(dist, w, h, tx, ty, lane, v);
     unsigned bits = 0;
     if (valid) {
-        const bool edge_l = wx == 0, edge_r = wx == G - 1 || tx == tiles_x - 1;
-        const bool edge_u = wy == 0, edge_d = wy == G - 1 || ty == tiles_y - 1;
+        const bool edge_l = wx == 0, edge_r = wx == GX - 1 || tx == tiles_x - 1;
+        const bool edge_u = wy == 0, edge_d = wy == GY - 1 || ty == tiles_y - 1;
         if (lane >= 1 && lane <= TW - 2) {
             if (edge_u && v[1] != init_top) bits |= CH_TOP;
             if (edge_d && v[TH - 2] != init_bot) bits |= CH_BOT;
         }
-        if (lane == 1 && edge_l) {
-            for (int y = 1; y < TH - 1; ++y)
-                if (v[y] != sh.init_l[wv][y]) bits |= CH_LEFT;
-        }
-        if (lane == TW - 2 && edge_r) {
-            for (int y = 1; y < TH - 1; ++y)
-                if (v[y] != sh.init_r[wv][y]) bits |= CH_RIGHT;
-        }
+        // an owned edge column changed iff its key sum rose (distances only fall)
+        if (lane == 1 && edge_l && own != own0) bits |= CH_LEFT;
+        if (lane == TW - 2 && edge_r && own != own0) bits |= CH_RIGHT;
     }
     if (dirty) __threadfence();  // the region's pixels are visible before any neighbour is marked
     if (bits) atomicOr(&sh.chg_bits, bits);
@@ -1719,31 +1714,31 @@ __device__ __forceinline__ void geo_region_solve(const uint4* __restrict__ flags
     __syncthreads();
 }
 
-template <int G>
-__global__ __launch_bounds__(64 * G * G) void geo_regions(const uint4* __restrict__ flags, unsigned* dist, int w, int h,
+template <int GX, int GY>
+__global__ __launch_bounds__(64 * GX * GY) void geo_regions(const uint4* __restrict__ flags, unsigned* dist, int w, int h,
                                                           int tiles_x, int tiles_y, int rgx, int rgy,
                                                           const int* __restrict__ list, const int* __restrict__ list_count,
                                                           int* __restrict__ next_list, int* __restrict__ next_count,
                                                           int* __restrict__ clear_count, unsigned* __restrict__ stamp,
                                                           int pass_no) {
-    __shared__ GeoRegion<G> sh;
+    __shared__ GeoRegion<GX, GY> sh;
     if (blockIdx.x == 0 && threadIdx.x == 0) *clear_count = 0;  // the counter of pass + 2 (ring)
     const int n = pass_no == 0 ? rgx * rgy : *list_count;
     for (int k = blockIdx.x; k < n; k += gridDim.x) {
         const int region = pass_no == 0 ? k : list[k];
-        geo_region_solve<G>(flags, dist, w, h, tiles_x, tiles_y, rgx, rgy, region, sh, next_list, next_count, stamp,
+        geo_region_solve<GX, GY>(flags, dist, w, h, tiles_x, tiles_y, rgx, rgy, region, sh, next_list, next_count, stamp,
                             pass_no, nullptr);
     }
 }
 
 // The geodesic distance as a device-side work queue (as recon_drain): every region starts dirty;
 // the first wave of the workgroup claims a region and the workgroup solves it.
-template <int G>
-__global__ __launch_bounds__(64 * G * G) void geo_drain(const uint4* __restrict__ flags, unsigned* dist, int w, int h,
+template <int GX, int GY>
+__global__ __launch_bounds__(64 * GX * GY) void geo_drain(const uint4* __restrict__ flags, unsigned* dist, int w, int h,
                                                         int tiles_x, int tiles_y, int rgx, int rgy, int nwords,
                                                         unsigned long long* dirty, unsigned long long* busy, int* running,
                                                         int spin_limit) {
-    __shared__ GeoRegion<G> sh;
+    __shared__ GeoRegion<GX, GY> sh;
     __shared__ int s_got;
     int idle = 0;
     const int start = (int)((blockIdx.x * 97u) % (unsigned)nwords);
@@ -1768,7 +1763,7 @@ __global__ __launch_bounds__(64 * G * G) void geo_drain(const uint4* __restrict_
             continue;
         }
         __threadfence();  // acquire: the neighbours' latest distances, not stale L1 lines
-        geo_region_solve<G>(flags, dist, w, h, tiles_x, tiles_y, rgx, rgy, got, sh, nullptr, nullptr, nullptr, 1, dirty);
+        geo_region_solve<GX, GY>(flags, dist, w, h, tiles_x, tiles_y, rgx, rgy, got, sh, nullptr, nullptr, nullptr, 1, dirty);
         if (threadIdx.x == 0) drain_release(got, busy, running);
     }
 }
@@ -1785,12 +1780,16 @@ __global__ void k_set_bits(unsigned long long* bits, int n) {
 }  // namespace
 
 int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t s) {
-    static int G = [] {
+    // region shape in windows: 2 x 2, or single windows with NSC_GEO_REGION=1 (4 x 2 and 2 x 4
+    // regions were measured slower on the 4096^2 chain tile: 3.2-3.4 ms against 2.7 ms, the extra
+    // rounds a region needs to carry its halos across more windows outweigh the fewer passes)
+    static const int GXY = [] {
         const char* e = getenv("NSC_GEO_REGION");
-        return e ? std::max(1, std::min(atoi(e), 2)) : 2;
+        return e && !strcmp(e, "1") ? 11 : 22;
     }();
+    const int GX = GXY / 10, GY = GXY % 10;
     const int tiles_x = (int)cdiv(w, OW), tiles_y = (int)cdiv(h, OH);
-    const int rgx = (int)cdiv(tiles_x, G), rgy = (int)cdiv(tiles_y, G);
+    const int rgx = (int)cdiv(tiles_x, GX), rgy = (int)cdiv(tiles_y, GY);
     const int nt = rgx * rgy;
     const long long max_pass = pass_limit((long long)w * h);
     const int nwin = tiles_x * tiles_y;
@@ -1804,7 +1803,7 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
     int* listA = B.listA;
     int* listB = B.listB;
     hipLaunchKernelGGL(geo_flags, dim3(nwin), dim3(TW), 0, s, pass, flags, w, h, tiles_x);
-    if (G == 2 && drain_env()) {
+    if (GXY == 22 && drain_env()) {
         // every region dirty, one drain launch, then the bitmap must be empty
         const int nw = (int)recon_words(nt);
         hipLaunchKernelGGL(k_set_bits, dim3(std::min<unsigned>(cdiv(nw, 256), 256u)), dim3(256), 0, s, B.dirty, nt);
@@ -1812,13 +1811,13 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
             int dev = 0, per_cu = 0, cus = 0;
             if (hipGetDevice(&dev) != hipSuccess ||
                 hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, geo_drain<2>, 256, 0) != hipSuccess) {
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, geo_drain<2, 2>, 256, 0) != hipSuccess) {
                 (void)hipGetLastError();
                 return 512;
             }
             return std::max(1, per_cu) * std::max(1, cus);
         }();
-        hipLaunchKernelGGL(geo_drain<2>, dim3(std::min(nt, grid_d)), dim3(256), 0, s, flags, dist, w, h, tiles_x, tiles_y,
+        hipLaunchKernelGGL((geo_drain<2, 2>), dim3(std::min(nt, grid_d)), dim3(256), 0, s, flags, dist, w, h, tiles_x, tiles_y,
                            rgx, rgy, nw, B.dirty, B.busy, &B.flags[0], 4096);
         hipLaunchKernelGGL(k_any_bits, dim3(std::min<unsigned>(cdiv(nw, 256), 256u)), dim3(256), 0, s, B.dirty, nw,
                            &B.flags[1]);
@@ -1832,9 +1831,9 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
     }
     const unsigned grid = (unsigned)std::min(nt, 1 << 16);
     auto launch = [&](int p, const int* lst, const int* cnt, int* nl, int* nc, int* clr) {
-        if (G == 2)
-            hipLaunchKernelGGL(geo_regions<2>, dim3(grid), dim3(256), 0, s, flags, dist, w, h, tiles_x, tiles_y, rgx, rgy, lst,
-                               cnt, nl, nc, clr, stamp, p);
+        if (GXY == 22)
+            hipLaunchKernelGGL((geo_regions<2, 2>), dim3(grid), dim3(256), 0, s, flags, dist, w, h, tiles_x, tiles_y, rgx, rgy,
+                               lst, cnt, nl, nc, clr, stamp, p);
         else
             hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, flags, dist, w, h, tiles_x, tiles_y, lst, cnt, nl, nc,
                                clr, stamp, p);

@@ -912,8 +912,11 @@ static int recon_passes(const RImg<T>* imgs_dev, const RImg<T>& one, const int* 
     launch(0, nullptr, nullptr, B.listA, &counts[0], &counts[1]);
     NSC_HIP(hipGetLastError());
     long long pass = 1;
-    for (;;) {
-        const int batch = std::max(1, tune<T>().batch);
+    // passes between host checks: the schedule's batch b, then 2b, then 4b from there on (a host
+    // round trip costs about as much as ten empty passes, and calls that run past 3b passes
+    // usually run far past it)
+    int batch = std::max(1, tune<T>().batch);
+    for (int check = 0;; ++check, batch = check <= 2 ? 2 * batch : batch) {
         for (int b = 0; b < batch && pass <= max_pass; ++b, ++pass) {
             int* cur = (pass & 1) ? B.listA : B.listB;
             int* nxt = (pass & 1) ? B.listB : B.listA;

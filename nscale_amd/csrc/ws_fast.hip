@@ -318,6 +318,15 @@ __global__ void k_phase1(Dev D, const int32_t* __restrict__ box5, int nrun, cons
     }
 }
 
+// Store to element i of a per-pixel or per-leaf array through a 32-bit byte offset (the
+// decomposition runs on images below 2^28 pixels, so every offset fits): with a uniform base the
+// compiler uses the scalar-base form of the vector store instead of forming a 64-bit address in
+// vector registers for each store.
+template <typename T, typename V>
+__device__ __forceinline__ void gst(T* base, uint32_t i, V v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(i * (uint32_t)sizeof(T))) = (T)v;
+}
+
 // ---- phase 1 with the run's neighbourhood staged in LDS ----
 // Same flood as k_phase1, for runs whose bounding box plus a 2-pixel margin has at most LCAP
 // pixels.  The wave stages the frame into LDS (class, in-run flag, seed labels, and the push level
@@ -450,11 +459,11 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             bloc[k] = (uint16_t)k;
             code[k] |= K_ST1;
             qn[k] = nextk < 0 ? NIL16 : (uint16_t)nextk;
-            D.ilvl[g] = (uint8_t)lvl;
-            D.pseq[g] = q;
-            D.parent[g] = -1;
-            D.bnode[g] = g;
-            D.st[g] = 1;
+            gst(D.ilvl, g, (uint8_t)lvl);
+            gst(D.pseq, g, q);
+            gst(D.parent, g, -1);
+            gst(D.bnode, g, g);
+            gst(D.st, g, 1);
             if (prevk < 0) {
                 const uint16_t t = qtail[lvl];
                 if (t == NIL16) qhead[lvl] = (uint16_t)k;
@@ -468,7 +477,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     __syncthreads();
     const int base = sh_base;
     // every reserved slot below the capacity reads as unused until a leaf fills it
-    for (int j = base + lane; j < min(base + cnt, D.leafcap); j += 64) D.lf_b[j] = -1;
+    for (int j = base + lane; j < min(base + cnt, D.leafcap); j += 64) gst(D.lf_b, j, -1);
     if (base + cnt > D.leafcap) {
         if (lane == 0) atomicOr(D.flags, FB_LEAF_CAP);
         return;
@@ -522,13 +531,13 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         hx = (int)nx;
         if (nx == NIL16) qtail[active] = NIL16;
         if ((cpc & 3) == C_B) {  // a leaf: record when it pops
-            D.lf_popidx[base + bl] = popc++;
+            gst(D.lf_popidx, base + bl, popc++);
             continue;
         }
         const int p = x;
         const int lvd[4] = {(int)(dleft & 255), (int)(dhere & 255), (int)(dup >> 8), (int)(dhere >> 8)};
         const int gp = gidx(p);
-        D.popidx[gp] = popc++;
+        gst(D.popidx, gp, popc++);
         int lab = 0;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
@@ -537,8 +546,8 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         }
         lk[p] = lab;
         code[p] = (uint8_t)((cpc & ~K_ST1) | K_ST2);
-        D.lab[gp] = lab;
-        D.st[gp] = 2;
+        gst(D.lab, gp, lab);
+        gst(D.st, gp, 2);
         if (lab == WSHED) continue;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
@@ -555,11 +564,11 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 bkey[sk] = newer ? key : bk;
                 bloc[sk] = (uint16_t)(newer ? sk : bl);
                 code[sk] = cs | K_ST1;
-                D.ilvl[gs] = (uint8_t)l;
-                D.pseq[gs] = seq;
-                D.parent[gs] = gp;
-                D.bnode[gs] = newer ? gs : gidx(bl);
-                D.st[gs] = 1;
+                gst(D.ilvl, gs, (uint8_t)l);
+                gst(D.pseq, gs, seq);
+                gst(D.parent, gs, gp);
+                gst(D.bnode, gs, newer ? gs : gidx(bl));
+                gst(D.st, gs, 1);
                 ++seq;
                 enq(l, sk);
             } else if ((cs & 3) == C_B && !(cs & (K_LV | K_ST1))) {
@@ -573,13 +582,13 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 code[sk] = cs | K_ST1;
                 const int l = lvd[d];
                 const uint32_t key = ((uint32_t)l << 16) | (uint32_t)seq;
-                D.lf_b[slot] = gp + gd[d];
-                D.lf_run[slot] = id;
-                D.lf_lvl[slot] = (uint8_t)l;
-                D.lf_pseq[slot] = seq;
-                D.lf_parent[slot] = gp;
-                D.lf_popidx[slot] = -1;
-                D.lf_bnode[slot] = bk < key ? D.N + slot : gidx(bl);
+                gst(D.lf_b, slot, gp + gd[d]);
+                gst(D.lf_run, slot, id);
+                gst(D.lf_lvl, slot, (uint8_t)l);
+                gst(D.lf_pseq, slot, seq);
+                gst(D.lf_parent, slot, gp);
+                gst(D.lf_popidx, slot, -1);
+                gst(D.lf_bnode, slot, bk < key ? D.N + slot : gidx(bl));
                 ++seq;
                 enq(l, sk);
             }

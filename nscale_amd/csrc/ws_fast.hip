@@ -27,6 +27,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #include <hipcub/hipcub.hpp>
 
@@ -1400,42 +1401,61 @@ __global__ __launch_bounds__(64) void k_flood_lds(Dev D, P2 Q, const int32_t* __
             }
         }
         int active = NQ;
+        int hx = NIL16;  // head of the active level's FIFO (a copy of qhead[active])
         auto enq = [&](int l, int x) {
             qn[x] = NIL16;
             const uint16_t t = qtail[l];
             if (t != NIL16) qn[t] = (uint16_t)x;
             else qhead[l] = (uint16_t)x;
             qtail[l] = (uint16_t)x;
-            if (l < active) active = l;
+            if (t == NIL16 && l <= active) {  // nothing is queued below `active`: l was empty
+                active = l;
+                hx = x;
+            }
         };
         for (int q = 0; q < ni; ++q) enq(olv[q], ord[q]);
         const int ld[4] = {-1, 1, -lw, lw};
         for (;;) {
-            if (active >= NQ) break;
-            if (qhead[active] == NIL16) {
+            int p = hx;
+            if (p == NIL16) {
                 int l = active + 1;
                 while (l < NQ && qhead[l] == NIL16) ++l;
-                if (l == NQ) break;
+                if (l >= NQ) break;
                 active = l;
+                p = qhead[l];
             }
-            const int p = qhead[active];
-            const uint16_t nx = qn[p];
-            qhead[active] = nx;
+            // everything the pop reads, in one round trip (this pop's own writes go to p and to
+            // distinct neighbours, so the values stay current for its pushes)
+            uint32_t nx = qn[p], cp = col[p];
+            int ns[4];
+            uint32_t ninc[4], ncol[4];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                ns[d] = st[p + ld[d]];
+                ninc[d] = inc[p + ld[d]];
+                ncol[d] = col[p + ld[d]];
+            }
+            asm volatile("" : "+v"(nx), "+v"(cp), "+v"(ns[0]), "+v"(ns[1]), "+v"(ns[2]), "+v"(ns[3]), "+v"(ninc[0]),
+                         "+v"(ninc[1]), "+v"(ninc[2]), "+v"(ninc[3]), "+v"(ncol[0]), "+v"(ncol[1]), "+v"(ncol[2]),
+                         "+v"(ncol[3]));
+            qhead[active] = (uint16_t)nx;
+            hx = (int)nx;
             if (nx == NIL16) qtail[active] = NIL16;
             int lab = 0;
+#pragma unroll
             for (int d = 0; d < 4; ++d) {
-                const int t = st[p + ld[d]];
+                const int t = ns[d];
                 if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
             }
             st[p] = lab;
             if (lab == WSHED) continue;
-            const uint32_t cp = col[p];
+#pragma unroll
             for (int d = 0; d < 4; ++d) {
                 const int s2 = p + ld[d];
-                if (st[s2] != 0) continue;
-                if (!inc[s2]) { bad = 1; continue; }  // an unresolved pixel outside the component
+                if (ns[d] != 0) continue;
+                if (!ninc[d]) { bad = 1; continue; }  // an unresolved pixel outside the component
                 st[s2] = IN_QUEUE;
-                enq(cdiff_packed(cp, col[s2]), s2);
+                enq(cdiff_packed(cp, ncol[d]), s2);
             }
         }
         if (bad) atomicOr(D.flags, FB_HEAP);
@@ -1703,6 +1723,21 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         Carver cut3(sc3.p);
         carve3(cut3, b5, area, off, scr, hk);
         NSC_CHECK(bbox_impl(lab2, W, H, ncomp, b5, area, s));
+        if (dbg) {  // phase-2 components by where they flood (tie-free ones; a tie moves a small one to LDS)
+            std::vector<int32_t> hb(5 * (size_t)ncomp), ha(ncomp);
+            (void)hipMemcpy(hb.data(), b5, sizeof(int32_t) * hb.size(), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(ha.data(), area, sizeof(int32_t) * ha.size(), hipMemcpyDeviceToHost);
+            long long nbig = 0, maxa_big = 0, maxl = 0, nsmall = 0, nlds = 0;
+            for (int c = 0; c < ncomp; ++c) {
+                const long long L = (long long)(hb[2 * ncomp + c] - hb[ncomp + c] + 3) * (hb[4 * ncomp + c] - hb[3 * ncomp + c] + 3);
+                maxl = std::max(maxl, L);
+                if (L > FL_BIG) { ++nbig; maxa_big = std::max<long long>(maxa_big, ha[c]); }
+                else if (ha[c] > FL_MIN_AREA) ++nlds;
+                else ++nsmall;
+            }
+            fprintf(stderr, "[ws] p2 comps %d: lane-flooded %lld, LDS %lld, global %lld (max area %lld), max frame %lld\n",
+                    ncomp, nsmall, nlds, nbig, maxa_big, maxl);
+        }
         {  // offsets of the components' scratch: exclusive scan of their areas
             size_t tb = 0;
             NSC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, area, off, ncomp, s));

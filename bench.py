@@ -31,9 +31,21 @@ sys.path.insert(0, ROOT)
 # process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default), where kernels of
 # unrelated streams wait behind each other: with 4 queues a kernel trace of the served step never
 # shows more than 4 kernels running at once.  16 queues measured 5% more served throughput than 8
-# and 8 about 7% more than 4 (the plain path is unchanged).  Must be set before the HIP runtime
-# starts (before torch touches the GPU).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# and 8 about 7% more than 4 (the plain path is unchanged).  --hw-queues sets it (0 keeps the
+# environment's value); it must be set before the HIP runtime starts (before torch touches the GPU),
+# so it is read from the command line here, ahead of the argument parser.
+def _hw_queues_arg(argv):
+    for i, a in enumerate(argv):
+        if a == "--hw-queues" and i + 1 < len(argv):
+            return int(argv[i + 1])
+        if a.startswith("--hw-queues="):
+            return int(a.split("=", 1)[1])
+    return 16
+
+
+_HWQ = _hw_queues_arg(sys.argv[1:])
+if _HWQ > 0:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(_HWQ, 32))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
@@ -241,7 +253,7 @@ def bench_serve(args):
                       "dtype": "u8", "data": "synthetic", "bit_exact_vs_plain": mismatches == 0,
                       "config": {"workload": "served chain, tile sides log-uniform [512,4096] x aspect U[1,2]",
                                  "slots": args.slots, "budget_gb": args.budget_gb, "distinct_tiles": len(tiles),
-                                 "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])},
+                                 "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
                       "levels": levels}), flush=True)
     if mismatches:
         sys.exit(1)
@@ -379,7 +391,7 @@ def bench_slide(args):
            "data": "synthetic (procedural slide)",
            "config": {"workload": "100000x80000 slide, 4096 tiles, 64-px overlap, served with canonical labels, area, "
                                   "bbox, stitched", "tiles": len(tiles), "slots": args.slots,
-                                  "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])},
+                                  "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
            "seconds": round(wall, 2), "nuclei_in_tiles": nuclei, "nuclei_stitched": stitch.next_id - 1,
            "stitched_nuclei_touching_a_seam_edge": stitch.truncated, "errors": errors,
            "watershed_sequential_fallbacks": gpu.watershed_stats()[1] - ws0[1],
@@ -498,6 +510,8 @@ def main():
     ap.add_argument("--budget-gb", type=int, default=200, help="server HBM budget")
     ap.add_argument("--rates", default="10,50,100,200,500,1000,2000", help="serve: arrival rates, req/s")
     ap.add_argument("--requests", type=int, default=500, help="serve: requests per rate")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (0 = keep the environment's value)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the segmentation path has no multi-GPU component in this build "
@@ -565,7 +579,7 @@ def main():
                                 "through the continuous-batching server (%d slots)" % (S, S, B, args.slots)) if B else
                                "segmentNuclei chain, one %dx%d BGR uint8 synthetic H&E tile per call (plain path)" % (S, S),
                    "tile": [S, S], "batch": B, "slots": args.slots, "status": st, "nuclei": cc,
-                   "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])},
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "plain_mpix_s": round(plain_mpix, 3), "served_vs_plain": round(served_mpix / plain_mpix, 3),
         "bit_exact_served_vs_plain": mismatches == 0,
         "p50_ms": round(pct(slat or lat, 50), 3), "p99_ms": round(pct(slat or lat, 99), 3),

@@ -14,7 +14,8 @@
 //        out = min(max(out, up-left, up, up-right), mask); the diagonal neighbours come from
 //        the adjacent lanes by full-wave DPP shifts;
 //      - row state (lane = row, after a transpose through LDS): left-right and right-left
-//        sweeps, out = min(max(out, left), mask).
+//        sweeps, out = min(max(out, left, up-left, down-left), mask) (the diagonal neighbours only
+//        with 8-connectivity; right instead of left for the right-left sweep).
 //    A fixpoint of these four sweeps satisfies every 8- (or 4-) neighbour constraint.
 //  * Pass 0 runs every tile once, reading marker and mask (halo = min(marker, mask)) and writing
 //    the output: 3 bytes/px for uint8, the algorithmic minimum.  A tile whose owned border
@@ -232,10 +233,14 @@ __device__ __forceinline__ void sweep_round(typename RegOf<T>::type (&v)[TH], Wi
     for (int x = 0; x < TW; ++x) v[x] = (R)tr[lane][x];
     M.opaque();
     {
+        // with 8-connectivity the row sweeps take the diagonal neighbours too (the rows above and
+        // below are the adjacent lanes), so a path that wanders by one row at a time while it runs
+        // along a row is carried in the same sweep instead of waiting for the next round
         R prev = v[0];
 #pragma unroll
         for (int x = 1; x < TW - 1; ++x) {
-            const R nv = M.clamp_row(v[x], prev, x);
+            const R lt = (CONN == 8) ? nbr3(prev) : prev;
+            const R nv = M.clamp_row(v[x], lt, x);
             v[x] = nv;
             prev = nv;
             M.fence(x);
@@ -244,7 +249,8 @@ __device__ __forceinline__ void sweep_round(typename RegOf<T>::type (&v)[TH], Wi
         prev = v[TW - 1];
 #pragma unroll
         for (int x = TW - 2; x >= 1; --x) {
-            const R nv = M.clamp_row(v[x], prev, x);
+            const R rt = (CONN == 8) ? nbr3(prev) : prev;
+            const R nv = M.clamp_row(v[x], rt, x);
             v[x] = nv;
             prev = nv;
             M.fence(x);
@@ -680,23 +686,33 @@ __device__ __forceinline__ void recon_region(const RImg<T>& im, int region, int 
                 const R n = (R)sh.xleft[wv + 1][lane];
                 if (v[TW - 1] < n) { v[TW - 1] = n; lm[lane][TW - 1] = (T)n; changed = true; }
             }
+            // row sweeps, with the diagonal neighbours (adjacent lanes) under 8-connectivity
             R prev = v[0];
+            R ql = from_left(prev, prev), qr = from_right(prev, prev);
 #pragma unroll
             for (int x = 1; x < TW - 1; ++x) {
+                R lt = prev;
+                if (CONN == 8) lt = smax(lt, smax(ql, qr));
                 const R mk = (R)lm[lane][x];
-                const R nv = smin(smax(v[x], prev), mk);
+                const R nv = smin(smax(v[x], lt), mk);
                 swept |= (nv != v[x]);
                 v[x] = nv;
                 prev = nv;
+                if (CONN == 8) { ql = from_left(prev, prev); qr = from_right(prev, prev); }
             }
             prev = v[TW - 1];
+            ql = from_left(prev, prev);
+            qr = from_right(prev, prev);
 #pragma unroll
             for (int x = TW - 2; x >= 1; --x) {
+                R rt = prev;
+                if (CONN == 8) rt = smax(rt, smax(ql, qr));
                 const R mk = (R)lm[lane][x];
-                const R nv = smin(smax(v[x], prev), mk);
+                const R nv = smin(smax(v[x], rt), mk);
                 swept |= (nv != v[x]);
                 v[x] = nv;
                 prev = nv;
+                if (CONN == 8) { ql = from_left(prev, prev); qr = from_right(prev, prev); }
             }
             sh.xleft[wv][lane] = (T)v[1];
             sh.xright[wv][lane] = (T)v[TW - 2];

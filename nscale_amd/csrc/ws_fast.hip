@@ -1727,16 +1727,18 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
             std::vector<int32_t> hb(5 * (size_t)ncomp), ha(ncomp);
             (void)hipMemcpy(hb.data(), b5, sizeof(int32_t) * hb.size(), hipMemcpyDeviceToHost);
             (void)hipMemcpy(ha.data(), area, sizeof(int32_t) * ha.size(), hipMemcpyDeviceToHost);
-            long long nbig = 0, maxa_big = 0, maxl = 0, nsmall = 0, nlds = 0;
+            long long nbig = 0, maxa_big = 0, maxl = 0, nsmall = 0, nlds = 0, n1 = 0, n2 = 0;
             for (int c = 0; c < ncomp; ++c) {
+                n1 += ha[c] == 1;
+                n2 += ha[c] == 2;
                 const long long L = (long long)(hb[2 * ncomp + c] - hb[ncomp + c] + 3) * (hb[4 * ncomp + c] - hb[3 * ncomp + c] + 3);
                 maxl = std::max(maxl, L);
                 if (L > FL_BIG) { ++nbig; maxa_big = std::max<long long>(maxa_big, ha[c]); }
                 else if (ha[c] > FL_MIN_AREA) ++nlds;
                 else ++nsmall;
             }
-            fprintf(stderr, "[ws] p2 comps %d: lane-flooded %lld, LDS %lld, global %lld (max area %lld), max frame %lld\n",
-                    ncomp, nsmall, nlds, nbig, maxa_big, maxl);
+            fprintf(stderr, "[ws] p2 comps %d: lane-flooded %lld (area 1: %lld, 2: %lld), LDS %lld, global %lld (max area %lld), max frame %lld\n",
+                    ncomp, nsmall, n1, n2, nlds, nbig, maxa_big, maxl);
         }
         {  // offsets of the components' scratch: exclusive scan of their areas
             size_t tb = 0;

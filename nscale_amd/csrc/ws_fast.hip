@@ -84,6 +84,16 @@ struct Dev {
     uint8_t* wl;
 };
 
+// The label a popped pixel takes from its four neighbours (cv::watershed): the common positive
+// label, WSHED where two positive labels differ, 0 where none is positive.  Branch-free: the
+// largest label, and the smallest positive one found as the unsigned minimum of t - 1 (0, WSHED and
+// IN_QUEUE wrap to the top).
+__device__ __forceinline__ int pop_label(int a, int b, int c, int d) {
+    const int mx = max(max(a, b), max(c, d));
+    const unsigned mn1 = min(min((unsigned)(a - 1), (unsigned)(b - 1)), min((unsigned)(c - 1), (unsigned)(d - 1)));
+    return mx <= 0 ? 0 : ((int)mn1 + 1 == mx ? mx : WSHED);
+}
+
 __device__ __forceinline__ int c_diff(const uint8_t* a, const uint8_t* b) {
     const int db = abs((int)a[0] - (int)b[0]);
     const int dg = abs((int)a[1] - (int)b[1]);
@@ -539,12 +549,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         const int lvd[4] = {(int)(dleft & 255), (int)(dhere & 255), (int)(dup >> 8), (int)(dhere >> 8)};
         const int gp = gidx(p);
         gst(D.popidx, gp, popc++);
-        int lab = 0;
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            const int t = nl[d];
-            if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
-        }
+        const int lab = pop_label(nl[0], nl[1], nl[2], nl[3]);
         lk[p] = lab;
         code[p] = (uint8_t)((cpc & ~K_ST1) | K_ST2);
         gst(D.lab, gp, lab);
@@ -1441,12 +1446,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(Dev D, P2 Q, const int32_t* __
             qhead[active] = (uint16_t)nx;
             hx = (int)nx;
             if (nx == NIL16) qtail[active] = NIL16;
-            int lab = 0;
-#pragma unroll
-            for (int d = 0; d < 4; ++d) {
-                const int t = ns[d];
-                if (t > 0) { if (lab == 0) lab = t; else if (t != lab) lab = WSHED; }
-            }
+            const int lab = pop_label(ns[0], ns[1], ns[2], ns[3]);
             st[p] = lab;
             if (lab == WSHED) continue;
 #pragma unroll

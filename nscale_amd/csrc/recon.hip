@@ -195,34 +195,30 @@ struct WinMask<uint8_t> {
     static __device__ __forceinline__ void fence(int) {}
 };
 
-// one round of the four direction sweeps over a window held in v (column state on entry and exit);
-// returns whether any value changed (uint8: decided by the caller from column sums)
+// one round of the four direction sweeps over a window held in v (column state on entry and exit).
+// The down and up sweeps run as two interleaved chains, each updating v in place: step k moves the
+// down chain to row k and the up chain to row 63 - k, so the two dependency chains overlap in the
+// pipeline instead of one waiting on the other.  Every update is min(max(v, neighbours), mask) of
+// current values, a valid relaxation whatever the other chain has written, and past the middle each
+// chain reads what the other has already raised.  Rows likewise with the left and right sweeps.
+// (uint8: whether anything changed is decided by the caller from column sums)
 template <typename T, int CONN>
 __device__ __forceinline__ void sweep_round(typename RegOf<T>::type (&v)[TH], WinMask<T>& M, T (*tr)[TW + 1],
                                             int lane) {
     using R = typename RegOf<T>::type;
     M.opaque();
     {
-        R pc = v[0];
+        R pd = v[0], pu = v[TH - 1];
 #pragma unroll
-        for (int y = 1; y < TH - 1; ++y) {
-            const R up = (CONN == 8) ? nbr3(pc) : pc;
-            const R nv = M.clamp_col(v[y], up, y);
-            v[y] = nv;
-            pc = nv;
-            M.fence(y);
-        }
-    }
-    {
-        M.opaque();  // no sharing of unpacked masks between the two directions (register pressure)
-        R pc = v[TH - 1];
-#pragma unroll
-        for (int y = TH - 2; y >= 1; --y) {
-            const R dn = (CONN == 8) ? nbr3(pc) : pc;
-            const R nv = M.clamp_col(v[y], dn, y);
-            v[y] = nv;
-            pc = nv;
-            M.fence(y);
+        for (int k = 1; k < TH - 1; ++k) {
+            const int yd = k, yu = TH - 1 - k;
+            const R nd = M.clamp_col(v[yd], (CONN == 8) ? nbr3(pd) : pd, yd);
+            v[yd] = nd;
+            pd = nd;
+            const R nu = M.clamp_col(v[yu], (CONN == 8) ? nbr3(pu) : pu, yu);
+            v[yu] = nu;
+            pu = nu;
+            M.fence(yd);
         }
     }
     // ---- transpose to row state (lane = row) ----
@@ -236,24 +232,17 @@ __device__ __forceinline__ void sweep_round(typename RegOf<T>::type (&v)[TH], Wi
         // with 8-connectivity the row sweeps take the diagonal neighbours too (the rows above and
         // below are the adjacent lanes), so a path that wanders by one row at a time while it runs
         // along a row is carried in the same sweep instead of waiting for the next round
-        R prev = v[0];
+        R pl = v[0], pr = v[TW - 1];
 #pragma unroll
-        for (int x = 1; x < TW - 1; ++x) {
-            const R lt = (CONN == 8) ? nbr3(prev) : prev;
-            const R nv = M.clamp_row(v[x], lt, x);
-            v[x] = nv;
-            prev = nv;
-            M.fence(x);
-        }
-        M.opaque();
-        prev = v[TW - 1];
-#pragma unroll
-        for (int x = TW - 2; x >= 1; --x) {
-            const R rt = (CONN == 8) ? nbr3(prev) : prev;
-            const R nv = M.clamp_row(v[x], rt, x);
-            v[x] = nv;
-            prev = nv;
-            M.fence(x);
+        for (int k = 1; k < TW - 1; ++k) {
+            const int xl = k, xr = TW - 1 - k;
+            const R nl = M.clamp_row(v[xl], (CONN == 8) ? nbr3(pl) : pl, xl);
+            v[xl] = nl;
+            pl = nl;
+            const R nr = M.clamp_row(v[xr], (CONN == 8) ? nbr3(pr) : pr, xr);
+            v[xr] = nr;
+            pr = nr;
+            M.fence(xl);
         }
     }
 #pragma unroll

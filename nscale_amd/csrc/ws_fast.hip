@@ -84,6 +84,11 @@ struct Dev {
     uint8_t* wl;
 };
 
+// A value every lane of the wave holds (or the one active lane's), as a wave-uniform value: the
+// compiler then keeps it in a scalar register and branches on it with scalar branches.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // The label a popped pixel takes from its four neighbours (cv::watershed): the common positive
 // label, WSHED where two positive labels differ, 0 where none is positive.  Branch-free: the
 // largest label, and the smallest positive one found as the unsigned minimum of t - 1 (0, WSHED and
@@ -500,7 +505,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
     int hx = qhead[0];  // head of the active level's FIFO (a copy of qhead[active])
     auto enq = [&](int l, int x) {  // FIFO of level l; the flood resumes at the lowest queued level
         qn[x] = NIL16;
-        const uint16_t t = qtail[l];
+        const uint32_t t = uni(qtail[l]);
         if (t != NIL16) qn[t] = (uint16_t)x;
         else qhead[l] = (uint16_t)x;
         qtail[l] = (uint16_t)x;
@@ -516,10 +521,10 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         int x = hx;
         if (x == NIL16) {
             int l = active + 1;
-            while (l < NQ && qhead[l] == NIL16) ++l;
+            while (l < NQ && uni(qhead[l]) == NIL16) ++l;
             if (l == NQ) break;
             active = l;
-            x = qhead[l];
+            x = uni(qhead[l]);
         }
         // the item's link, class and chain entry and its neighbours' values, in one round trip
         // (the neighbours of a leaf's B pixel lie inside the frame too, so the reads are safe for
@@ -538,6 +543,20 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         int bl = bloc[x];
         asm volatile("" : "+v"(nx), "+v"(cpc), "+v"(nl[0]), "+v"(nl[1]), "+v"(nl[2]), "+v"(nl[3]), "+v"(nc[0]),
                      "+v"(nc[1]), "+v"(nc[2]), "+v"(nc[3]), "+v"(dleft), "+v"(dhere), "+v"(dup), "+v"(bk), "+v"(bl));
+        // one lane floods: every value it read is the same for the wave, so the bookkeeping below
+        // runs on the scalar unit with scalar branches instead of masked vector code
+        nx = uni(nx);
+        cpc = uni(cpc);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            nl[d] = uni(nl[d]);
+            nc[d] = uni(nc[d]);
+        }
+        dleft = uni(dleft);
+        dhere = uni(dhere);
+        dup = uni(dup);
+        bk = uni(bk);
+        bl = uni(bl);
         qhead[active] = (uint16_t)nx;
         hx = (int)nx;
         if (nx == NIL16) qtail[active] = NIL16;
@@ -560,7 +579,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             const int sk = p + ld[d];
             // a neighbour's state may have changed since the batch read only through this pop's own
             // earlier pushes, which go to other pixels
-            const uint8_t cs = nc[d];
+            const uint32_t cs = nc[d];
             if ((cs & 3) == C_F) {
                 if (cs & (K_ST1 | K_ST2)) continue;
                 const int l = lvd[d];
@@ -1409,7 +1428,7 @@ __global__ __launch_bounds__(64) void k_flood_lds(Dev D, P2 Q, const int32_t* __
         int hx = NIL16;  // head of the active level's FIFO (a copy of qhead[active])
         auto enq = [&](int l, int x) {
             qn[x] = NIL16;
-            const uint16_t t = qtail[l];
+            const uint32_t t = uni(qtail[l]);
             if (t != NIL16) qn[t] = (uint16_t)x;
             else qhead[l] = (uint16_t)x;
             qtail[l] = (uint16_t)x;
@@ -1424,10 +1443,10 @@ __global__ __launch_bounds__(64) void k_flood_lds(Dev D, P2 Q, const int32_t* __
             int p = hx;
             if (p == NIL16) {
                 int l = active + 1;
-                while (l < NQ && qhead[l] == NIL16) ++l;
+                while (l < NQ && uni(qhead[l]) == NIL16) ++l;
                 if (l >= NQ) break;
                 active = l;
-                p = qhead[l];
+                p = uni(qhead[l]);
             }
             // everything the pop reads, in one round trip (this pop's own writes go to p and to
             // distinct neighbours, so the values stay current for its pushes)
@@ -1443,6 +1462,15 @@ __global__ __launch_bounds__(64) void k_flood_lds(Dev D, P2 Q, const int32_t* __
             asm volatile("" : "+v"(nx), "+v"(cp), "+v"(ns[0]), "+v"(ns[1]), "+v"(ns[2]), "+v"(ns[3]), "+v"(ninc[0]),
                          "+v"(ninc[1]), "+v"(ninc[2]), "+v"(ninc[3]), "+v"(ncol[0]), "+v"(ncol[1]), "+v"(ncol[2]),
                          "+v"(ncol[3]));
+            // one lane floods: the values are the wave's, kept in scalar registers
+            nx = uni(nx);
+            cp = uni(cp);
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                ns[d] = uni(ns[d]);
+                ninc[d] = uni(ninc[d]);
+                ncol[d] = uni(ncol[d]);
+            }
             qhead[active] = (uint16_t)nx;
             hx = (int)nx;
             if (nx == NIL16) qtail[active] = NIL16;

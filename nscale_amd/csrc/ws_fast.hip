@@ -357,6 +357,9 @@ __device__ __forceinline__ void gst(T* base, uint32_t i, V v) {
 // LDS per frame pixel: 15 bytes (class 1, label 4, chain key 4, push levels 2, chain item or leaf
 // index 2, FIFO link 2), so a CU holds as many concurrent floods as its LDS allows for the tier.
 constexpr uint16_t NIL16 = 0xFFFF;
+// phase-1 FIFO links: an item index below 0x7FFF (frames hold at most 6400 pixels), P1_LEAF when
+// the item is a leaf, P1_NIL for none
+constexpr uint32_t P1_NIL = 0x7FFF, P1_LEAF = 0x8000;
 enum { K_INRUN = 4, K_ST1 = 8, K_ST2 = 16, K_LV = 32 };
 constexpr int P1_MARGIN = 2;
 
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         lk[k] = l0;
         bkey[k] = cc;
     }
-    for (int l = lane; l < NQ; l += 64) { qhead[l] = NIL16; qtail[l] = NIL16; }
+    for (int l = lane; l < NQ; l += 64) { qhead[l] = P1_NIL; qtail[l] = P1_NIL; }
     __syncthreads();
     // push levels between 4-neighbours (a pair that wraps around a frame row is never used: pushes
     // start at run pixels, which lie the margin away from the frame's edge)
@@ -474,7 +477,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             bkey[k] = ((uint32_t)lvl << 16) | (uint32_t)q;
             bloc[k] = (uint16_t)k;
             code[k] |= K_ST1;
-            qn[k] = nextk < 0 ? NIL16 : (uint16_t)nextk;
+            qn[k] = nextk < 0 ? P1_NIL : (uint16_t)nextk;
             gst(D.ilvl, g, (uint8_t)lvl);
             gst(D.pseq, g, q);
             gst(D.parent, g, -1);
@@ -482,7 +485,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
             gst(D.st, g, 1);
             if (prevk < 0) {
                 const uint16_t t = qtail[lvl];
-                if (t == NIL16) qhead[lvl] = (uint16_t)k;
+                if (t == P1_NIL) qhead[lvl] = (uint16_t)k;
                 else qn[t] = (uint16_t)k;
             }
             if (nextk < 0) qtail[lvl] = (uint16_t)k;
@@ -503,33 +506,48 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
 
     int active = 0;  // no queued item below this level
     int hx = qhead[0];  // head of the active level's FIFO (a copy of qhead[active])
-    auto enq = [&](int l, int x) {  // FIFO of level l; the flood resumes at the lowest queued level
-        qn[x] = NIL16;
+    // FIFO of level l; the flood resumes at the lowest queued level.  Links and heads carry the
+    // queued item's kind (P1_LEAF), so a leaf's pop knows what it pops before reading anything.
+    auto enq = [&](int l, int x, uint32_t kind) {
+        qn[x] = P1_NIL;
         const uint32_t t = uni(qtail[l]);
-        if (t != NIL16) qn[t] = (uint16_t)x;
-        else qhead[l] = (uint16_t)x;
+        const uint32_t xv = (uint32_t)x | kind;
+        if (t != P1_NIL) qn[t] = (uint16_t)xv;
+        else qhead[l] = (uint16_t)xv;
         qtail[l] = (uint16_t)x;
-        if (t == NIL16 && l <= active) {  // nothing is queued below `active`: l was empty
+        if (t == P1_NIL && l <= active) {  // nothing is queued below `active`: l was empty
             active = l;
-            hx = x;
+            hx = (int)xv;
         }
     };
     int seq = ninit, popc = 0, nlf = 0;
     const int ld[4] = {-1, 1, -lw, lw};
     const int gd[4] = {-1, 1, -W, W};
     for (;;) {
-        int x = hx;
-        if (x == NIL16) {
+        int xv = hx;
+        if (xv == P1_NIL) {
             int l = active + 1;
-            while (l < NQ && uni(qhead[l]) == NIL16) ++l;
+            while (l < NQ && uni(qhead[l]) == P1_NIL) ++l;
             if (l == NQ) break;
             active = l;
-            x = uni(qhead[l]);
+            xv = uni(qhead[l]);
         }
-        // the item's link, class and chain entry and its neighbours' values, in one round trip
-        // (the neighbours of a leaf's B pixel lie inside the frame too, so the reads are safe for
-        // either kind of item); the compiler would otherwise sink the pixel-side reads past the
-        // leaf and dam tests and fetch them in further round trips
+        const int x = xv & (int)P1_NIL;
+        if (xv & P1_LEAF) {  // a leaf: record when it pops
+            uint32_t nx = qn[x];
+            int bl = bloc[x];
+            asm volatile("" : "+v"(nx), "+v"(bl));
+            nx = uni(nx);
+            bl = uni(bl);
+            qhead[active] = (uint16_t)nx;
+            hx = (int)nx;
+            if (nx == P1_NIL) qtail[active] = P1_NIL;
+            gst(D.lf_popidx, base + bl, popc++);
+            continue;
+        }
+        // an F item: its link, class and chain entry and its neighbours' values, in one round
+        // trip (the compiler would otherwise sink the push-side reads past the dam test and fetch
+        // them in a second round trip)
         uint32_t nx = qn[x], cpc = code[x];
         int nl[4];
         uint32_t nc[4];
@@ -559,11 +577,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
         bl = uni(bl);
         qhead[active] = (uint16_t)nx;
         hx = (int)nx;
-        if (nx == NIL16) qtail[active] = NIL16;
-        if ((cpc & 3) == C_B) {  // a leaf: record when it pops
-            gst(D.lf_popidx, base + bl, popc++);
-            continue;
-        }
+        if (nx == P1_NIL) qtail[active] = P1_NIL;
         const int p = x;
         const int lvd[4] = {(int)(dleft & 255), (int)(dhere & 255), (int)(dup >> 8), (int)(dhere >> 8)};
         const int gp = gidx(p);
@@ -595,7 +609,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 gst(D.bnode, gs, newer ? gs : gidx(bl));
                 gst(D.st, gs, 1);
                 ++seq;
-                enq(l, sk);
+                enq(l, sk, 0);
             } else if ((cs & 3) == C_B && !(cs & (K_LV | K_ST1))) {
                 // K_ST1 on a B pixel: queued already, by an earlier labelled neighbour of this run
                 // (every popped F pixel of the run is labelled or a dam, and a labelled one queues
@@ -615,7 +629,7 @@ __global__ __launch_bounds__(64) void k_phase1_lds(Dev D, const int32_t* __restr
                 gst(D.lf_popidx, slot, -1);
                 gst(D.lf_bnode, slot, bk < key ? D.N + slot : gidx(bl));
                 ++seq;
-                enq(l, sk);
+                enq(l, sk, P1_LEAF);
             }
         }
         if (seq >= 65535) { atomicOr(D.flags, FB_LEAF_CAP); return; }

@@ -53,12 +53,14 @@ __global__ __launch_bounds__(256) void morph_kernel(const uint8_t* __restrict__ 
     // use in the same iteration compiles to one memory round trip after another)
     const auto gin = as_global(in);
     const int WN = WW * WW;
+    // k / WW by multiply-high (exact for k < 2^16; the window holds at most 94^2 pixels)
+    const uint32_t wmagic = (uint32_t)((0x100000000ull + (unsigned)WW - 1) / (unsigned)WW);
     for (int k0 = 0; k0 < WN; k0 += 8 * 256) {
         uint32_t v[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int k = min(k0 + (int)threadIdx.x + 256 * j, WN - 1);
-            const int lx = k % WW, ly = k / WW;
+            const int ly = (int)__umulhi((uint32_t)k, wmagic), lx = k - ly * WW;
             v[j] = gin[(size_t)clampi(y0 + ly, 0, h - 1) * w + clampi(x0 + lx, 0, w - 1)];
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(256) void morph_kernel(const uint8_t* __restrict__ 
         for (int j = 0; j < 8; ++j) {
             const int k = k0 + (int)threadIdx.x + 256 * j;
             if (k >= WN) continue;
-            const int lx = k % WW, ly = k / WW;
+            const int ly = (int)__umulhi((uint32_t)k, wmagic), lx = k - ly * WW;
             const int gx = x0 + lx, gy = y0 + ly;
             const uint32_t keep = (gx >= 0 && gx < w && gy >= 0 && gy < h) ? 0xFFu : 0u;
             win[ly * WS + lx] = (uint8_t)((v[j] & keep) | ((uint32_t)outside & ~keep));
@@ -153,12 +155,14 @@ __global__ __launch_bounds__(M2_THREADS) void morph2_kernel(const uint8_t* __res
     const int x0 = tx * MT - r, y0 = ty * MT - r;
     const auto gin = as_global(in);
     const int WN = WW * WW;
+    // k / WW by multiply-high (exact for k < 2^16; the window holds at most 94^2 pixels)
+    const uint32_t wmagic = (uint32_t)((0x100000000ull + (unsigned)WW - 1) / (unsigned)WW);
     for (int k0 = 0; k0 < WN; k0 += 8 * M2_THREADS) {
         uint32_t v[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int k = min(k0 + (int)threadIdx.x + M2_THREADS * j, WN - 1);
-            const int lx = k % WW, ly = k / WW;
+            const int ly = (int)__umulhi((uint32_t)k, wmagic), lx = k - ly * WW;
             v[j] = gin[(size_t)clampi(y0 + ly, 0, h - 1) * w + clampi(x0 + lx, 0, w - 1)];
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -166,7 +170,7 @@ __global__ __launch_bounds__(M2_THREADS) void morph2_kernel(const uint8_t* __res
         for (int j = 0; j < 8; ++j) {
             const int k = k0 + (int)threadIdx.x + M2_THREADS * j;
             if (k >= WN) continue;
-            const int lx = k % WW, ly = k / WW;
+            const int ly = (int)__umulhi((uint32_t)k, wmagic), lx = k - ly * WW;
             const int gx = x0 + lx, gy = y0 + ly;
             const bool in_img = gx >= 0 && gx < w && gy >= 0 && gy < h;
             win[ly * WS + lx] = (uint16_t)(in_img ? v[j] : (uint32_t)outside);

@@ -16,6 +16,7 @@
 //      - row state (lane = row, after a transpose through LDS): left-right and right-left
 //        sweeps, out = min(max(out, left, up-left, down-left), mask) (the diagonal neighbours only
 //        with 8-connectivity; right instead of left for the right-left sweep).
+//    The two sweeps of each state run as two interleaved chains updating the values in place.
 //    A fixpoint of these four sweeps satisfies every 8- (or 4-) neighbour constraint.
 //  * Pass 0 runs every tile once, reading marker and mask (halo = min(marker, mask)) and writing
 //    the output: 3 bytes/px for uint8, the algorithmic minimum.  A tile whose owned border

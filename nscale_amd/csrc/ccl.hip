@@ -851,6 +851,19 @@ int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int3
     return NSC_OK;
 }
 
+// the number of components of a labelling (its roots: label == own index), without relabelling
+int root_count_impl(const int32_t* labels, int w, int h, int* dcount, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    const int nb = (int)cdiv(n, SCAN_B);
+    Scratch sc;
+    NSC_CHECK(sc.alloc(sizeof(int32_t) * (nb + 1), s));
+    int32_t* bsum = sc.as<int32_t>();
+    hipLaunchKernelGGL(k_root_block_count, dim3(nb), dim3(SCAN_B / 4), 0, s, labels, n, bsum);
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, bsum, nb, dcount);
+    NSC_HIP(hipGetLastError());
+    return NSC_OK;
+}
+
 int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s);
 
 // ConnComponents::boundingBox(w, h, labels, bgval, compcount) with the reference's result block

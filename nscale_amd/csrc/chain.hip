@@ -30,6 +30,7 @@ size_t ws2_scratch(int w, int h);
 int ws2_from_labels(const uint8_t* bgr, const int32_t* labels, int32_t* out, int w, int h, void* scratch, hipStream_t s);
 int ccl_u8(const uint8_t* img, int32_t* labels, int w, int h, int conn, bool equal_values, int32_t bgval, hipStream_t s);
 int relabel_impl(int32_t* labels, int w, int h, int32_t bgval, int* dcount, int32_t* rank, hipStream_t s);
+int root_count_impl(const int32_t* labels, int w, int h, int* dcount, hipStream_t s);
 int bbox_impl(const int32_t* canon, int w, int h, int n, int32_t* out5, int32_t* area, hipStream_t s);
 int bbox_literal_impl(const int32_t* L, int w, int h, int32_t bgval, int32_t* out5, int cap, int* compcount,
                       int32_t* firstpos, int32_t* R, int32_t* rank, int* cnt, hipStream_t s);
@@ -532,9 +533,12 @@ int segment_body(const uint8_t* bgr_in, int w, int h, const nsc_seg_params& p, i
     const bool cpu_boxes = io && io->bbox_mode == NSC_BBOX_REFERENCE_CPU;
     if (canonical && !cpu_boxes) {
         NSC_CHECK(relabel_impl(labels, w, h, -1, &W.counts[3], W.area, s));
-    } else {
+    } else if (io && io->bbox_mode == NSC_BBOX_ALL) {
+        // the boxes of the reference's labels come from a canonical copy
         NSC_HIP(hipMemcpyAsync(W.L, labels, 4 * n, hipMemcpyDeviceToDevice, s));
         NSC_CHECK(relabel_impl(W.L, w, h, -1, &W.counts[3], W.area, s));
+    } else {
+        NSC_CHECK(root_count_impl(labels, w, h, &W.counts[3], s));  // the nucleus count only
     }
     C.mark(25);
     NSC_HIP(hipGetLastError());

@@ -139,9 +139,10 @@ __global__ void k_ws_border_crop(const int32_t* __restrict__ P, int32_t* __restr
     }
 }
 
-// labels (w x h) -> padded marker image with a 0 frame; BGR image -> padded with (0,0,0)
+// labels (w x h) -> padded marker image with a 0 frame, written twice (P, the flood's in/out image,
+// and P0, its copy of the initial markers); BGR image -> padded with (0,0,0)
 __global__ void k_ws_pad(const int32_t* __restrict__ lab, const uint8_t* __restrict__ bgr, int32_t* __restrict__ P,
-                         uint8_t* __restrict__ Pimg, int w, int h) {
+                         int32_t* __restrict__ P0, uint8_t* __restrict__ Pimg, int w, int h) {
     const int W = w + 2, H = h + 2;
     const size_t n = (size_t)W * H;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -152,6 +153,7 @@ __global__ void k_ws_pad(const int32_t* __restrict__ lab, const uint8_t* __restr
         const int32_t lv = lab[j];
         const uint8_t c0 = bgr[3 * j], c1 = bgr[3 * j + 1], c2 = bgr[3 * j + 2];
         P[i] = in ? lv : 0;
+        P0[i] = in ? lv : 0;
         Pimg[3 * i] = in ? c0 : 0;
         Pimg[3 * i + 1] = in ? c1 : 0;
         Pimg[3 * i + 2] = in ? c2 : 0;
@@ -179,17 +181,18 @@ size_t cvws_scratch(int W, int H) {
     return n * (sizeof(int32_t) * 3 + 1 + sizeof(uint16_t)) + 64;
 }
 
-// markers m (W x H) in/out; m0 scratch copy is made here.  The decomposed flood (cvws_fast) is
-// tried first; when its assumptions do not hold it reports so and the plain per-component
-// sequential flood below produces the result.  NSC_WATERSHED=seq forces the sequential flood.
-int cvws_impl(const uint8_t* img, int32_t* m, int W, int H, void* scratch, hipStream_t s) {
+// markers m (W x H) in/out; the copy of the initial markers, m0, is the start of the scratch (made
+// here unless the caller has written it already).  The decomposed flood (cvws_fast) is tried first;
+// when its assumptions do not hold it reports so and the plain per-component sequential flood
+// below produces the result.  NSC_WATERSHED=seq forces the sequential flood.
+int cvws_impl(const uint8_t* img, int32_t* m, int W, int H, void* scratch, hipStream_t s, bool m0_written = false) {
     const size_t n = (size_t)W * H;
     int32_t* m0 = static_cast<int32_t*>(scratch);
     int32_t* comp = m0 + n;
     int32_t* rank = comp + n;  // relabel scratch, then the flood's next[] links
     uint16_t* lvl = reinterpret_cast<uint16_t*>(rank + n);
     uint8_t* unknown = reinterpret_cast<uint8_t*>(lvl + n);
-    NSC_HIP(hipMemcpyAsync(m0, m, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    if (!m0_written) NSC_HIP(hipMemcpyAsync(m0, m, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
     const char* mode = getenv("NSC_WATERSHED");
     const bool force_seq = mode && mode[0] == 's';
     if (!force_seq) {
@@ -234,8 +237,9 @@ int ws2_from_labels(const uint8_t* bgr, const int32_t* labels, int32_t* out, int
     int32_t* P = static_cast<int32_t*>(scratch);
     uint8_t* Pimg = reinterpret_cast<uint8_t*>(P + N);
     void* rest = Pimg + 3 * N + (16 - (3 * N) % 16) % 16;
-    hipLaunchKernelGGL(k_ws_pad, grid1(N), dim3(256), 0, s, labels, bgr, P, Pimg, w, h);
-    NSC_CHECK(cvws_impl(Pimg, P, w + 2, h + 2, rest, s));
+    // the flood's copy of the initial markers (the start of its scratch) is written by the padding
+    hipLaunchKernelGGL(k_ws_pad, grid1(N), dim3(256), 0, s, labels, bgr, P, static_cast<int32_t*>(rest), Pimg, w, h);
+    NSC_CHECK(cvws_impl(Pimg, P, w + 2, h + 2, rest, s, true));
     hipLaunchKernelGGL(k_ws_border_crop, grid1((size_t)w * h), dim3(256), 0, s, P, out, w, h);
     NSC_HIP(hipGetLastError());
     return NSC_OK;

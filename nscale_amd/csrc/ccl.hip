@@ -120,7 +120,8 @@ __device__ void gunite(int32_t* L, int a, int b) {
 // Every skipped join is implied by a path through joins that are united or skipped only by a rule
 // that looks further left, so the components are those of all joins.
 template <int MODE, int CONN>
-__global__ __launch_bounds__(256) void ccl_local(Px P, int32_t* __restrict__ L, int w, int h, int tiles_x) {
+__global__ __launch_bounds__(256) void ccl_local(Px P, int32_t* __restrict__ L, int w, int h, int tiles_x,
+                                                 uint8_t* __restrict__ clear8) {
     constexpr int PER = CT_N / 256;  // rows per thread
     using V = typename std::conditional<MODE == FG_FEQ, float, uint32_t>::type;
     __shared__ int ll[CT_N];
@@ -216,6 +217,9 @@ __global__ __launch_bounds__(256) void ccl_local(Px P, int32_t* __restrict__ L, 
         if (r >= 0) {
             r = lfind(ll, k);
             out = (int32_t)((size_t)(y0 + r / CT_W) * w + (x0 + r % CT_W));
+            // clear8: entry cleared at every tile-local root (every final root is one), so a per-root
+            // table filled while flattening needs no image-sized fill
+            if (clear8 && r == k) clear8[out] = 0;
         }
         if (gx < w && gy < h) L[(size_t)gy * w + gx] = out;
     }
@@ -305,15 +309,28 @@ __global__ __launch_bounds__(256) void ccl_seams(Px P, int32_t* L, int w, int h,
 // together instead of one dependent round trip after another.
 // `zero_at_roots` / `zero8_at_roots` (optional): entry i cleared for every root i, so a per-root
 // table (areas, selection flags) needs no image-sized fill before it is filled in.
+// `flag_set` (optional): flag_set[root] = 1 for every component holding a pixel with
+// flag_src[i] != 0 (flag_src null: a pixel on the border of the fw x fh image); the entries must
+// have been cleared at the roots before (ccl_local's clear8)
 __global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval, int32_t* __restrict__ zero_at_roots,
-                            uint8_t* __restrict__ zero8_at_roots) {
+                            uint8_t* __restrict__ zero8_at_roots, const uint8_t* __restrict__ flag_src,
+                            uint8_t* flag_set, int fw, int fh) {
     const size_t S = (size_t)gridDim.x * blockDim.x;
     for (size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * S) {
         int p[4], q[4];
+        bool sv[4] = {false, false, false, false};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const size_t i = i0 + j * S;
             p[j] = i < n ? ld_wg(&L[i]) : -1;
+            if (flag_set && i < n) {
+                if (flag_src) {
+                    sv[j] = flag_src[i] != 0;
+                } else {
+                    const int x = (int)(i % fw), y = (int)(i / fw);
+                    sv[j] = x == 0 || y == 0 || x == fw - 1 || y == fh - 1;
+                }
+            }
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) q[j] = p[j] >= 0 ? ld_wg(&L[p[j]]) : p[j];
@@ -336,26 +353,28 @@ __global__ void ccl_flatten(int32_t* L, size_t n, int32_t bgval, int32_t* __rest
                 nx = ld_wg(&L[r]);
             }
             if (r != p[j]) L[i] = r;
+            if (sv[j]) flag_set[r] = 1;
         }
     }
 }
 
 template <int MODE>
 int ccl_run(const Px& P, int32_t* L, int w, int h, int conn, int32_t bgval, hipStream_t s,
-            int32_t* zero_at_roots = nullptr, uint8_t* zero8_at_roots = nullptr) {
+            int32_t* zero_at_roots = nullptr, uint8_t* zero8_at_roots = nullptr, uint8_t* flag_set = nullptr,
+            const uint8_t* flag_src = nullptr) {
     const int tiles_x = (int)cdiv(w, CT_W), tiles_y = (int)cdiv(h, CT_H);
     const int nt = tiles_x * tiles_y;
     const int nseam8 = nt * (CT_W + 2 * CT_H), nseam4 = nt * (CT_W + CT_H);
     const size_t n = (size_t)w * h;
     if (conn == 8) {
-        hipLaunchKernelGGL((ccl_local<MODE, 8>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x);
+        hipLaunchKernelGGL((ccl_local<MODE, 8>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x, flag_set);
         hipLaunchKernelGGL((ccl_seams<MODE, 8>), dim3(cdiv(nseam8, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
     } else {
-        hipLaunchKernelGGL((ccl_local<MODE, 4>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x);
+        hipLaunchKernelGGL((ccl_local<MODE, 4>), dim3(nt), dim3(256), 0, s, P, L, w, h, tiles_x, flag_set);
         hipLaunchKernelGGL((ccl_seams<MODE, 4>), dim3(cdiv(nseam4, 256)), dim3(256), 0, s, P, L, w, h, tiles_x, nt);
     }
     hipLaunchKernelGGL(ccl_flatten, dim3(std::min<unsigned>(cdiv(n, 1024), 65535u)), dim3(256), 0, s, L, n, bgval,
-                       zero_at_roots, zero8_at_roots);
+                       zero_at_roots, zero8_at_roots, flag_src, flag_set, w, h);
     NSC_HIP(hipGetLastError());
     return NSC_OK;
 }
@@ -415,30 +434,6 @@ __global__ void k_area_select(const int32_t* __restrict__ L, const int32_t* __re
         }
     }
     if (count) block_count_add(local, count);  // `count` is the same for every thread
-}
-
-// flag[root] = 1 for every component that holds a pixel with src[i] != 0 (src may be null:
-// then the image border acts as the source)
-__global__ void k_flag_roots(const int32_t* __restrict__ L, const uint8_t* __restrict__ src, uint8_t* flag, int w, int h) {
-    const size_t n = (size_t)w * h;
-    NSC_FOR4(n) {
-        int r[4];
-        bool sv[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const size_t i = i0 + j * S;
-            r[j] = i < n ? L[i] : -1;
-            if (src) {
-                sv[j] = i < n && src[i] != 0;
-            } else {
-                const int x = (int)(i % w), y = (int)(i / w);
-                sv[j] = x == 0 || y == 0 || x == w - 1 || y == h - 1;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (r[j] >= 0 && sv[j]) flag[r[j]] = 1;
-    }
 }
 
 // imreconstructBinary output: marker where marker != 0, else 255 where the mask component holds a
@@ -813,12 +808,9 @@ int bwareaopen_impl(const uint8_t* img, uint8_t* out, int w, int h, int lo, int 
 // component selection: passable = (img != key), sources = src != 0 (or the border when src null)
 static int select_components(const uint8_t* img, uint8_t key, const uint8_t* src, int32_t* labels, uint8_t* flag,
                              int w, int h, int conn, hipStream_t s) {
-    const size_t n = (size_t)w * h;
-    // labelling whose flatten clears the roots' flags (the only entries read or set)
-    NSC_CHECK(ccl_run<FG_NEQ>(Px{img, nullptr, key}, labels, w, h, conn, -1, s, nullptr, flag));
-    hipLaunchKernelGGL(k_flag_roots, grid1(n), dim3(256), 0, s, labels, src, flag, w, h);
-    NSC_HIP(hipGetLastError());
-    return NSC_OK;
+    // labelling that clears the roots' flags (the only entries read or set) in its tile pass and
+    // sets them while flattening: flag[root] = 1 for components holding a source
+    return ccl_run<FG_NEQ>(Px{img, nullptr, key}, labels, w, h, conn, -1, s, nullptr, nullptr, flag, src);
 }
 
 int fill_holes_impl(const uint8_t* img, uint8_t* out, int w, int h, int conn, int32_t* labels, uint8_t* flag,

@@ -1680,6 +1680,20 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
             int bc[kRunTiers];
             (void)hipMemcpy(bc, bcount, sizeof(bc), hipMemcpyDeviceToHost);
             fprintf(stderr, "[ws] run tiers %d %d %d %d %d %d\n", bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
+            std::vector<int32_t> hb((size_t)5 * nrun);
+            (void)hipMemcpy(hb.data(), box5, sizeof(int32_t) * hb.size(), hipMemcpyDeviceToHost);
+            long long area[kRunTiers] = {}, amax[kRunTiers] = {};
+            for (int r = 0; r < nrun; ++r) {
+                const long long a = (long long)(hb[2 * (size_t)nrun + r] - hb[nrun + r] + 1 + 2 * P1_MARGIN) *
+                                    (hb[4 * (size_t)nrun + r] - hb[3 * (size_t)nrun + r] + 1 + 2 * P1_MARGIN);
+                const int tier[kRunTiers - 1] = {512, 1024, 2048, 4096, 6400};
+                int b = 0;
+                while (b < kRunTiers - 1 && a > tier[b]) ++b;
+                area[b] += a;
+                amax[b] = std::max(amax[b], a);
+            }
+            for (int b = 0; b < kRunTiers; ++b)
+                fprintf(stderr, "[ws] tier %d: runs %d, box px %lld, max %lld\n", b, bc[b], area[b], amax[b]);
         }
         auto L = [&](int b) { return lists + (size_t)b * nrun; };
         // the few large runs set the latency: the largest tiers start first, each on a side stream,

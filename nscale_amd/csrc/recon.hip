@@ -1396,8 +1396,15 @@ struct GeoFlags {
     __device__ __forceinline__ unsigned blk_row(int x) const { return 0u - ((fr[x >> 5] >> (x & 31)) & 1u); }
 };
 
+// One step along a line: min(v, prev + 1) for a passable pixel (blk = 0); a frozen pixel inside a
+// line always holds GINF (outside the image, or not passable: never lowered) and keeps it, since
+// med3(GINF, x, GINF) = GINF, while med3(v, x, 0) = min(v, x).  The halo lanes (lanes 0 and TW-1,
+// frozen but holding finite neighbour distances) are kept out of the sweeps by the callers' lane
+// test instead.
 __device__ __forceinline__ unsigned geo_step(unsigned v, unsigned prev, unsigned blk) {
-    return min(v, __builtin_elementwise_add_sat(prev, 1u) | blk);
+    unsigned r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(__builtin_elementwise_add_sat(prev, 1u)), "v"(blk));
+    return r;
 }
 
 // Frozen-pixel bits of every window (the passable set is fixed for a whole geodesic call, so
@@ -1473,6 +1480,8 @@ __device__ __forceinline__ void geo_load(const uint4* __restrict__ flags, const 
 // their minimum is exactly the sequential down-then-up result.
 __device__ __forceinline__ void geo_col_sweeps(unsigned (&v)[TH], GeoFlags& F) {
     F.opaque();
+    const int lane = threadIdx.x & 63;
+    if (lane < 1 || lane > TW - 2) return;  // halo columns: sources, never stepped
     unsigned dn[TH], up[TH];
     unsigned pd = v[0], pu = v[TH - 1];
 #pragma unroll
@@ -1489,6 +1498,8 @@ __device__ __forceinline__ void geo_col_sweeps(unsigned (&v)[TH], GeoFlags& F) {
 
 __device__ __forceinline__ void geo_row_sweeps(unsigned (&v)[TW], GeoFlags& F) {
     F.opaque();
+    const int lane = threadIdx.x & 63;
+    if (lane < 1 || lane > TH - 2) return;  // halo rows: sources, never stepped
     unsigned lt[TW], rt[TW];
     unsigned pl = v[0], pr = v[TW - 1];
 #pragma unroll

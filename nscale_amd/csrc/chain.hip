@@ -8,6 +8,8 @@
 #include <cstring>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "common.hpp"
 
 namespace nsc {
@@ -314,18 +316,52 @@ __global__ void k_root_label_column(const int32_t* __restrict__ L, const int32_t
     }
 }
 
+// The reference's logger stage name (SimpleCSVLogger::logTimeSinceLastLog, HistologicalEntities.cpp
+// :121-588) of the stage that follows the mark with stage id `id`; null after the last one.
+static const char* stage_after(int id) {
+    switch (id) {
+    case 0: return "background";      // toRGB + background ratio (:121-140)
+    case 1: return "RBC";             // :144
+    case 2: return "open19";          // invert + open19 (:159-194)
+    case 3: return "reconToNuclei";   // :213
+    case 26: return "threshold1";     // :228
+    case 4: return "fillHoles1";      // :234
+    case 7: return "areaThreshold1";  // :268
+    case 8: return "blobsGt45";       // :299
+    case 10: return "fillHoles2";     // :495
+    case 12: return "openBlobs";      // :520
+    case 13: return "30To1000";       // :323
+    case 14: return "dilate";         // :334
+    case 15: return "distTransform";  // :376
+    case 16: return "imhmin";         // :385
+    case 18: return "watershed";      // watershed + water to mask (:419-425)
+    case 20: return "20To1000";       // :556
+    case 23: return "fillHolesLast";  // :580
+    case 24: return "bwlabel2";       // :588
+    default: return nullptr;
+    }
+}
+
 struct Chain {
     Workspace& W;
     hipStream_t s;
     void** stage;
     double* stage_ms;
     std::vector<std::pair<int, hipEvent_t>> marks;
+    bool range_open = false;  // a roctx range named for the current stage is pushed
     Chain(Workspace& w, hipStream_t st, void** sg, double* ms) : W(w), s(st), stage(sg), stage_ms(ms) {}
     ~Chain() {
+        if (range_open) roctxRangePop();
         for (auto& m : marks) (void)hipEventDestroy(m.second);
     }
-    // timing mark: the time since the previous mark is charged to stage id
+    // stage boundary: closes the roctx range of the stage that ends here (host time: the interval in
+    // which its kernels were queued) and opens the next stage's, under the reference's logger names;
+    // with timers, the GPU time since the previous mark is charged to stage id
     void mark(int id) {
+        if (range_open) roctxRangePop();
+        const char* next = stage_after(id);
+        range_open = next != nullptr;
+        if (next) roctxRangePushA(next);
         if (!stage_ms) return;
         hipEvent_t e;
         if (hipEventCreate(&e) == hipSuccess) {

@@ -184,7 +184,10 @@ size_t cvws_scratch(int W, int H) {
 // markers m (W x H) in/out; the copy of the initial markers, m0, is the start of the scratch (made
 // here unless the caller has written it already).  The decomposed flood (cvws_fast) is tried first;
 // when its assumptions do not hold it reports so and the plain per-component sequential flood
-// below produces the result.  NSC_WATERSHED=seq forces the sequential flood.
+// below produces the result.  NSC_WATERSHED=seq forces the sequential flood; NSC_WATERSHED=fast
+// bounds the latency instead: such an input returns NSC_ERR_LIMIT (the sequential flood of a
+// 4096^2 tile's background takes ~20 s).  NSC_WATERSHED_REJECT=1 makes the host treat every
+// decomposed result as not applicable (failure-handling tests of both paths).
 int cvws_impl(const uint8_t* img, int32_t* m, int W, int H, void* scratch, hipStream_t s, bool m0_written = false) {
     const size_t n = (size_t)W * H;
     int32_t* m0 = static_cast<int32_t*>(scratch);
@@ -198,6 +201,8 @@ int cvws_impl(const uint8_t* img, int32_t* m, int W, int H, void* scratch, hipSt
     if (!force_seq) {
         int ok = 0;
         NSC_CHECK(cvws_fast(img, m0, m, W, H, &ok, s));
+        const char* reject = getenv("NSC_WATERSHED_REJECT");
+        if (reject && reject[0] == '1') ok = 0;
         ws_stats_record(ok);
         if (ok) return NSC_OK;
         if (mode && mode[0] == 'f') {

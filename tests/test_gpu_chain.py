@@ -89,3 +89,28 @@ def test_watershed_decomposition_used_and_equal_to_sequential(monkeypatch, size,
     torch.cuda.synchronize()
     assert st == st2 == 0
     assert torch.equal(bufs[20], bufs2[20]) and torch.equal(lab, lab2)
+
+
+def test_watershed_fallback_and_latency_guard(monkeypatch):
+    """A watershed input that the decomposed flood does not serve (forced from the host with
+    NSC_WATERSHED_REJECT=1) goes to the exact sequential flood and gives the decomposed result's
+    bytes; with NSC_WATERSHED=fast the chain returns NSC_ERR_LIMIT instead of waiting for it."""
+    from nscale_amd import gpu
+    from nscale_amd._lib import NscError
+    from nscale_amd.synth import synth_he
+    print("watershed fallback and latency guard 512 seed 5", flush=True)
+    bgr = torch.from_numpy(synth_he(512, 512, 5)).cuda()
+    st, lab, _, bufs, _ = gpu.segment_nuclei(bgr, stages=(20,))
+    torch.cuda.synchronize()
+    assert st == 0
+    monkeypatch.setenv("NSC_WATERSHED_REJECT", "1")
+    f0, b0 = gpu.watershed_stats()
+    st2, lab2, _, bufs2, _ = gpu.segment_nuclei(bgr, stages=(20,))
+    torch.cuda.synchronize()
+    assert gpu.watershed_stats() == (f0, b0 + 1), "the sequential flood must serve the rejected input"
+    assert st2 == 0 and torch.equal(bufs[20], bufs2[20]) and torch.equal(lab, lab2)
+    monkeypatch.setenv("NSC_WATERSHED", "fast")
+    with pytest.raises(NscError, match="decomposed flood does not apply"):
+        gpu.segment_nuclei(bgr, stages=(20,))
+    torch.cuda.synchronize()
+    assert gpu.watershed_stats() == (f0, b0 + 2)

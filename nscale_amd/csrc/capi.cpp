@@ -34,6 +34,55 @@ SyncState& sync_state() {
 
 void set_thread_blocking_sync(bool on) { sync_state().blocking = on; }
 
+namespace {
+// per-thread pinned word and event for the pass loops' asynchronous count reads
+struct CountReader {
+    int* host = nullptr;
+    hipEvent_t ev = nullptr;
+    int dev = -1;
+    bool bridge = true;
+    bool failed = false;
+};
+CountReader& count_reader() {
+    static thread_local CountReader r;  // the pinned word and event live until the process ends
+    return r;
+}
+}  // namespace
+
+void set_thread_pass_bridge(bool on) { count_reader().bridge = on; }
+
+int* count_read_async(const int* d, hipStream_t s) {
+    CountReader& r = count_reader();
+    if (!r.bridge || r.failed) return nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    if (!r.host && hipHostMalloc(reinterpret_cast<void**>(&r.host), 64, hipHostMallocDefault) != hipSuccess) {
+        r.host = nullptr;
+        r.failed = true;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (!r.ev || r.dev != dev) {
+        if (r.ev) (void)hipEventDestroy(r.ev);
+        r.ev = nullptr;
+        if (hipEventCreateWithFlags(&r.ev, hipEventDisableTiming) != hipSuccess) {
+            r.ev = nullptr;
+            r.failed = true;
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        r.dev = dev;
+    }
+    if (hipMemcpyAsync(r.host, d, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipEventRecord(r.ev, s) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return r.host;
+}
+
+hipError_t count_wait() { return hipEventSynchronize(count_reader().ev); }
+
 Arena*& thread_arena() {
     static thread_local Arena* a = nullptr;
     return a;

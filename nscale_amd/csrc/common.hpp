@@ -183,6 +183,15 @@ SideStream& side_stream();
 // blocking waits (the server's workers, many of which wait at once) sleeps on an event instead.
 hipError_t stream_sync(hipStream_t s);
 void set_thread_blocking_sync(bool on);
+// Pass loops (reconstruction, geodesic distance) read each check's list length through a pinned
+// word and queue a short bridge of further passes before waiting, so the GPU keeps working during
+// the host round trip (passes after the last busy one find empty lists and exit).  On by default;
+// the server's slot threads turn it off (other slots fill those gaps, and the bridge's empty
+// passes would take their GPU time).  count_read_async queues the copy of *d and returns the host
+// word (valid after count_wait), or nullptr when the bridge is off or unavailable.
+void set_thread_pass_bridge(bool on);
+int* count_read_async(const int* d, hipStream_t s);
+hipError_t count_wait();
 
 // ---- device helpers ----------------------------------------------------------------------
 // max/min with the selection rule of std::max / std::min (first argument on ties)

@@ -822,6 +822,9 @@ constexpr int kRing = 4;
 // least one pixel along some propagation path, and every path is shorter than the pixel count.
 static long long pass_limit(long long pixels) { return pixels + 8; }
 
+// passes queued behind a host check's count read (about the round trip's length in empty passes)
+constexpr int kPassBridge = 4;
+
 // Device buffers of a reconstruction call for nt tiles (regions), carved from one block: the drain's
 // dirty and busy bitmaps and its counters, the passes' stamps and counter ring (all of these start
 // at zero: one clear covers them), then the passes' two tile lists.
@@ -917,24 +920,35 @@ static int recon_passes(const RImg<T>* imgs_dev, const RImg<T>& one, const int* 
     };
     launch(0, nullptr, nullptr, B.listA, &counts[0], &counts[1]);
     NSC_HIP(hipGetLastError());
-    long long pass = 1;
-    // passes between host checks: the schedule's batch b, then 2b, then 4b from there on (a host
-    // round trip costs about as much as ten empty passes, and calls that run past 3b passes
-    // usually run far past it)
-    int batch = std::max(1, tune<T>().batch);
-    for (int check = 0;; ++check, batch = check <= 2 ? 2 * batch : batch) {
-        for (int b = 0; b < batch && pass <= max_pass; ++b, ++pass) {
+    long long pass = 1;  // next pass to queue
+    auto queue_to = [&](long long last) {
+        for (; pass <= last && pass <= max_pass; ++pass) {
             int* cur = (pass & 1) ? B.listA : B.listB;
             int* nxt = (pass & 1) ? B.listB : B.listA;
             launch((int)std::min<long long>(pass, INT_MAX - 1), cur, &counts[(pass - 1) % kRing], nxt, &counts[pass % kRing],
                    &counts[(pass + 1) % kRing]);
         }
+    };
+    // passes between host checks: the schedule's batch b, then 2b, then 4b from there on (a host
+    // round trip costs about as much as ten empty passes, and calls that run past 3b passes
+    // usually run far past it); a bridge of kPassBridge passes is queued behind each count read
+    int batch = std::max(1, tune<T>().batch);
+    for (int check = 0;; ++check, batch = check <= 2 ? 2 * batch : batch) {
+        queue_to(pass - 1 + batch);
         NSC_HIP(hipGetLastError());
+        const long long last = pass - 1;  // the check reads the list length pass `last` produced
         int c = 0;
-        NSC_HIP(hipMemcpyAsync(&c, &counts[(pass - 1) % kRing], sizeof(int), hipMemcpyDeviceToHost, s));
-        NSC_HIP(stream_sync(s));
+        if (const int* hc = count_read_async(&counts[last % kRing], s)) {
+            queue_to(last + kPassBridge);
+            NSC_HIP(hipGetLastError());
+            NSC_HIP(count_wait());
+            c = *hc;
+        } else {
+            NSC_HIP(hipMemcpyAsync(&c, &counts[last % kRing], sizeof(int), hipMemcpyDeviceToHost, s));
+            NSC_HIP(stream_sync(s));
+        }
         if (c == 0) break;
-        if (pass > max_pass) {
+        if (last >= max_pass) {
             set_error("imreconstruct: no convergence after %lld passes", max_pass);
             return NSC_ERR_LIMIT;
         }
@@ -1859,23 +1873,34 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
                                clr, stamp, p);
     };
     launch(0, nullptr, nullptr, listA, &counts[0], &counts[1]);
-    long long p = 1;
-    // host checks after 8, 24 and 56 passes, then every 8: the front needs tens of passes to cross
-    // a tile (about 60 on a 4096^2 nuclei tile), an empty pass costs a few microseconds and a host
-    // round trip a few tens
-    for (int batch = 8;; batch = p > 56 ? 8 : 2 * batch) {
-        for (int b = 0; b < batch && p <= max_pass; ++b, ++p) {
+    long long p = 1;  // next pass to queue
+    auto queue_to = [&](long long last) {
+        for (; p <= last && p <= max_pass; ++p) {
             int* cur = (p & 1) ? listA : listB;
             int* nxt = (p & 1) ? listB : listA;
             launch((int)std::min<long long>(p, INT_MAX - 1), cur, &counts[(p - 1) % kRing], nxt, &counts[p % kRing],
                    &counts[(p + 1) % kRing]);
         }
+    };
+    // host checks after 8, 24 and 56 passes, then every 8: the front needs tens of passes to cross
+    // a tile (about 60 on a 4096^2 nuclei tile), an empty pass costs a few microseconds and a host
+    // round trip a few tens; a bridge of kPassBridge passes is queued behind each count read
+    for (long long target = 8, batch = 8;; batch = target >= 56 ? 8 : 2 * batch, target += batch) {
+        queue_to(target);
         NSC_HIP(hipGetLastError());
+        const long long last = p - 1;  // the check reads the list length pass `last` produced
         int c = 0;
-        NSC_HIP(hipMemcpyAsync(&c, &counts[(p - 1) % kRing], sizeof(int), hipMemcpyDeviceToHost, s));
-        NSC_HIP(stream_sync(s));
+        if (const int* hc = count_read_async(&counts[last % kRing], s)) {
+            queue_to(last + kPassBridge);
+            NSC_HIP(hipGetLastError());
+            NSC_HIP(count_wait());
+            c = *hc;
+        } else {
+            NSC_HIP(hipMemcpyAsync(&c, &counts[last % kRing], sizeof(int), hipMemcpyDeviceToHost, s));
+            NSC_HIP(stream_sync(s));
+        }
         if (c == 0) break;
-        if (p > max_pass) {
+        if (last >= max_pass) {
             set_error("geodesic distance: no convergence after %lld passes", max_pass);
             return NSC_ERR_LIMIT;
         }

@@ -42,9 +42,13 @@ struct CountReader {
     int dev = -1;
     bool bridge = true;
     bool failed = false;
+    ~CountReader() {
+        if (ev) (void)hipEventDestroy(ev);
+        if (host) (void)hipHostFree(host);
+    }
 };
 CountReader& count_reader() {
-    static thread_local CountReader r;  // the pinned word and event live until the process ends
+    static thread_local CountReader r;
     return r;
 }
 }  // namespace

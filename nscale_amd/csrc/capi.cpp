@@ -53,7 +53,11 @@ CountReader& count_reader() {
 }
 }  // namespace
 
-void set_thread_pass_bridge(bool on) { count_reader().bridge = on; }
+bool set_thread_pass_bridge(bool on) {
+    const bool prev = count_reader().bridge;
+    count_reader().bridge = on;
+    return prev;
+}
 
 int* count_read_async(const int* d, hipStream_t s) {
     CountReader& r = count_reader();

@@ -187,9 +187,11 @@ void set_thread_blocking_sync(bool on);
 // word and queue a short bridge of further passes before waiting, so the GPU keeps working during
 // the host round trip (passes after the last busy one find empty lists and exit).  On by default;
 // the server's slot threads turn it off (other slots fill those gaps, and the bridge's empty
-// passes would take their GPU time).  count_read_async queues the copy of *d and returns the host
-// word (valid after count_wait), or nullptr when the bridge is off or unavailable.
-void set_thread_pass_bridge(bool on);
+// passes would take their GPU time), and so does the no-alloc reconstruction (the bridge's pinned
+// word is allocated on a thread's first use).  set_thread_pass_bridge returns the previous setting.
+// count_read_async queues the copy of *d and returns the host word (valid after count_wait), or
+// nullptr when the bridge is off or unavailable.
+bool set_thread_pass_bridge(bool on);
 int* count_read_async(const int* d, hipStream_t s);
 hipError_t count_wait();
 

@@ -1367,7 +1367,12 @@ extern "C" int nsc_imreconstruct_u8_noalloc(const uint8_t* marker, const uint8_t
     hipStream_t s = as_stream(stream);
     const ReconBufs B = recon_carve(base, (int)nt);  // fits: nt <= kNoAllocMaxTiles
     recon_clear(B, s);
-    return recon_run<uint8_t>(nullptr, one, nullptr, (int)nt, (long long)w * h, conn, G, B, s);
+    // the host checks read their counts synchronously here: the bridged reads would allocate a
+    // pinned word on this thread's first use
+    const bool bridge = set_thread_pass_bridge(false);
+    const int rc = recon_run<uint8_t>(nullptr, one, nullptr, (int)nt, (long long)w * h, conn, G, B, s);
+    (void)set_thread_pass_bridge(bridge);
+    return rc;
 }
 
 extern "C" int nsc_recon_set_schedule(int is_float, int region, int passes, int queue) {

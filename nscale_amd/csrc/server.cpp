@@ -170,7 +170,7 @@ struct nsc_server {
         Slot& S = slots[k];
         (void)hipSetDevice(device);
         nsc::set_thread_blocking_sync(true);
-        nsc::set_thread_pass_bridge(false);
+        (void)nsc::set_thread_pass_bridge(false);
         for (;;) {
             Req* q = nullptr;
             size_t ws_new = 0, stage_new = 0;

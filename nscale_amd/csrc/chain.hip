@@ -597,7 +597,12 @@ int segment_body(const uint8_t* bgr_in, int w, int h, const nsc_seg_params& p, i
 // A failed call may leave work queued on the stream against the workspace: wait for it before
 // returning, so a retry (possibly on another stream) cannot reuse buffers still in use.
 int settle(int rc, hipStream_t s) {
-    if (rc <= NSC_ERR_ARG) (void)hipStreamSynchronize(s);
+    if (rc <= NSC_ERR_ARG) {
+        (void)hipStreamSynchronize(s);
+        // work the chain forked onto this thread's side streams (and never joined on error)
+        for (hipStream_t side : side_stream().side)
+            if (side) (void)hipStreamSynchronize(side);
+    }
     return rc;
 }
 

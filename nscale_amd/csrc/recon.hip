@@ -1807,6 +1807,22 @@ __global__ __launch_bounds__(64 * GX * GY) void geo_drain(const uint4* __restric
     }
 }
 
+// The first pass's list when the only finite distance is one seed pixel (every other pixel starts
+// at GINF, so no other region can change): the seed's region, or an empty list without a seed.
+__global__ void k_geo_seed_list(const int* __restrict__ seedp, int w, int h, int rgx, int gx, int gy, int* list,
+                                int* count, unsigned* stamp) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int sd = *seedp;
+    if (sd >= 0 && (long long)sd < (long long)w * h) {
+        const int r = ((sd / w) / OH / gy) * rgx + (sd % w) / OW / gx;
+        list[0] = r;
+        stamp[r] = 1;  // listed for pass 1, as pass 0 would mark it
+        *count = 1;
+    } else {
+        *count = 0;
+    }
+}
+
 // sets the first n bits of a bitmap (every region dirty)
 __global__ void k_set_bits(unsigned long long* bits, int n) {
     const int nw = (n + 63) / 64;
@@ -1818,7 +1834,7 @@ __global__ void k_set_bits(unsigned long long* bits, int n) {
 
 }  // namespace
 
-int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t s) {
+int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t s, const int* seed) {
     // region shape in windows: 2 x 2, or single windows with NSC_GEO_REGION=1 (4 x 2 and 2 x 4
     // regions were measured slower on the 4096^2 chain tile: 3.2-3.4 ms against 2.7 ms, the extra
     // rounds a region needs to carry its halos across more windows outweigh the fewer passes)
@@ -1877,7 +1893,12 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
             hipLaunchKernelGGL(geo_tiles, dim3(nt), dim3(64), 0, s, flags, dist, w, h, tiles_x, tiles_y, lst, cnt, nl, nc,
                                clr, stamp, p);
     };
-    launch(0, nullptr, nullptr, listA, &counts[0], &counts[1]);
+    if (seed) {
+        // dist holds GINF everywhere but at *seed: pass 0 would change the seed's region only
+        hipLaunchKernelGGL(k_geo_seed_list, dim3(1), dim3(64), 0, s, seed, w, h, rgx, GX, GY, listA, &counts[0], stamp);
+    } else {
+        launch(0, nullptr, nullptr, listA, &counts[0], &counts[1]);
+    }
     long long p = 1;  // next pass to queue
     auto queue_to = [&](long long last) {
         for (; p <= last && p <= max_pass; ++p) {

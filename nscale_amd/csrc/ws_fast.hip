@@ -1568,7 +1568,7 @@ inline dim3 grid1(size_t n) { return dim3(std::min<unsigned>(cdiv(n, 256), 8192u
 void ws_last_flags_record(int flags);
 void ws_tier_record(const int* runs, int ntiers);
 
-int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t s);
+int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t s, const int* seed);
 
 int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, int* ok, hipStream_t s) {
     const size_t N = (size_t)W * H;
@@ -1744,7 +1744,7 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
     {
         unsigned* dist = reinterpret_cast<unsigned*>(head);  // head[] is dead after k_bresolve
         hipLaunchKernelGGL(k_geo_init, grid1(N), dim3(256), 0, s, D, &small[8], fmask, dist);
-        const int grc = geodist_impl(fmask, dist, W, H, s);
+        const int grc = geodist_impl(fmask, dist, W, H, s, &small[8]);  // the seed: b1
         if (grc == NSC_ERR_LIMIT) {  // outside the tile search's range: the sequential flood decides
             clear_error();  // the call still succeeds: no error text left behind
             ws_last_flags_record(FB_CMP);

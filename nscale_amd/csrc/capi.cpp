@@ -53,6 +53,11 @@ CountReader& count_reader() {
 }
 }  // namespace
 
+bool& thread_serving() {
+    static thread_local bool serving = false;
+    return serving;
+}
+
 bool set_thread_pass_bridge(bool on) {
     const bool prev = count_reader().bridge;
     count_reader().bridge = on;

@@ -192,6 +192,10 @@ void set_thread_blocking_sync(bool on);
 // count_read_async queues the copy of *d and returns the host word (valid after count_wait), or
 // nullptr when the bridge is off or unavailable.
 bool set_thread_pass_bridge(bool on);
+// Set on the server's slot threads, whose chains share the GPU with other slots: host round trips
+// cost those threads little there, and kernels sized to their work leave more of the GPU to the
+// other slots (the watershed reads its phase-1 tier counts and launches each tier's grid exactly).
+bool& thread_serving();
 int* count_read_async(const int* d, hipStream_t s);
 hipError_t count_wait();
 

@@ -171,6 +171,7 @@ struct nsc_server {
         (void)hipSetDevice(device);
         nsc::set_thread_blocking_sync(true);
         (void)nsc::set_thread_pass_bridge(false);
+        nsc::thread_serving() = true;
         for (;;) {
             Req* q = nullptr;
             size_t ws_new = 0, stage_new = 0;

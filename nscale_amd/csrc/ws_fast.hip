@@ -1700,20 +1700,35 @@ int cvws_fast(const uint8_t* img, const int32_t* m0, int32_t* m, int W, int H, i
         // beside the many small runs on the caller's stream (the 1024 tier, then the 512 tier: the
         // tiers share the LDS, so the order decides little; measured against the 512 tier after the
         // 2048 tier, this was 0.07 ms shorter on the same box)
+        // grids: the workgroups the device holds at once per tier (idle ones exit at once); on a
+        // serving thread the tier counts are read and each grid holds one workgroup per run at most
+        // (idle workgroups of the large tiers would wait for a CU with their LDS free, beside the
+        // other slots' kernels), and empty tiers launch nothing
+        int tier_runs[kRunTiers];
+        const bool exact = thread_serving();
+        if (exact) {
+            NSC_HIP(hipMemcpyAsync(tier_runs, bcount, sizeof(tier_runs), hipMemcpyDeviceToHost, s));
+            NSC_HIP(stream_sync(s));
+        }
+        auto grid = [&](int b, int resident) { return exact ? std::min(tier_runs[b], resident) : resident; };
         SideStream& ss = side_stream();
         NSC_CHECK(ss.fork(s));
         int* next_run = &small[40];  // per-tier work counters (zeroed with small[])
-        hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(std::min(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(4), bcount + 4,
-                           next_run + 4);
-        hipLaunchKernelGGL(k_phase1, dim3(cdiv(nrun, 64)), dim3(64), 0, ss.side[0], D, box5, nrun, L(5), bcount + 5, qh2, qt);
-        hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(p1_grid<4096>(nrun)), dim3(64), 0, ss.side[1], D, box5, nrun, L(3),
-                           bcount + 3, next_run + 3);
-        hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(p1_grid<2048>(nrun)), dim3(64), 0, ss.side[2], D, box5, nrun, L(2),
-                           bcount + 2, next_run + 2);
-        hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(p1_grid<1024>(nrun)), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1,
-                           next_run + 1);
-        hipLaunchKernelGGL(k_phase1_lds<512>, dim3(p1_grid<512>(nrun)), dim3(64), 0, s, D, box5, nrun, L(0), bcount,
-                           next_run);
+        if (const int g = grid(4, std::min(nrun, 64)))
+            hipLaunchKernelGGL(k_phase1_lds<6400>, dim3(g), dim3(64), 0, ss.side[0], D, box5, nrun, L(4), bcount + 4,
+                               next_run + 4);
+        if (const int g = exact ? (int)cdiv(tier_runs[5], 64) : (int)cdiv(nrun, 64))
+            hipLaunchKernelGGL(k_phase1, dim3(g), dim3(64), 0, ss.side[0], D, box5, nrun, L(5), bcount + 5, qh2, qt);
+        if (const int g = grid(3, p1_grid<4096>(nrun)))
+            hipLaunchKernelGGL(k_phase1_lds<4096>, dim3(g), dim3(64), 0, ss.side[1], D, box5, nrun, L(3), bcount + 3,
+                               next_run + 3);
+        if (const int g = grid(2, p1_grid<2048>(nrun)))
+            hipLaunchKernelGGL(k_phase1_lds<2048>, dim3(g), dim3(64), 0, ss.side[2], D, box5, nrun, L(2), bcount + 2,
+                               next_run + 2);
+        if (const int g = grid(1, p1_grid<1024>(nrun)))
+            hipLaunchKernelGGL(k_phase1_lds<1024>, dim3(g), dim3(64), 0, s, D, box5, nrun, L(1), bcount + 1, next_run + 1);
+        if (const int g = grid(0, p1_grid<512>(nrun)))
+            hipLaunchKernelGGL(k_phase1_lds<512>, dim3(g), dim3(64), 0, s, D, box5, nrun, L(0), bcount, next_run);
         NSC_CHECK(ss.join(s));
     }
     // B pixels with a queue entry: initially queued ones, then those with leaves (the phase-1

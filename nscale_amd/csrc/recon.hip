@@ -904,10 +904,13 @@ template <typename T>
 static int recon_passes(const RImg<T>* imgs_dev, const RImg<T>& one, const int* tile_img, int nt, long long pixels, int conn,
                         int G, const ReconBufs& B, hipStream_t s) {
     const long long max_pass = pass_limit(pixels);
-    const int grid = std::min(nt, 1 << 16);
+    const int grid0 = std::min(nt, 1 << 16);
     unsigned* stamp = B.stamp;
     int* counts = B.counts;
     auto launch = [&](int pass, const int* lst, const int* cnt, int* nl, int* nc, int* clr) {
+        // a serving thread's later passes run on at most 1024 workgroups that loop over the list:
+        // beside the other slots' kernels, a pass's idle workgroups only wait for CUs
+        const int grid = pass == 0 || !thread_serving() ? grid0 : std::min(grid0, 1024);
         switch (G) {
             case 2: launch_conn<T, 2>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl, nc, clr, stamp, pass); break;
             case 3: launch_conn<T, (max_region<T>() >= 3 ? 3 : 2)>(conn, grid, s, imgs_dev, one, tile_img, nt, lst, cnt, nl,
@@ -1884,7 +1887,10 @@ int geodist_impl(const uint8_t* pass, unsigned* dist, int w, int h, hipStream_t 
         // waves gave up with regions still marked: the passes finish from the current distances
         ++g_drain_redo;
     }
-    const unsigned grid = (unsigned)std::min(nt, 1 << 16);
+    // a serving thread's passes (lists of a few tens of regions) run on a small grid that loops
+    // over longer lists: idle region workgroups would each wait for 71 KB of free LDS beside the
+    // other slots' kernels
+    const unsigned grid = (unsigned)std::min(nt, thread_serving() ? 128 : 1 << 16);
     auto launch = [&](int p, const int* lst, const int* cnt, int* nl, int* nc, int* clr) {
         if (GXY == 22)
             hipLaunchKernelGGL((geo_regions<2, 2>), dim3(grid), dim3(256), 0, s, flags, dist, w, h, tiles_x, tiles_y, rgx, rgy,

@@ -21,7 +21,9 @@
 //  * Pass 0 runs every tile once, reading marker and mask (halo = min(marker, mask)) and writing
 //    the output: 3 bytes/px for uint8, the algorithmic minimum.  A tile whose owned border
 //    changed marks the neighbours whose halo it feeds; later passes re-run only marked tiles
-//    (reading out + mask) until no tile is marked.
+//    (reading out + mask) until no tile is marked.  The host reads the last pass's list length
+//    every few passes; outside the server a short bridge of passes is queued behind each read so
+//    the GPU keeps working during the round trip.
 #include <algorithm>
 #include <atomic>
 #include <climits>
@@ -1407,8 +1409,8 @@ namespace {
 constexpr unsigned GINF = 0xFFFFFFFFu;
 // Frozen pixels of a window (window halo, outside the image, not passable) as bits, in both
 // orientations: column state fc (bit y & 31 of fc[y >> 5] = pixel (y, lane)), row state fr (bit
-// x & 31 of fr[x >> 5] = pixel (lane, x)).  A step is min(v, sat(prev + 1) | blk) with blk = all
-// ones on frozen pixels (value kept) and 0 on free ones.
+// x & 31 of fr[x >> 5] = pixel (lane, x)).  blk = all ones on frozen pixels (value kept) and 0 on
+// free ones feeds each step (geo_step).
 struct GeoFlags {
     uint32_t fc[2], fr[2];
     // called at the top of every round: keeps the 124 unpacked flags out of loop-invariant code
